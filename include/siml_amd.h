@@ -1,0 +1,173 @@
+/*
+ * siml_amd.h -- C ABI of libsiml_amd.so, the MI355X (gfx950) LINGO
+ * similarity library.
+ *
+ * Conventions
+ *   - Every function returns 0 (SIML_OK) or a negative SIML_E* code; the
+ *     message of the last failure on the calling thread is siml_last_error().
+ *   - Sizes, offsets and pitches are int64 element counts (not bytes).
+ *   - A "SMILES set" is four int32 arrays in the reference's argument order:
+ *     Lingo matrix, count matrix, MAGNITUDES, LENGTHS (_CPULingo.c:103-105).
+ *     Matrices are row-major (n, width) with row pitch `pitch` >= width;
+ *     row r holds its lengths[r] Lingo ids ascending in the first columns.
+ *   - Inputs are borrowed for the duration of the call; outputs are
+ *     caller-owned host buffers.  Device buffers are owned by the handle.
+ *   - Rows are the reference set (set_refsmiles), columns the query set
+ *     (set_qsmiles), as in GPULingo.getMultipleRows (GPULingo.py:365-405).
+ */
+#ifndef SIML_AMD_H
+#define SIML_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIML_OK 0
+#define SIML_EINVAL (-1)  /* argument error (shape, dtype, range)          */
+#define SIML_ENOMEM (-2)  /* host or device allocation failed               */
+#define SIML_EDEVICE (-3) /* HIP runtime or kernel launch error             */
+#define SIML_ESTATE (-4)  /* set not initialised / no pending result        */
+#define SIML_ELIMIT (-5)  /* a size beyond a documented limit of this build */
+
+/* Result kinds reported by siml_result_info(). */
+#define SIML_RESULT_NONE 0
+#define SIML_RESULT_F32 1   /* float32 (rows, cols) matrix                  */
+#define SIML_RESULT_I32 2   /* int32 (rows, cols) matrix                    */
+#define SIML_RESULT_CSR 3   /* int64 offsets (rows+1) + int32 indices (nnz) */
+#define SIML_RESULT_TOPK 4  /* int32 (rows, k) indices + float32 (rows, k)  */
+#define SIML_RESULT_I64 5   /* int64 vector of `cols` bins                  */
+
+/* Largest number of unique Lingos in one reference row (a row is staged in
+ * LDS).  The reference's limit came from 16 KiB of shared memory
+ * (GPULingo.py:391): 2048 Lingos. */
+#define SIML_MAX_ROW_LINGOS 8192
+
+const char* siml_last_error(void);
+int siml_version(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int siml_device_count(int* count);
+
+/* ---------------------------------------------------------------------- */
+/* SMILES packer.  Replaces _ccompiler.SMILEStoMatrices                    */
+/* (_ccompiler.cpp:276-381; digit state machine :178-264; LingoBag        */
+/* :47-176).  Strings are one byte buffer with n+1 offsets.                */
+/* ---------------------------------------------------------------------- */
+
+/* Pass 1: unique-Lingo count (lengths) and magnitude (mags) per string. */
+int siml_pack_measure(const char* buf, const int64_t* offsets, int64_t n, int32_t* lengths, int32_t* mags,
+                      int nthreads);
+/* Pass 2: zero-padded (n, width) Lingo and count matrices. */
+int siml_pack_fill(const char* buf, const int64_t* offsets, int64_t n, int32_t* lingos, int32_t* counts,
+                   int64_t width, int nthreads);
+/* Ragged form: ids/counts of string r at [row_offsets[r], row_offsets[r+1]).
+ * row_offsets is an exclusive scan of the lengths from siml_pack_measure. */
+int siml_pack_fill_ragged(const char* buf, const int64_t* offsets, int64_t n, const int64_t* row_offsets,
+                          int32_t* ids, int32_t* counts, int nthreads);
+
+/* ---------------------------------------------------------------------- */
+/* Stateless block on the GPU.  Replaces _CPULingo.getTanimotoBlock and    */
+/* getTanimotoBlockParallel (_CPULingo.c:83-274, :277-465): float32       */
+/* (nr, nq) Tanimoto block into a host buffer with row pitch `opitch`.     */
+/* ---------------------------------------------------------------------- */
+int siml_tanimoto_block(int device, const int32_t* rlingos, const int32_t* rcounts, const int32_t* rmags,
+                        const int32_t* rlengths, int64_t nr, int64_t rwidth, int64_t rpitch,
+                        const int32_t* qlingos, const int32_t* qcounts, const int32_t* qmags,
+                        const int32_t* qlengths, int64_t nq, int64_t qwidth, int64_t qpitch, float* out,
+                        int64_t opitch);
+
+/* ---------------------------------------------------------------------- */
+/* Row histograms of a host float matrix on the GPU.  Replaces             */
+/* _cpuutil.rowHistogram101 (_cpuutil.cpp:74-126): bin                     */
+/* min((unsigned)(x*100), 100), uint32 (rows, 101).  The float64 variant   */
+/* multiplies in double (_cpuutil.cpp:92,123).                             */
+/* ---------------------------------------------------------------------- */
+int siml_row_hist101_f32(int device, const float* data, int64_t rows, int64_t cols, int64_t pitch,
+                         uint32_t* hist);
+int siml_row_hist101_f64(int device, const double* data, int64_t rows, int64_t cols, int64_t pitch,
+                         uint32_t* hist);
+
+/* ---------------------------------------------------------------------- */
+/* Comparator handle.  Replaces GPULingo's device glue and launches        */
+/* (_pycudaData, GPULingo.py:55-142; methods :186-648).  One handle owns   */
+/* one HIP stream and at most one pending result (GPULingo.py:537-641).    */
+/* ---------------------------------------------------------------------- */
+typedef struct siml_ctx siml_ctx;
+
+int siml_create(int device, siml_ctx** out);
+int siml_destroy(siml_ctx* ctx);
+
+/* set_refsmiles / set_qsmiles (GPULingo.py:186-314).  mags may be NULL:
+ * the magnitudes are then the row sums of the counts, computed on the
+ * device (accumulateRefMagnitudes, kernels.py:45-79). */
+int siml_set_ref(siml_ctx* ctx, const int32_t* lingos, const int32_t* counts, const int32_t* mags,
+                 const int32_t* lengths, int64_t n, int64_t width, int64_t pitch);
+int siml_set_query(siml_ctx* ctx, const int32_t* lingos, const int32_t* counts, const int32_t* mags,
+                   const int32_t* lengths, int64_t n, int64_t width, int64_t pitch);
+/* Ragged variants (no counterpart in the reference): the ids/counts of
+ * molecule r are at [offsets[r], offsets[r+1]); n+1 offsets. */
+int siml_set_ref_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* counts, const int32_t* mags,
+                        const int64_t* offsets, int64_t n);
+int siml_set_query_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* counts, const int32_t* mags,
+                          const int64_t* offsets, int64_t n);
+/* Sizes of the current sets (-1 when unset). */
+int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery);
+
+/* gpuLingoSim3_multirow via getMultipleRows_async (GPULingo.py:365-405,
+ * kernels.py:315-352): float32 (rowlimit-rowbase, nquery). */
+int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
+/* gpuLingoSim3_srgf_T_tex2ns via getTanimotoRow_async (GPULingo.py:327-352,
+ * kernels.py:234-270, :122-159): float32 (1, nquery); magnitudes are
+ * recomputed from the counts and the zero test is on lengths. */
+int siml_single_row_async(siml_ctx* ctx, int64_t row);
+/* gpuHistogrammedLingo_multirow via getMultipleHistogrammedRows_async
+ * (GPULingo.py:419-464, kernels.py:606-667): int32 (rows, 101). */
+int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
+/* gpuGetNeighbors_multirow via getNeighbors_async (GPULingo.py:466-514,
+ * kernels.py:393-475): for each row, every query column with
+ * fl32(min(ma,mb)/(float)max(ma,mb)) >= lo and lo <= T < hi, ascending.
+ * The result is complete CSR (the reference truncates at its pitch). */
+int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float lo, float hi);
+/* Top-k per row (no counterpart in the reference): larger T first, equal T
+ * ordered by smaller column index; rows with fewer than k columns padded
+ * with index -1 and T = -inf.  1 <= k <= SIML_MAX_TOPK. */
+#define SIML_MAX_TOPK 256
+int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k);
+/* Global 1000-bin histogram (no counterpart in the reference) of every T in
+ * the block, float32 edges fl32(k/1000), bin = max{k : T >= edge_k} clipped
+ * to 999; exact int64 counts. */
+int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
+
+/* Pending-result protocol (GPULingo.py:537-648). */
+int siml_query_done(siml_ctx* ctx, int* done);   /* asyncOperationsDone */
+int siml_synchronize(siml_ctx* ctx);
+int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int64_t* nnz);
+/* Copy the pending result to host buffers and clear it.  Matrix results use
+ * the host row pitch `pitch`; CSR fills offsets (rows+1) and indices (nnz);
+ * top-k fills idx and t with pitch k; I64 fills `cols` values. */
+int siml_retrieve_f32(siml_ctx* ctx, float* host, int64_t pitch);
+int siml_retrieve_i32(siml_ctx* ctx, int32_t* host, int64_t pitch);
+int siml_retrieve_csr(siml_ctx* ctx, int64_t* offsets, int32_t* indices);
+int siml_retrieve_topk(siml_ctx* ctx, int32_t* idx, float* t);
+int siml_retrieve_i64(siml_ctx* ctx, int64_t* host);
+/* Copy rows [r0, r1) of a pending F32 result without clearing it. */
+int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64_t pitch);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement (no counterpart in the reference).                          */
+/* ---------------------------------------------------------------------- */
+/* When on, every compute kernel launch is bracketed by HIP events on the
+ * handle's stream; siml_kernel_stats returns the summed kernel time (ms),
+ * the number of launches and the number of pairs those launches evaluated
+ * since the last reset. */
+int siml_set_timing(siml_ctx* ctx, int on);
+int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_t* pairs, int reset);
+/* Name of the kernel variant the last similarity launch used. */
+const char* siml_last_kernel(siml_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIML_AMD_H */

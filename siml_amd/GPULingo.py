@@ -1,0 +1,265 @@
+"""GPULingo: LINGO similarities on an MI355X (drop-in for pysiml.GPULingo).
+
+Method names, argument meaning, return shapes and dtypes follow
+GPULingo.py:158-648 of the reference; computation runs in the HIP kernels of
+libsiml_amd.so.  Rows are the reference set (set_refsmiles), columns the
+query set (set_qsmiles).
+
+Deliberate differences from the reference, all on paths where it fails or is
+undefined:
+  * argument errors raise ValueError (the reference raises a bare ``raise``
+    or the undefined ``NotImplementedException``);
+  * there are no 32K/64K set-size limits (GPULingo.py:206-207, 269-270);
+  * neighbour lists are computed completely; getNeighbors reports the
+    reference's overflow count (kernels.py:466-469) but its matrix never
+    exceeds the padded pitch, and unused entries are -1 rather than
+    uninitialised memory.
+Additive methods: getTopK, getGlobalHistogram, getNeighborsCSR.
+"""
+import numpy as np
+
+from . import _lib
+
+
+def _check_int32_matrix(a, name):
+    a = np.asarray(a)
+    if a.dtype != np.int32:
+        raise ValueError(f"{name} was not of type int32")
+    if a.ndim != 2:
+        raise ValueError(f"{name} did not have dimension 2")
+    return np.ascontiguousarray(a)
+
+
+def _check_vector(v, n, name):
+    v = np.asarray(v)
+    if v.dtype != np.int32:
+        raise ValueError(f"{name} was not of type int32")
+    if not (v.ndim == 1 or (v.ndim == 2 and v.shape[1] == 1)):
+        raise ValueError(f"{name} was not 1-D")
+    if v.shape[0] != n:
+        raise ValueError(f"{name} length did not equal number of rows of the Lingo matrix")
+    return np.ascontiguousarray(v.reshape(-1))
+
+
+def normalise_set(lingos, counts, lengths, mags, label):
+    """Validate a SMILES set the way _CPULingo.c:139-215 does and return
+    contiguous int32 arrays (mags may be None)."""
+    lingos = _check_int32_matrix(lingos, f"{label} Lingo matrix")
+    counts = _check_int32_matrix(counts, f"{label} Lingo count matrix")
+    if lingos.shape != counts.shape:
+        raise ValueError(f"{label} Lingo and Lingo count matrix did not have identical shapes")
+    n = lingos.shape[0]
+    lengths = _check_vector(lengths, n, f"{label} length vector")
+    if mags is not None:
+        mags = _check_vector(mags, n, f"{label} magnitude vector")
+    return lingos, counts, lengths, mags
+
+
+def reference_neighbor_counts(offsets, indices, pitch):
+    """Per-row counts as gpuGetNeighbors_multirow reports them
+    (kernels.py:433-474): it scans the columns in waves of 128, adds each
+    wave's neighbours and stops after the first wave that takes the count
+    past `pitch`, so an overflowing row reports the number of neighbours
+    with column < 128 * (wave of neighbour `pitch` + 1)."""
+    rows = len(offsets) - 1
+    out = np.empty(rows, dtype=np.int32)
+    for r in range(rows):
+        lst = indices[offsets[r]:offsets[r + 1]]
+        c = len(lst)
+        if c <= pitch:
+            out[r] = c
+        else:
+            limit = 128 * (int(lst[pitch]) // 128 + 1)
+            out[r] = int(np.searchsorted(lst, limit, side="left"))
+    return out
+
+
+class GPULingo:
+    """Object to handle computation of LINGO similarities on the GPU with
+    HIP device ID *deviceID* (GPULingo.py:158-174)."""
+
+    def __init__(self, deviceID=0):
+        self.deviceID = deviceID
+        self.nref = 0
+        self.nquery = 0
+        self.rlengths = None
+        self.qshape = None
+        self._h = _lib.Handle(deviceID)
+        self._retrieve = None
+
+    # -- set-up ------------------------------------------------------------
+    def set_refsmiles(self, refsmilesmat, refcountsmat, reflengths, refmags=None):
+        """GPULingo.py:186-243.  Without *refmags* the magnitudes are the
+        row sums of the count matrix (accumulateRefMagnitudes,
+        kernels.py:45-61)."""
+        l, c, n, m = normalise_set(refsmilesmat, refcountsmat, reflengths, refmags, "Reference")
+        self._h.set_padded("ref", l, c, n, m)
+        self.rlengths = n
+        self.rshape = l.shape
+        self.nref = l.shape[0]
+
+    def set_qsmiles(self, qsmilesmat, qcountsmat, querylengths, querymags=None):
+        """GPULingo.py:245-314 (magnitudes: accumulateQueryMagnitudes,
+        kernels.py:63-79)."""
+        l, c, n, m = normalise_set(qsmilesmat, qcountsmat, querylengths, querymags, "Query")
+        self._h.set_padded("query", l, c, n, m)
+        self.qshape = l.shape
+        self.nquery = l.shape[0]
+
+    def set_refsmiles_ragged(self, ids, counts, offsets, mags=None):
+        """Ragged reference set (no counterpart in the reference)."""
+        ids, counts, offsets, mags = _ragged(ids, counts, offsets, mags)
+        self._h.set_ragged("ref", ids, counts, offsets, mags)
+        self.nref = len(offsets) - 1
+        self.rlengths = np.diff(offsets).astype(np.int32)
+
+    def set_qsmiles_ragged(self, ids, counts, offsets, mags=None):
+        """Ragged query set (no counterpart in the reference)."""
+        ids, counts, offsets, mags = _ragged(ids, counts, offsets, mags)
+        self._h.set_ragged("query", ids, counts, offsets, mags)
+        self.nquery = len(offsets) - 1
+        self.qshape = (self.nquery, 0)
+
+    # -- helpers -----------------------------------------------------------
+    def _rows(self, rowbase, rowlimit):
+        rowbase, rowlimit = int(rowbase), int(rowlimit)
+        if rowbase < 0 or rowlimit > self.nref or rowlimit < rowbase:
+            raise ValueError(f"rows [{rowbase}, {rowlimit}) outside [0, {self.nref})")
+        return rowbase, rowlimit
+
+    def _pend(self, fn):
+        self._retrieve = fn
+
+    # -- similarity rows -----------------------------------------------------
+    def getTanimotoRow(self, rowidx):
+        """Single Tanimoto row, synchronous (GPULingo.py:316-325)."""
+        self.getTanimotoRow_async(rowidx)
+        return self.retrieveAsyncResult()
+
+    def getTanimotoRow_async(self, rowidx):
+        """GPULingo.py:327-352 / gpuLingoSim3_srgf_T_tex2ns: magnitudes are
+        recomputed from the counts and the zero test is on lengths."""
+        rowidx = int(rowidx)
+        if rowidx < 0 or rowidx >= self.nref:
+            raise ValueError(f"row {rowidx} outside [0, {self.nref})")
+        self._h.call("siml_single_row_async", rowidx)
+        self._pend(self._retrieve_matrix)
+
+    def getMultipleRows(self, rowbase, rowlimit):
+        """GPULingo.py:354-363."""
+        self.getMultipleRows_async(rowbase, rowlimit)
+        return self.retrieveAsyncResult()
+
+    def getMultipleRows_async(self, rowbase, rowlimit):
+        """GPULingo.py:365-405 / gpuLingoSim3_multirow."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        self._h.call("siml_multirow_async", rowbase, rowlimit)
+        self._pend(self._retrieve_matrix)
+
+    def getMultipleHistogrammedRows(self, rowbase, rowlimit):
+        """GPULingo.py:407-417."""
+        self.getMultipleHistogrammedRows_async(rowbase, rowlimit)
+        return self.retrieveAsyncResult()
+
+    def getMultipleHistogrammedRows_async(self, rowbase, rowlimit):
+        """GPULingo.py:419-464 / gpuHistogrammedLingo_multirow: int32
+        (rows, 101), bin (unsigned)(T*100)."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        self._h.call("siml_hist_rows_async", rowbase, rowlimit)
+        self._pend(self._retrieve_hist)
+
+    def getNeighbors(self, rowbase, rowlimit, lowerbound, upperbound=1.1, maxneighbors=None):
+        """GPULingo.py:516-535."""
+        self.getNeighbors_async(rowbase, rowlimit, lowerbound, upperbound, maxneighbors)
+        return self.retrieveAsyncResult()
+
+    def getNeighbors_async(self, rowbase, rowlimit, lowerbound, upperbound=1.1, maxneighbors=None):
+        """GPULingo.py:466-514 / gpuGetNeighbors_multirow: (matrix, counts)
+        with ascending query indices, lowerbound <= T < upperbound (float32
+        bounds)."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        if maxneighbors is None:
+            maxneighbors = self.nquery
+        pitch = 16 * ((int(maxneighbors) + 15) // 16)
+        self._h.call("siml_neighbors_async", rowbase, rowlimit, float(np.float32(lowerbound)),
+                     float(np.float32(upperbound)))
+        self._pend(lambda: self._retrieve_neighbors(pitch))
+
+    # -- additive -------------------------------------------------------------
+    def getNeighborsCSR(self, rowbase, rowlimit, lowerbound, upperbound=1.1):
+        """Complete neighbour lists as CSR: (offsets int64 (rows+1,),
+        indices int32 (nnz,)), ascending within each row."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        self._h.call("siml_neighbors_async", rowbase, rowlimit, float(np.float32(lowerbound)),
+                     float(np.float32(upperbound)))
+        self._retrieve = None
+        return self._h.retrieve_csr()
+
+    def getTopK(self, rowbase, rowlimit, k):
+        """Top-k query columns per row: (indices int32 (rows,k), T float32
+        (rows,k)); larger T first, ties by smaller index."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        self._h.call("siml_topk_async", rowbase, rowlimit, int(k))
+        self._retrieve = None
+        return self._h.retrieve_topk()
+
+    def getGlobalHistogram(self, rowbase, rowlimit):
+        """int64 counts of all T of the block in 1000 bins with float32 edges
+        fl32(k/1000)."""
+        rowbase, rowlimit = self._rows(rowbase, rowlimit)
+        self._h.call("siml_hist1000_async", rowbase, rowlimit)
+        self._retrieve = None
+        return self._h.retrieve_i64()
+
+    # -- async protocol -------------------------------------------------------
+    def _retrieve_matrix(self):
+        kind, rows, cols, _ = self._h.result_info()
+        pitch = 16 * ((cols + 15) // 16)
+        out = np.empty((rows, pitch), dtype=np.float32)
+        self._h.retrieve_f32(out, pitch)
+        return out[:, :cols]
+
+    def _retrieve_hist(self):
+        kind, rows, cols, _ = self._h.result_info()
+        out = np.empty((rows, 112), dtype=np.int32)
+        self._h.retrieve_i32(out, 112)
+        return out[:, :101]
+
+    def _retrieve_neighbors(self, pitch):
+        offs, idx = self._h.retrieve_csr()
+        counts = reference_neighbor_counts(offs, idx, pitch)
+        rows = len(offs) - 1
+        width = int(min(counts.max(), pitch)) if rows else 0
+        mat = np.full((rows, width), -1, dtype=np.int32)
+        for r in range(rows):
+            lst = idx[offs[r]:offs[r + 1]][:width]
+            mat[r, :len(lst)] = lst
+        return mat, counts
+
+    def retrieveAsyncResult(self):
+        """Result of the last asynchronous call (GPULingo.py:537-641); blocks
+        until it is complete.  Valid once per asynchronous call."""
+        fn, self._retrieve = self._retrieve, None
+        if fn is None:
+            raise RuntimeError("Error: tried to call retrieveAsyncResult without a pending asynchronous result")
+        return fn()
+
+    def asyncOperationsDone(self):
+        """True once all work queued on this object has completed
+        (GPULingo.py:643-647)."""
+        return self._h.done()
+
+
+def _ragged(ids, counts, offsets, mags):
+    ids = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
+    counts = np.ascontiguousarray(counts, dtype=np.int32).reshape(-1)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64).reshape(-1)
+    if ids.shape != counts.shape:
+        raise ValueError("ids and counts differ in length")
+    if offsets.size == 0 or offsets[-1] > ids.size:
+        raise ValueError("offsets do not fit the id array")
+    if mags is not None:
+        mags = np.ascontiguousarray(mags, dtype=np.int32).reshape(-1)
+        if mags.size != offsets.size - 1:
+            raise ValueError("magnitude vector length does not match offsets")
+    return ids, counts, offsets, mags

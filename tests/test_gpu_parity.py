@@ -1,0 +1,257 @@
+"""Bit-exact parity of the HIP path with the CPU oracle.
+
+All comparisons are on the float32 bit patterns (view as uint32) or on
+integer arrays: the bar is identical intersection counts, identical float32
+similarities, identical neighbour sets, histogram counts and top-k lists.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from siml_amd import _lib, synth
+from siml_amd.CPULingo import CPULingo
+from siml_amd.GPULingo import GPULingo, reference_neighbor_counts
+from siml_amd.utilities import rowHistogram101
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def assert_same_f32(got, want):
+    assert got.shape == want.shape
+    bad = np.count_nonzero(bits(got) != bits(want))
+    assert bad == 0, f"{bad} of {want.size} float32 values differ"
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return GPULingo(0)
+
+
+@pytest.fixture(scope="module")
+def demo_set(demo_lines):
+    return oracle.pack(demo_lines)
+
+
+def _set(g, ref, qry):
+    g.set_refsmiles(*ref)
+    g.set_qsmiles(*qry)
+
+
+def test_multirow_demo4096_full(gpu, demo_set):
+    """examples/benchmark.py:123-192: N^2 matrix in 256-row blocks, GPU == CPU."""
+    _set(gpu, demo_set, demo_set)
+    want = oracle.tanimoto_block(demo_set, demo_set, nthreads=0)
+    got = np.empty_like(want)
+    for base in range(0, 4096, 256):
+        got[base:base + 256] = gpu.getMultipleRows(base, min(base + 256, 4096))
+    assert_same_f32(got, want)
+
+
+def test_multirow_all_pairs_4096(gpu):
+    s = synth.packed_padded(4096, 20, 80, seed=1)
+    _set(gpu, s, s)
+    got = gpu.getMultipleRows(0, 4096)
+    assert_same_f32(got, oracle.tanimoto_block(s, s, nthreads=0))
+
+
+def test_multirow_all_pairs_32768_sampled(gpu):
+    """32768 x 32768 full matrix on the device; sampled rows checked exactly,
+    diagonal == 1 and symmetry on the sampled block."""
+    s = synth.packed_padded(32768, 20, 120, seed=2)
+    _set(gpu, s, s)
+    gpu.getMultipleRows_async(0, 32768)
+    rng = np.random.default_rng(0)
+    rows = np.unique(np.concatenate([[0, 32767], rng.choice(32768, 64, replace=False)]))
+    got = np.empty((len(rows), 32768), np.float32)
+    for i, r in enumerate(rows):
+        gpu._h.call("siml_peek_rows_f32", int(r), int(r) + 1, _lib.f32p(got[i:i + 1]), 32768)
+    sub = tuple(x[rows] for x in s)
+    assert_same_f32(got, oracle.tanimoto_block(sub, s, nthreads=0))
+    assert np.all(got[np.arange(len(rows)), rows] == np.float32(1.0))
+    blk = got[:, rows]
+    assert np.array_equal(bits(blk), bits(blk.T))
+    gpu._retrieve = None  # leave the 4 GiB result on the device
+
+
+def test_single_row_semantics(gpu, demo_lines):
+    ref = list(oracle.pack(demo_lines[:50]))
+    qry = list(oracle.pack(demo_lines[1000:3000]))
+    ref_bad = (ref[0], ref[1], ref[2], ref[3] + 3)
+    qry_bad = (qry[0], qry[1], qry[2], qry[3] * 2)
+    _set(gpu, ref_bad, qry_bad)
+    for r in (0, 17, 49):
+        assert_same_f32(gpu.getTanimotoRow(r)[0], oracle.single_row(ref_bad, r, qry_bad))
+
+
+def test_hist_rows(gpu, demo_set, demo_lines):
+    ref = oracle.pack(demo_lines[:300])
+    _set(gpu, ref, demo_set)
+    got = gpu.getMultipleHistogrammedRows(0, 300)
+    assert got.shape == (300, 101) and got.dtype == np.int32
+    assert np.array_equal(got, oracle.hist_rows101(ref, demo_set))
+    assert np.all(got.sum(1) == 4096)
+
+
+@pytest.mark.parametrize("lo,hi", [(0.0, 1.1), (0.25, 1.1), (0.5, 0.8), (0.7, 1.1), (1.0, 1.1)])
+def test_neighbors_csr(gpu, demo_set, demo_lines, lo, hi):
+    ref = oracle.pack(demo_lines[:200])
+    _set(gpu, ref, demo_set)
+    offs, idx = gpu.getNeighborsCSR(0, 200, lo, hi)
+    lists, counts = oracle.neighbors(ref, demo_set, lo, hi)
+    assert np.array_equal(np.diff(offs), counts)
+    want = np.concatenate(lists) if len(lists) else np.zeros(0, np.int32)
+    assert np.array_equal(idx, want)
+
+
+@pytest.mark.parametrize("maxn", [None, 8, 40])
+def test_get_neighbors_reference_shape(gpu, demo_set, demo_lines, maxn):
+    ref = oracle.pack(demo_lines[:64])
+    _set(gpu, ref, demo_set)
+    mat, cnt = gpu.getNeighbors(0, 64, 0.2, maxneighbors=maxn)
+    lists, full = oracle.neighbors(ref, demo_set, 0.2)
+    pitch = 16 * (((4096 if maxn is None else maxn) + 15) // 16)
+    offs = np.zeros(65, np.int64)
+    offs[1:] = np.cumsum(full)
+    want_cnt = reference_neighbor_counts(offs, np.concatenate(lists).astype(np.int32), pitch)
+    assert cnt.dtype == np.int32 and np.array_equal(cnt, want_cnt)
+    for r in range(64):
+        k = min(int(cnt[r]), mat.shape[1])
+        assert np.array_equal(mat[r, :k], lists[r][:k])
+
+
+def test_row_histogram101(gpu):
+    rng = np.random.default_rng(3)
+    x = rng.random((37, 5000), dtype=np.float32)
+    x[0, :101] = np.arange(101, dtype=np.float32) / np.float32(100)
+    x[1, :4] = [-0.5, np.nan, 2.0, 1.0]
+    assert np.array_equal(rowHistogram101(x), oracle.row_hist101(x))
+    x64 = x.astype(np.float64)
+    assert np.array_equal(rowHistogram101(x64), oracle.row_hist101(x64))
+
+
+def test_division_exhaustive(gpu):
+    """All (I, U) with 0 <= I <= U <= 600: device T == numpy float32 I/U.
+    The row is one Lingo with count 600 and magnitude 1; column (I, U) is
+    that Lingo with count I (or another Lingo when I = 0) and magnitude
+    U + I - 1, so isct = I and the union is exactly U."""
+    Is, Us = [], []
+    for I in range(601):
+        U = np.arange(max(I, 1), 601)
+        Is.append(np.full(len(U), I))
+        Us.append(U)
+    I = np.concatenate(Is)
+    U = np.concatenate(Us)
+    n = len(I)
+    ref = (np.zeros((1, 1), np.int32), np.array([[600]], np.int32), np.ones((1, 1), np.int32),
+           np.ones((1, 1), np.int32))
+    q_l = np.where(I == 0, 1, 0).astype(np.int32).reshape(-1, 1)
+    q_c = np.where(I == 0, 1, I).astype(np.int32).reshape(-1, 1)
+    qry = (q_l, q_c, np.ones((n, 1), np.int32), (U + I - 1).astype(np.int32).reshape(-1, 1))
+    _set(gpu, ref, qry)
+    got = gpu.getMultipleRows(0, 1)[0]
+    assert_same_f32(got, I.astype(np.float32) / U.astype(np.float32))
+    assert_same_f32(got, oracle.tanimoto_block(ref, qry)[0])
+
+
+def test_cpulingo_dropin(demo_set, demo_lines):
+    c = CPULingo()
+    ref = oracle.pack(demo_lines[:500])
+    c.set_refsmiles(*ref)
+    c.set_qsmiles(*demo_set)
+    want = oracle.tanimoto_block(ref, demo_set, nthreads=0)
+    assert_same_f32(c.getMultipleRows(0, 500, nprocs=4), want)
+    assert_same_f32(c.getMultipleRows(-10, 1000), want[-10:])      # Python slicing
+    assert c.getMultipleRows(400, 100).shape == (0, 4096)
+    assert_same_f32(c.getTanimotoRow(7), want[7:8])
+    assert c.retrieveAsyncResult() is c.retrieveAsyncResult()
+    # default magnitudes = lengths.sum(1) (CPULingo.py:82,98)
+    c2 = CPULingo()
+    c2.set_refsmiles(ref[0], ref[1], ref[2])
+    c2.set_qsmiles(demo_set[0], demo_set[1], demo_set[2])
+    r2 = (ref[0], ref[1], ref[2], ref[2].sum(1).astype(np.int32))
+    q2 = (demo_set[0], demo_set[1], demo_set[2], demo_set[2].sum(1).astype(np.int32))
+    assert_same_f32(c2.getMultipleRows(0, 100), oracle.tanimoto_block(tuple(x[:100] for x in r2), q2, 0))
+
+
+def test_stateless_block(demo_lines):
+    import ctypes
+    from siml_amd import _lib
+    ref = oracle.pack(demo_lines[:100])
+    qry = oracle.pack(demo_lines[100:900])
+    out = np.empty((100, 800), np.float32)
+    L = _lib.load()
+    p = _lib.i32p
+    _lib.check(L.siml_tanimoto_block(0, p(ref[0]), p(ref[1]), p(ref[3]), p(ref[2]), 100, ref[0].shape[1],
+                                     ref[0].shape[1], p(qry[0]), p(qry[1]), p(qry[3]), p(qry[2]), 800,
+                                     qry[0].shape[1], qry[0].shape[1],
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 800))
+    assert_same_f32(out, oracle.tanimoto_block(ref, qry))
+
+
+def test_async_protocol(gpu, demo_set):
+    _set(gpu, demo_set, demo_set)
+    gpu.getMultipleRows_async(0, 64)
+    while not gpu.asyncOperationsDone():
+        pass
+    a = gpu.retrieveAsyncResult()
+    assert a.shape == (64, 4096)
+    assert a.base is not None and a.base.shape[1] % 16 == 0  # padded pitch view
+    with pytest.raises(RuntimeError):
+        gpu.retrieveAsyncResult()
+    with pytest.raises(ValueError):
+        gpu.getMultipleRows_async(10, 5000)
+
+
+def test_unsorted_and_duplicate_lists(gpu):
+    """The j-outer merge equals the three-way merge state for state, so even
+    unsorted or duplicated Lingo lists give the oracle's values."""
+    rng = np.random.default_rng(7)
+    n, w = 300, 24
+    L = rng.integers(-20, 20, size=(n, w)).astype(np.int32)
+    C = rng.integers(1, 4, size=(n, w)).astype(np.int32)
+    lens = rng.integers(0, w + 1, size=(n, 1)).astype(np.int32)
+    mags = rng.integers(0, 60, size=(n, 1)).astype(np.int32)
+    s = (L, C, lens, mags)
+    _set(gpu, s, s)
+    assert_same_f32(gpu.getMultipleRows(0, n), oracle.tanimoto_block(s, s))
+
+
+def test_edge_sizes(gpu, demo_set, demo_lines):
+    ref = oracle.pack(demo_lines[:3])
+    empty = tuple(x[:0] for x in demo_set)
+    _set(gpu, ref, empty)
+    assert gpu.getMultipleRows(0, 3).shape == (3, 0)
+    _set(gpu, ref, demo_set)
+    assert gpu.getMultipleRows(1, 1).shape == (0, 4096)
+    # a ragged column count (not a multiple of 64) and zero-length molecules
+    qry = oracle.pack([b"", b"C", b"CCCC"] + demo_lines[:61])
+    _set(gpu, ref, qry)
+    assert_same_f32(gpu.getMultipleRows(0, 3), oracle.tanimoto_block(ref, qry))
+
+
+def test_row_length_limit(gpu):
+    from siml_amd import _lib
+    w = _lib.MAX_ROW_LINGOS
+    big = (np.arange(w, dtype=np.int32).reshape(1, -1), np.ones((1, w), np.int32),
+           np.array([[w]], np.int32), np.array([[w]], np.int32))
+    qry = (np.arange(0, 2 * w, 2, dtype=np.int32).reshape(1, -1), np.ones((1, w), np.int32),
+           np.array([[w]], np.int32), np.array([[w]], np.int32))
+    _set(gpu, big, qry)
+    assert_same_f32(gpu.getMultipleRows(0, 1), oracle.tanimoto_block(big, qry))
+    over = (np.arange(w + 1, dtype=np.int32).reshape(1, -1), np.ones((1, w + 1), np.int32),
+            np.array([[w + 1]], np.int32), np.array([[w + 1]], np.int32))
+    with pytest.raises(ValueError, match="limit"):
+        gpu.set_refsmiles(*over)
+
+
+def test_ragged_sets_equal_padded(gpu):
+    ids, cnt, offs, mags = synth.packed_ragged(2000, 8, 256, seed=21, dist="lognormal")
+    pad = synth.ragged_to_padded(ids, cnt, offs, mags)
+    gpu.set_refsmiles_ragged(ids[:offs[300]], cnt[:offs[300]], offs[:301], mags[:300])
+    gpu.set_qsmiles_ragged(ids, cnt, offs, mags)
+    got = gpu.getMultipleRows(0, 300)
+    assert_same_f32(got, oracle.tanimoto_block(tuple(x[:300] for x in pad), pad, 0))
