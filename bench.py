@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--n", type=int, default=N_ALLPAIRS, help="molecules (rows = columns)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
+                    help="auto: hashed multi-row kernels; merge: one-row-per-workgroup merge kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -76,6 +78,8 @@ def main():
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
     h = g._h
+    from siml_amd import _lib
+    h.call("siml_set_policy", _lib.POLICY_MERGE if args.policy == "merge" else _lib.POLICY_AUTO)
 
     for i in range(args.warmup):
         h.call("siml_multirow_async", 0, n)
@@ -113,7 +117,7 @@ def main():
         "dtype": "int32 merge, float32 division",
         "data": "synthetic (seeded SMILES-like strings, packed with the native packer)",
         "config": {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
-                   "rows": n, "columns": n, "seed": 2},
+                   "rows": n, "columns": n, "seed": 2, "policy": args.policy},
         "kernel": kernel,
         "kernel_ms_mean": kernel_s * 1e3,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": P_VALU / 1e12,

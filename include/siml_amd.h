@@ -99,8 +99,9 @@ int siml_create(int device, siml_ctx** out);
 int siml_destroy(siml_ctx* ctx);
 
 /* set_refsmiles / set_qsmiles (GPULingo.py:186-314).  mags may be NULL:
- * the magnitudes are then the row sums of the counts, computed on the
- * device (accumulateRefMagnitudes, kernels.py:45-79). */
+ * the magnitudes are then the row sums of the counts, the values
+ * accumulateRefMagnitudes / accumulateQueryMagnitudes produce
+ * (kernels.py:45-79). */
 int siml_set_ref(siml_ctx* ctx, const int32_t* lingos, const int32_t* counts, const int32_t* mags,
                  const int32_t* lengths, int64_t n, int64_t width, int64_t pitch);
 int siml_set_query(siml_ctx* ctx, const int32_t* lingos, const int32_t* counts, const int32_t* mags,
@@ -163,6 +164,13 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
  * since the last reset. */
 int siml_set_timing(siml_ctx* ctx, int on);
 int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_t* pairs, int reset);
+/* Kernel policy: AUTO serves sorted-unique sets with the hashed multi-row
+ * kernels (rows that do not qualify, and unsorted sets, use the merge
+ * kernel); MERGE uses the merge kernel for everything.  Both give the same
+ * bits. */
+#define SIML_POLICY_AUTO 0
+#define SIML_POLICY_MERGE 1
+int siml_set_policy(siml_ctx* ctx, int policy);
 /* Name of the kernel variant the last similarity launch used. */
 const char* siml_last_kernel(siml_ctx* ctx);
 

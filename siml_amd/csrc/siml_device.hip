@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "groups.h"
 #include "siml_internal.h"
 
 namespace siml {
@@ -79,6 +80,7 @@ struct RowArgs {
 struct ColArgs {
     const int2* tiles;
     const int64_t* toff;
+    const int32_t* tlen;
     const int32_t* len;
     const int32_t* mag;
     const int32_t* csum;
@@ -164,10 +166,10 @@ __device__ __forceinline__ int bin101_f64(double t) {
 // loads from the tile layout.  RECOMPUTE selects the single-row semantics of
 // gpuLingoSim3_srgf_T_tex2ns (kernels.py:234-270).
 template <bool RECOMPUTE>
-__global__ __launch_bounds__(kBlock) void k_rows_f32(RowArgs R, ColArgs C, int64_t rowbase, float* __restrict__ out,
-                                                     int64_t opitch) {
+__global__ __launch_bounds__(kBlock) void k_rows_f32(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
+                                                     int64_t rowbase, float* __restrict__ out, int64_t opitch) {
     extern __shared__ int2 lds_row[];
-    const int64_t row = rowbase + blockIdx.x;
+    const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
     const int La = R.len[row];
     const int ma = RECOMPUTE ? R.csum[row] : R.mag[row];
     stage_row(lds_row, R.elems + R.off[row], La);
@@ -175,13 +177,111 @@ __global__ __launch_bounds__(kBlock) void k_rows_f32(RowArgs R, ColArgs C, int64
     int64_t t0, t1, gw, nw;
     wave_tiles(C.ntiles, t0, t1, gw, nw);
     const int lane = lane_id();
-    float* orow = out + (int64_t)blockIdx.x * opitch;
+    float* orow = out + (row - rowbase) * opitch;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t c = t * kWave + lane;
         if (c >= C.n) continue;
         const int Lb = C.len[c];
         const int isct = merge_isct(lds_row, La, C.tiles + C.toff[t] + lane, Lb);
         orow[c] = RECOMPUTE ? tanimoto_recomputed(isct, La, Lb, ma, C.csum[c]) : tanimoto_mags(isct, ma, C.mag[c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hashed multi-row kernels (row groups, see groups.h)
+// ---------------------------------------------------------------------------
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+struct GroupArgs {
+    const siml::GroupHdr* hdr;
+    const int2* ekey;   // (id, slot) per entry
+    const uint4* ecnt;  // eight u16 counts per entry
+};
+
+// acc += min(cv, cb2) on two u16 lanes (v_pk_min_u16 + v_pk_add_u16).
+__device__ __forceinline__ uint32_t pk_minadd(uint32_t acc, uint32_t cv, uint32_t cb2) {
+    us2 a = __builtin_bit_cast(us2, acc);
+    a += __builtin_elementwise_min(__builtin_bit_cast(us2, cv), __builtin_bit_cast(us2, cb2));
+    return __builtin_bit_cast(uint32_t, a);
+}
+
+struct GroupLds {
+    int32_t keys[siml::kSlots];
+    uint4 cnt[siml::kSlots];
+};
+
+// Fills the group's cuckoo table into LDS: empty slots hold empty_key with
+// all-zero counts (slot 0 always stays empty).
+__device__ __forceinline__ void stage_group(GroupLds& T, const GroupArgs& G, const siml::GroupHdr& hd) {
+    for (int s = threadIdx.x; s < siml::kSlots; s += blockDim.x) {
+        T.keys[s] = hd.empty_key;
+        T.cnt[s] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
+        const int2 e = G.ekey[hd.entry_off + k];
+        T.keys[e.y] = e.x;
+        T.cnt[e.y] = G.ecnt[hd.entry_off + k];
+    }
+    __syncthreads();
+}
+
+// Intersections of one query molecule (a lane) with all rows of the group:
+// acc[w] holds rows 2w (low u16) and 2w+1 (high u16).  The loop runs over
+// the tile's uniform step count; padding entries carry count 0 and add 0.
+__device__ __forceinline__ void group_isct(const GroupLds& T, const siml::GroupHdr& hd, const int2* __restrict__ p,
+                                           int Lt, uint32_t acc[4]) {
+    acc[0] = acc[1] = acc[2] = acc[3] = 0u;
+    const uint32_t a1 = hd.a1, a2 = hd.a2;
+    for (int j = 0; j < Lt; ++j, p += kWave) {
+        const int2 b = *p;
+        const uint32_t x = (uint32_t)b.x;
+        const uint32_t y = x ^ ((x >> 13) | (x << 19));
+        const uint32_t h1 = __umul24(y, a1) >> (32 - siml::kSlotBits);
+        const uint32_t h2 = __umul24(y, a2) >> (32 - siml::kSlotBits);
+        const uint32_t s = (T.keys[h1] == b.x) ? h1 : ((T.keys[h2] == b.x) ? h2 : 0u);
+        const uint4 cv = T.cnt[s];
+        uint32_t cb = min((uint32_t)b.y, siml::kCountMax);
+        cb |= cb << 16;
+        acc[0] = pk_minadd(acc[0], cv.x, cb);
+        acc[1] = pk_minadd(acc[1], cv.y, cb);
+        acc[2] = pk_minadd(acc[2], cv.z, cb);
+        acc[3] = pk_minadd(acc[3], cv.w, cb);
+    }
+}
+
+__device__ __forceinline__ int acc_row(const uint32_t acc[4], int r) {
+    return (int)((acc[r >> 1] >> (16 * (r & 1))) & 0xFFFFu);
+}
+
+// Multi-row Tanimoto block: one workgroup per row group (grid.x), query
+// tiles split across waves (grid.y * 4).  Each query entry is loaded once
+// and probed once for all rows of the group.
+__global__ __launch_bounds__(kBlock) void k_hash_rows_f32(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
+                                                          ColArgs C, int64_t rb, int64_t rl, float* __restrict__ out,
+                                                          int64_t opitch) {
+    __shared__ GroupLds T;
+    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
+    stage_group(T, G, hd);
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    int ma[siml::kGroupRows];
+#pragma unroll
+    for (int r = 0; r < siml::kGroupRows; ++r) ma[r] = r < hd.nrows ? rmag[hd.row0 + r] : 0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t acc[4];
+        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
+        if (c >= C.n) continue;
+        const int mb = C.mag[c];
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r) {
+            const int64_t row = (int64_t)hd.row0 + r;
+            if (r < hd.nrows && row >= rb && row < rl)
+                out[(row - rb) * opitch + c] = tanimoto_mags(acc_row(acc, r), ma[r], mb);
+        }
     }
 }
 
@@ -325,6 +425,7 @@ struct SetInfo {
     std::vector<int32_t> len, mag, csum;
     int32_t maxlen = 0;
     bool sorted_unique = true;
+    bool counts_nonneg = true;
 };
 
 template <class F>
@@ -364,17 +465,20 @@ int validate(const HostSet& s, SetInfo& info, const char* what) {
     info.mag.resize((size_t)s.n);
     info.csum.resize((size_t)s.n);
     std::vector<char> unsorted((size_t)std::max<int64_t>(s.n, 1), 0);
+    std::vector<char> negative((size_t)std::max<int64_t>(s.n, 1), 0);
     parallel_for(s.n, [&](int64_t a, int64_t b) {
         for (int64_t r = a; r < b; ++r) {
             const int32_t L = s.length(r);
             const int32_t* id = s.ids + s.start(r);
             const int32_t* ct = s.counts + s.start(r);
             uint32_t sum = 0;
-            bool ok = true;
+            bool ok = true, neg = false;
             for (int32_t k = 0; k < L; ++k) {
                 sum += (uint32_t)ct[k];
                 if (k && id[k] <= id[k - 1]) ok = false;
+                if (ct[k] < 0) neg = true;
             }
+            if (neg) negative[(size_t)r] = 1;
             info.len[(size_t)r] = L;
             info.csum[(size_t)r] = (int32_t)sum;
             info.mag[(size_t)r] = s.mags ? s.mags[r] : (int32_t)sum;
@@ -385,6 +489,7 @@ int validate(const HostSet& s, SetInfo& info, const char* what) {
     for (int64_t r = 0; r < s.n; ++r) {
         info.maxlen = std::max(info.maxlen, info.len[(size_t)r]);
         if (unsorted[(size_t)r]) info.sorted_unique = false;
+        if (negative[(size_t)r]) info.counts_nonneg = false;
     }
     return SIML_OK;
 }
@@ -402,6 +507,11 @@ struct RowSet {
     bool sorted_unique = true;
     DevBuf elems, off, len, mag, csum;
     std::vector<int32_t> h_len;
+    // row groups of the hashed kernels
+    DevBuf g_hdr, g_ekey, g_ecnt;
+    std::vector<siml::GroupHdr> h_hdr;
+    std::vector<int32_t> merge_rows;  // rows served by the merge kernel
+    GroupArgs gargs() const { return {g_hdr.as<siml::GroupHdr>(), g_ekey.as<int2>(), g_ecnt.as<uint4>()}; }
     RowArgs args() const {
         return {elems.as<int2>(), off.as<int64_t>(), len.as<int32_t>(), mag.as<int32_t>(), csum.as<int32_t>()};
     }
@@ -411,6 +521,11 @@ struct RowSet {
         len.release();
         mag.release();
         csum.release();
+        g_hdr.release();
+        g_ekey.release();
+        g_ecnt.release();
+        h_hdr.clear();
+        merge_rows.clear();
         n = -1;
     }
 };
@@ -419,10 +534,11 @@ struct ColSet {
     int64_t n = -1, ntiles = 0;
     int32_t maxlen = 0;
     bool sorted_unique = true;
+    bool counts_nonneg = true;
     DevBuf tiles, toff, tlen, len, mag, csum;
     ColArgs args() const {
-        return {tiles.as<int2>(), toff.as<int64_t>(), len.as<int32_t>(), mag.as<int32_t>(), csum.as<int32_t>(),
-                n, ntiles};
+        return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
+                csum.as<int32_t>(), n, ntiles};
     }
     void release() {
         tiles.release();
@@ -454,6 +570,9 @@ struct siml_ctx {
     double kernel_ms = 0.0;
     int64_t launches = 0, pairs = 0, pending_pairs = 0;
     std::string last_kernel = "none";
+    int policy = SIML_POLICY_AUTO;
+    DevBuf rowlist;
+    std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
 };
 
 namespace {
@@ -543,11 +662,22 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     TRY(upload(R.len, info.len.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.mag, info.mag.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.csum, info.csum.data(), (size_t)s.n, ctx->stream));
+    // row groups and their cuckoo tables for the hashed kernels
+    std::vector<int64_t> starts((size_t)std::max<int64_t>(s.n, 1));
+    for (int64_t r = 0; r < s.n; ++r) starts[(size_t)r] = s.start(r);
+    siml::GroupInput gin{s.ids, s.counts, starts.data(), info.len.data(), info.csum.data(), s.n};
+    siml::GroupTables gt;
+    siml::build_groups(gin, gt, 0);
+    TRY(upload(R.g_hdr, gt.hdr.data(), gt.hdr.size(), ctx->stream));
+    TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
+    TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     R.n = s.n;
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
     R.h_len = std::move(info.len);
+    R.h_hdr = std::move(gt.hdr);
+    R.merge_rows = std::move(gt.merge_rows);
     return SIML_OK;
 }
 
@@ -587,6 +717,7 @@ int upload_cols(siml_ctx* ctx, const HostSet& s) {
     C.ntiles = ntiles;
     C.maxlen = info.maxlen;
     C.sorted_unique = info.sorted_unique;
+    C.counts_nonneg = info.counts_nonneg;
     return SIML_OK;
 }
 
@@ -604,6 +735,34 @@ int32_t max_len_in(const RowSet& R, int64_t r0, int64_t r1) {
     int32_t m = 0;
     for (int64_t r = r0; r < r1; ++r) m = std::max(m, R.h_len[(size_t)r]);
     return m;
+}
+
+// How rows [rb, rl) are served: hashed groups [g0, g1) plus the rows in
+// `merge` on the merge kernel, or every row on the merge kernel.
+struct RowPlan {
+    bool all_merge = true;
+    int64_t g0 = 0, g1 = 0;
+    std::vector<int32_t> merge;
+};
+
+void plan_rows(const siml_ctx* ctx, int64_t rb, int64_t rl, RowPlan& p) {
+    const RowSet& R = ctx->ref;
+    const ColSet& C = ctx->qry;
+    p = RowPlan();
+    if (ctx->policy == SIML_POLICY_MERGE || !C.sorted_unique || !C.counts_nonneg) return;
+    p.all_merge = false;
+    const auto& H = R.h_hdr;
+    p.g0 = std::partition_point(H.begin(), H.end(), [&](const siml::GroupHdr& h) { return h.row0 + h.nrows <= rb; }) -
+           H.begin();
+    p.g1 = std::partition_point(H.begin(), H.end(), [&](const siml::GroupHdr& h) { return h.row0 < rl; }) - H.begin();
+    auto a = std::lower_bound(R.merge_rows.begin(), R.merge_rows.end(), (int32_t)rb);
+    auto b = std::lower_bound(R.merge_rows.begin(), R.merge_rows.end(), (int64_t)rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
+    p.merge.assign(a, b);
+}
+
+int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
+    ctx->h_rowlist = rows;
+    return upload(ctx->rowlist, ctx->h_rowlist.data(), ctx->h_rowlist.size(), ctx->stream);
 }
 
 // Grid for row-parallel kernels: grid.x = rows, grid.y splits the query
@@ -692,6 +851,7 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->res.release();
     ctx->res2.release();
     ctx->scratch.release();
+    ctx->rowlist.release();
     for (auto& pr : ctx->pending_ev) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
@@ -777,13 +937,33 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     ctx->rows = nr;
     ctx->cols = nq;
     if (nr == 0 || nq == 0) return SIML_OK;
+    RowPlan plan;
+    plan_rows(ctx, rowbase, rowlimit, plan);
+    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     TimedRegion tr(ctx, nr * nq);
-    k_rows_f32<false><<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
-        ctx->ref.args(), ctx->qry.args(), rowbase, ctx->res.as<float>(), nq);
-    TRY(launch_check("k_rows_f32"));
+    if (plan.all_merge) {
+        k_rows_f32<false><<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
+            ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, ctx->res.as<float>(), nq);
+        TRY(launch_check("k_rows_f32"));
+        ctx->last_kernel = "rows_merge";
+    } else {
+        const int64_t ng = plan.g1 - plan.g0;
+        if (ng > 0) {
+            k_hash_rows_f32<<<row_grid(ng, ctx->qry.ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit,
+                ctx->res.as<float>(), nq);
+            TRY(launch_check("k_hash_rows_f32"));
+        }
+        if (!plan.merge.empty()) {
+            k_rows_f32<false><<<row_grid((int64_t)plan.merge.size(), ctx->qry.ntiles), dim3(kBlock), lds,
+                                ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase,
+                                               ctx->res.as<float>(), nq);
+            TRY(launch_check("k_rows_f32<rowlist>"));
+        }
+        ctx->last_kernel = plan.merge.empty() ? "rows_hash" : (ng > 0 ? "rows_hash+merge" : "rows_merge");
+    }
     tr.done();
-    ctx->last_kernel = "rows_merge";
     return SIML_OK;
 }
 
@@ -800,7 +980,7 @@ int siml_single_row_async(siml_ctx* ctx, int64_t row) {
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, row, row + 1));
     TimedRegion tr(ctx, nq);
     k_rows_f32<true><<<row_grid(1, ctx->qry.ntiles, 2048), dim3(kBlock), lds, ctx->stream>>>(
-        ctx->ref.args(), ctx->qry.args(), row, ctx->res.as<float>(), nq);
+        ctx->ref.args(), ctx->qry.args(), nullptr, row, ctx->res.as<float>(), nq);
     TRY(launch_check("k_rows_f32<single>"));
     tr.done();
     ctx->last_kernel = "single_row_merge";
@@ -988,6 +1168,13 @@ int siml_retrieve_i64(siml_ctx* ctx, int64_t* host) {
     (void)host;
     if (!ctx) return fail(SIML_EINVAL, "null handle");
     return fail(SIML_ESTATE, "no pending int64 result");
+}
+
+int siml_set_policy(siml_ctx* ctx, int policy) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE) return fail(SIML_EINVAL, "unknown policy %d", policy);
+    ctx->policy = policy;
+    return SIML_OK;
 }
 
 int siml_set_timing(siml_ctx* ctx, int on) {

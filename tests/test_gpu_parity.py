@@ -255,3 +255,45 @@ def test_ragged_sets_equal_padded(gpu):
     gpu.set_qsmiles_ragged(ids, cnt, offs, mags)
     got = gpu.getMultipleRows(0, 300)
     assert_same_f32(got, oracle.tanimoto_block(tuple(x[:300] for x in pad), pad, 0))
+
+
+@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+def test_policies_bit_identical(demo_lines, policy):
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[:777])
+    qry = oracle.pack(demo_lines[500:4096])
+    _set(g, ref, qry)
+    want = oracle.tanimoto_block(ref, qry, nthreads=0)
+    assert_same_f32(g.getMultipleRows(0, 777), want)
+    assert_same_f32(g.getMultipleRows(5, 13), want[5:13])     # range inside one group
+    assert_same_f32(g.getMultipleRows(250, 251), want[250:251])
+    name = g._h.last_kernel()
+    assert name == ("rows_hash" if policy == _lib.POLICY_AUTO else "rows_merge")
+
+
+def test_mixed_hash_and_merge_rows(demo_lines):
+    """Rows that cannot use the hashed kernel (count > 65535, unsorted ids,
+    too many ids) are served by the merge kernel in the same call."""
+    g = GPULingo(0)
+    ref = [x.copy() for x in oracle.pack(demo_lines[:100])]
+    L, C, n, m = ref
+    C[3, 0] = 70000                      # count beyond u16
+    m[3, 0] += 70000 - 1
+    if n[10, 0] >= 2:                    # unsorted row
+        L[10, [0, 1]] = L[10, [1, 0]]
+    big = 1000                           # more ids than one table holds
+    wide = np.zeros((100, max(L.shape[1], big)), np.int32)
+    wide[:, :L.shape[1]] = L
+    cwide = np.zeros_like(wide)
+    cwide[:, :C.shape[1]] = C
+    wide[20, :big] = np.arange(big)
+    cwide[20, :big] = 1
+    n[20, 0] = big
+    m[20, 0] = big
+    ref = (wide, cwide, n, m)
+    qry = oracle.pack(demo_lines[:2000])
+    _set(g, ref, qry)
+    got = g.getMultipleRows(0, 100)
+    assert g._h.last_kernel() == "rows_hash+merge"
+    assert_same_f32(got, oracle.tanimoto_block(ref, qry, nthreads=0))
