@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
                     help="auto: hashed multi-row kernels; merge: one-row-per-workgroup merge kernel")
+    ap.add_argument("--layout", choices=["block", "natural"], default="block",
+                    help="query storage order: length-sorted 256-column blocks, or the caller's order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -74,12 +76,13 @@ def main():
     log(f"packing {n} synthetic strings (U[20,120], seed 2)")
     s = synth.packed_padded(n, 20, 120, seed=2)
     lens = s[2].ravel().astype(np.int64)
+    from siml_amd import _lib
     g = GPULingo(0)
+    h = g._h
+    h.call("siml_set_layout", _lib.LAYOUT_NATURAL if args.layout == "natural" else _lib.LAYOUT_BLOCK)
+    h.call("siml_set_policy", _lib.POLICY_MERGE if args.policy == "merge" else _lib.POLICY_AUTO)
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
-    h = g._h
-    from siml_amd import _lib
-    h.call("siml_set_policy", _lib.POLICY_MERGE if args.policy == "merge" else _lib.POLICY_AUTO)
 
     for i in range(args.warmup):
         h.call("siml_multirow_async", 0, n)
@@ -117,7 +120,8 @@ def main():
         "dtype": "int32 merge, float32 division",
         "data": "synthetic (seeded SMILES-like strings, packed with the native packer)",
         "config": {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
-                   "rows": n, "columns": n, "seed": 2, "policy": args.policy},
+                   "rows": n, "columns": n, "seed": 2, "policy": args.policy,
+                   "layout": args.layout},
         "kernel": kernel,
         "kernel_ms_mean": kernel_s * 1e3,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": P_VALU / 1e12,

@@ -171,6 +171,12 @@ int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
 int siml_set_policy(siml_ctx* ctx, int policy);
+/* Storage order used by the next set_query: NATURAL keeps the caller's
+ * order; BLOCK (default) sorts the query molecules by length inside blocks
+ * of 256 consecutive columns.  Results are identical; only speed differs. */
+#define SIML_LAYOUT_NATURAL 0
+#define SIML_LAYOUT_BLOCK 1
+int siml_set_layout(siml_ctx* ctx, int layout);
 /* Name of the kernel variant the last similarity launch used. */
 const char* siml_last_kernel(siml_ctx* ctx);
 

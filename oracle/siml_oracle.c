@@ -351,9 +351,8 @@ void oracle_hist_rows101(const int32_t* rl, const int32_t* rc, const int32_t* rm
  * (No counterpart in the reference; int64 counts.) */
 static inline int32_t bin1000(float t) {
     if (!(t >= 0.0f)) return 0;
-    int32_t k = (int32_t)(t * 1000.0f);
-    if (k > 999) k = 999;
-    if (k < 0) k = 0;
+    float p = t * 1000.0f;
+    int32_t k = p >= 999.0f ? 999 : (int32_t)p;
     while (k < 999 && t >= (float)(k + 1) / 1000.0f) ++k;
     while (k > 0 && t < (float)k / 1000.0f) --k;
     return k;

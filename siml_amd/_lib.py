@@ -23,6 +23,7 @@ RESULT_NONE, RESULT_F32, RESULT_I32, RESULT_CSR, RESULT_TOPK, RESULT_I64 = range
 MAX_ROW_LINGOS = 8192
 MAX_TOPK = 256
 POLICY_AUTO, POLICY_MERGE = 0, 1
+LAYOUT_NATURAL, LAYOUT_BLOCK = 0, 1
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -69,6 +70,7 @@ SIGNATURES = {
     "siml_peek_rows_f32": (_int, [_vp, _i64, _i64, _f32p, _i64]),
     "siml_set_timing": (_int, [_vp, _int]),
     "siml_set_policy": (_int, [_vp, _int]),
+    "siml_set_layout": (_int, [_vp, _int]),
     "siml_kernel_stats": (_int, [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _int]),
     "siml_last_kernel": (ctypes.c_char_p, [_vp]),
 }

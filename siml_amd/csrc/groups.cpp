@@ -58,10 +58,13 @@ inline uint64_t next_rand(uint64_t& s) {
 }
 
 // Places keys[0..K) in a cuckoo table; returns false if the seed fails.
-bool place(const std::vector<int32_t>& keys, uint32_t a1, uint32_t a2, std::vector<int32_t>& tab) {
+bool place(const std::vector<int32_t>& keys, const GroupHdr& h, std::vector<int32_t>& tab) {
     tab.assign(kSlots, -1);
     const int K = (int)keys.size();
-    auto slot = [&](int k, int which) { return group_hash(group_mix((uint32_t)keys[(size_t)k]), which ? a2 : a1); };
+    auto slot = [&](int k, int which) {
+        const uint32_t x = (uint32_t)keys[(size_t)k];
+        return which ? group_hash(x, h.a2, h.b2) : group_hash(x, h.a1, h.b1);
+    };
     for (int k = 0; k < K; ++k) {
         int cur = k;
         uint32_t pos = slot(cur, 0);
@@ -156,9 +159,12 @@ bool build_groups(const GroupInput& in, GroupTables& out, int nthreads) {
             bool ok = false;
             for (int attempt = 0; attempt < 64 && !ok; ++attempt) {
                 h.a1 = (uint32_t)(next_rand(rs) >> 40) | 1u;
+                h.b1 = (uint32_t)(next_rand(rs) >> 40) | 1u;
                 h.a2 = (uint32_t)(next_rand(rs) >> 40) | 1u;
-                if (h.a1 == h.a2) continue;
-                ok = place(keys, h.a1, h.a2, tab);
+                h.b2 = (uint32_t)(next_rand(rs) >> 40) | 1u;
+                h.pad = 0;
+                if (h.a1 == h.a2 && h.b1 == h.b2) continue;
+                ok = place(keys, h, tab);
             }
             if (!ok) {
                 failed[(size_t)g] = 1;

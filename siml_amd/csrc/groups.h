@@ -3,7 +3,9 @@
 // A group is up to kGroupRows consecutive reference rows.  The union of
 // their Lingo ids (at most kMaxKeys) is placed in a two-choice cuckoo table
 // of kSlots slots: id x lives in slot h1(x) or h2(x), where
-//   y = x ^ rotr(x, 13);  h_i(x) = (umul24(y, a_i) mod 2^32) >> (32 - kSlotBits)
+//   h_i(x) = ((umul24(x, a_i) + umul24(x >> 8, b_i)) mod 2^32) >> (32 - kSlotBits)
+// and umul24(u, v) is the low 32 bits of (u mod 2^24) * (v mod 2^24).  Both
+// hashes see all 32 bits of x (x mod 2^24 and x >> 8 determine x).
 // Slot 0 is never used, so a probe that matches neither slot reads the
 // all-zero count vector of slot 0.  Each slot carries the id's count in each
 // row of the group as eight u16 (zero where the row lacks the id).  Empty
@@ -31,7 +33,9 @@ struct GroupHdr {
     int32_t nrows;      // 1..kGroupRows
     int32_t nkeys;      // union size
     int32_t empty_key;  // id stored in empty slots (not in the union)
-    uint32_t a1, a2;    // hash multipliers (24-bit)
+    uint32_t a1, b1;    // multipliers of h1 (24-bit)
+    uint32_t a2, b2;    // multipliers of h2 (24-bit)
+    int32_t pad;
     int64_t entry_off;  // first entry of the group in the entry arrays
 };
 
@@ -40,9 +44,9 @@ struct GroupEntry {
     int32_t slot;
 };
 
-inline uint32_t group_mix(uint32_t x) { return x ^ ((x >> 13) | (x << 19)); }
-inline uint32_t group_hash(uint32_t y, uint32_t a) {
-    return (uint32_t)((uint64_t)(y & 0xFFFFFFu) * (uint64_t)(a & 0xFFFFFFu)) >> (32 - kSlotBits);
+inline uint32_t umul24(uint32_t u, uint32_t v) { return (uint32_t)((uint64_t)(u & 0xFFFFFFu) * (v & 0xFFFFFFu)); }
+inline uint32_t group_hash(uint32_t x, uint32_t a, uint32_t b) {
+    return (umul24(x, a) + umul24(x >> 8, b)) >> (32 - kSlotBits);
 }
 
 // Host description of one reference set for grouping: molecule r has len[r]

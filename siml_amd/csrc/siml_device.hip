@@ -43,6 +43,11 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+// csr_sort.hip: sorts each segment [d_offsets[s], d_offsets[s+1]) of keys_in
+// ascending into keys_out; with d_temp == nullptr only *temp_bytes is set.
+int sort_segments_i32(const int32_t* keys_in, int32_t* keys_out, int64_t n, int64_t nseg, const int64_t* d_offsets,
+                      void* d_temp, size_t* temp_bytes, hipStream_t stream);
+
 }  // namespace siml
 
 using siml::fail;
@@ -77,6 +82,9 @@ struct RowArgs {
     const int32_t* csum;
 };
 
+// Query molecules are stored at "positions" (lane slots of the tiles);
+// perm[p] is the caller's column index of position p (null: identity).
+// len/mag/csum are indexed by position.
 struct ColArgs {
     const int2* tiles;
     const int64_t* toff;
@@ -84,9 +92,12 @@ struct ColArgs {
     const int32_t* len;
     const int32_t* mag;
     const int32_t* csum;
+    const int32_t* perm;
     int64_t n;
     int64_t ntiles;
 };
+
+__device__ __forceinline__ int64_t col_of(const ColArgs& C, int64_t p) { return C.perm ? (int64_t)C.perm[p] : p; }
 
 // multisetTanimoto3_mags' float expression (_CPULingo.c:55,70).
 __device__ __forceinline__ float tanimoto_mags(int isct, int ma, int mb) {
@@ -183,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_rows_f32(RowArgs R, ColArgs C, const
         if (c >= C.n) continue;
         const int Lb = C.len[c];
         const int isct = merge_isct(lds_row, La, C.tiles + C.toff[t] + lane, Lb);
-        orow[c] = RECOMPUTE ? tanimoto_recomputed(isct, La, Lb, ma, C.csum[c]) : tanimoto_mags(isct, ma, C.mag[c]);
+        orow[col_of(C, c)] = RECOMPUTE ? tanimoto_recomputed(isct, La, Lb, ma, C.csum[c]) : tanimoto_mags(isct, ma, C.mag[c]);
     }
 }
 
@@ -230,25 +241,62 @@ __device__ __forceinline__ void stage_group(GroupLds& T, const GroupArgs& G, con
 // Intersections of one query molecule (a lane) with all rows of the group:
 // acc[w] holds rows 2w (low u16) and 2w+1 (high u16).  The loop runs over
 // the tile's uniform step count; padding entries carry count 0 and add 0.
+// One column entry against the group's table: both cuckoo slots are read
+// unconditionally and the matching one (or the all-zero slot 0) selected
+// with masks, so the probe has no branch.
+struct HashParams {
+    uint32_t a1, b1, a2, b2;
+};
+
+// umul24 as in groups.h.  __umul24 returns a signed int: the sum is formed
+// unsigned, so the shifted slot is always in [0, kSlots).
+__device__ __forceinline__ uint32_t umul24u(uint32_t u, uint32_t v) { return (uint32_t)__umul24(u, v); }
+
+__device__ __forceinline__ void group_probe(const GroupLds& T, const HashParams& hp, int2 b, uint32_t acc[4]) {
+    const uint32_t x = (uint32_t)b.x;
+    const uint32_t xh = x >> 8;
+    const uint32_t h1 = (umul24u(x, hp.a1) + umul24u(xh, hp.b1)) >> (32 - siml::kSlotBits);
+    const uint32_t h2 = (umul24u(x, hp.a2) + umul24u(xh, hp.b2)) >> (32 - siml::kSlotBits);
+    const int32_t k1 = T.keys[h1];
+    const int32_t k2 = T.keys[h2];
+    const uint32_t m1 = 0u - (uint32_t)(k1 == b.x);
+    const uint32_t m2 = (0u - (uint32_t)(k2 == b.x)) & ~m1;
+    const uint4 cv = T.cnt[(h1 & m1) | (h2 & m2)];
+    uint32_t cb = min((uint32_t)b.y, siml::kCountMax);
+    cb |= cb << 16;
+    acc[0] = pk_minadd(acc[0], cv.x, cb);
+    acc[1] = pk_minadd(acc[1], cv.y, cb);
+    acc[2] = pk_minadd(acc[2], cv.z, cb);
+    acc[3] = pk_minadd(acc[3], cv.w, cb);
+}
+
 __device__ __forceinline__ void group_isct(const GroupLds& T, const siml::GroupHdr& hd, const int2* __restrict__ p,
                                            int Lt, uint32_t acc[4]) {
     acc[0] = acc[1] = acc[2] = acc[3] = 0u;
-    const uint32_t a1 = hd.a1, a2 = hd.a2;
-    for (int j = 0; j < Lt; ++j, p += kWave) {
-        const int2 b = *p;
-        const uint32_t x = (uint32_t)b.x;
-        const uint32_t y = x ^ ((x >> 13) | (x << 19));
-        const uint32_t h1 = __umul24(y, a1) >> (32 - siml::kSlotBits);
-        const uint32_t h2 = __umul24(y, a2) >> (32 - siml::kSlotBits);
-        const uint32_t s = (T.keys[h1] == b.x) ? h1 : ((T.keys[h2] == b.x) ? h2 : 0u);
-        const uint4 cv = T.cnt[s];
-        uint32_t cb = min((uint32_t)b.y, siml::kCountMax);
-        cb |= cb << 16;
-        acc[0] = pk_minadd(acc[0], cv.x, cb);
-        acc[1] = pk_minadd(acc[1], cv.y, cb);
-        acc[2] = pk_minadd(acc[2], cv.z, cb);
-        acc[3] = pk_minadd(acc[3], cv.w, cb);
+    const HashParams hp{hd.a1, hd.b1, hd.a2, hd.b2};
+    int j = 0;
+    if (Lt >= 8) {
+        // loads run four steps ahead of the probes
+        int2 b0 = p[0], b1 = p[kWave], b2 = p[2 * kWave], b3 = p[3 * kWave];
+        for (; j + 8 <= Lt; j += 4) {
+            const int2* q = p + (size_t)(j + 4) * kWave;
+            const int2 n0 = q[0], n1 = q[kWave], n2 = q[2 * kWave], n3 = q[3 * kWave];
+            group_probe(T, hp, b0, acc);
+            group_probe(T, hp, b1, acc);
+            group_probe(T, hp, b2, acc);
+            group_probe(T, hp, b3, acc);
+            b0 = n0;
+            b1 = n1;
+            b2 = n2;
+            b3 = n3;
+        }
+        group_probe(T, hp, b0, acc);
+        group_probe(T, hp, b1, acc);
+        group_probe(T, hp, b2, acc);
+        group_probe(T, hp, b3, acc);
+        j += 4;
     }
+    for (; j < Lt; ++j) group_probe(T, hp, p[(size_t)j * kWave], acc);
 }
 
 __device__ __forceinline__ int acc_row(const uint32_t acc[4], int r) {
@@ -280,18 +328,452 @@ __global__ __launch_bounds__(kBlock) void k_hash_rows_f32(GroupArgs G, int64_t g
         for (int r = 0; r < siml::kGroupRows; ++r) {
             const int64_t row = (int64_t)hd.row0 + r;
             if (r < hd.nrows && row >= rb && row < rl)
-                out[(row - rb) * opitch + c] = tanimoto_mags(acc_row(acc, r), ma[r], mb);
+                out[(row - rb) * opitch + col_of(C, c)] = tanimoto_mags(acc_row(acc, r), ma[r], mb);
         }
+    }
+}
+
+// gpuHistogrammedLingo_multirow (kernels.py:606-667) on row groups: per-row
+// LDS bins, flushed to the int32 (rows, 101) result with global atomics.
+__global__ __launch_bounds__(kBlock) void k_hash_hist_rows(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
+                                                           ColArgs C, int64_t rb, int64_t rl,
+                                                           int32_t* __restrict__ hist) {
+    __shared__ GroupLds T;
+    __shared__ int bins[siml::kGroupRows][101];
+    for (int k = threadIdx.x; k < siml::kGroupRows * 101; k += blockDim.x) (&bins[0][0])[k] = 0;
+    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
+    stage_group(T, G, hd);
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    int ma[siml::kGroupRows];
+    bool live[siml::kGroupRows];
+#pragma unroll
+    for (int r = 0; r < siml::kGroupRows; ++r) {
+        const int64_t row = (int64_t)hd.row0 + r;
+        live[r] = r < hd.nrows && row >= rb && row < rl;
+        ma[r] = live[r] ? rmag[row] : 0;
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t acc[4];
+        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
+        if (c >= C.n) continue;
+        const int mb = C.mag[c];
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r)
+            if (live[r]) atomicAdd(&bins[r][bin101_f32(tanimoto_mags(acc_row(acc, r), ma[r], mb))], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < siml::kGroupRows * 101; k += blockDim.x) {
+        const int r = k / 101, b = k - r * 101;
+        if (live[r] && bins[r][b]) atomicAdd(&hist[((int64_t)hd.row0 + r - rb) * 101 + b], bins[r][b]);
+    }
+}
+
+// gpuGetNeighbors_multirow (kernels.py:393-475) on row groups.  Same
+// criterion and ordering as k_neighbors: each wave owns a contiguous tile
+// range and keeps one running count per row; pass 1 counts, pass 2 writes
+// ascending column indices at the scanned offsets.
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_hash_neighbors(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
+                                                           ColArgs C, int64_t rb, int64_t rl, float lo, float hi,
+                                                           int32_t* __restrict__ counts,
+                                                           const int64_t* __restrict__ offs,
+                                                           int32_t* __restrict__ idx) {
+    __shared__ GroupLds T;
+    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
+    stage_group(T, G, hd);
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    int ma[siml::kGroupRows];
+    bool live[siml::kGroupRows];
+    int64_t pos[siml::kGroupRows];
+#pragma unroll
+    for (int r = 0; r < siml::kGroupRows; ++r) {
+        const int64_t row = (int64_t)hd.row0 + r;
+        live[r] = r < hd.nrows && row >= rb && row < rl;
+        ma[r] = live[r] ? rmag[row] : 0;
+        pos[r] = (FILL && live[r]) ? offs[(row - rb) * nw + gw] : 0;
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t acc[4];
+        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
+        const bool valid = c < C.n;
+        const int mb = valid ? C.mag[c] : 0;
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r) {
+            if (!live[r]) continue;
+            bool nb = false;
+            if (valid) {
+                const float bound = (float)min(ma[r], mb) / (float)max(ma[r], mb);
+                if (bound >= lo) {
+                    const float tv = tanimoto_mags(acc_row(acc, r), ma[r], mb);
+                    nb = (tv >= lo) && (tv < hi);
+                }
+            }
+            const uint64_t m = __ballot(nb);
+            if (FILL && nb) idx[pos[r] + rank_below(m)] = (int32_t)col_of(C, c);
+            pos[r] += __popcll(m);
+        }
+    }
+    if (!FILL && lane == 0) {
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r)
+            if (live[r]) counts[((int64_t)hd.row0 + r - rb) * nw + gw] = (int32_t)pos[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1000-bin global histogram and top-k (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+
+// Edges fl32(k/1000), k = 0..1000, as one correctly rounded division each.
+__device__ __forceinline__ void stage_edges(float* e) {
+    for (int k = threadIdx.x; k <= 1000; k += blockDim.x) e[k] = (float)k / 1000.0f;
+}
+
+// max{k : t >= e[k]} clipped to [0, 999]; t < 0 or NaN -> 0.  The guess
+// from t*1000 is corrected against the float32 edges.
+__device__ __forceinline__ int bin1000(float t, const float* e) {
+    if (!(t >= 0.0f)) return 0;
+    const float p = t * 1000.0f;
+    int k = p >= 999.0f ? 999 : (int)p;
+    while (k < 999 && t >= e[k + 1]) ++k;
+    while (k > 0 && t < e[k]) --k;
+    return k;
+}
+
+// Ordering key of (T, column): larger T first, equal T by smaller column.
+__device__ __forceinline__ uint64_t topk_key(float t, int64_t c) {
+    const uint32_t u = __float_as_uint(t);
+    const uint32_t o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((uint64_t)o << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)c);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The k-th largest of the n > k distinct keys in buf[0..n), found by one
+// wave with an MSB-first 8-bit radix select; hist is 256 wave-private LDS
+// counters.  Called in wave-uniform control flow.
+__device__ uint64_t wave_kth_largest(const uint64_t* buf, int n, int k, uint32_t* hist) {
+    const int lane = lane_id();
+    uint64_t prefix = 0, pmask = 0;
+    uint32_t need = (uint32_t)k;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int b = lane; b < 256; b += kWave) hist[b] = 0u;
+        wave_lds_sync();
+        for (int i = lane; i < n; i += kWave) {
+            const uint64_t key = buf[i];
+            if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        wave_lds_sync();
+        const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+        const uint32_t s = h0 + h1 + h2 + h3;
+        uint32_t S = s;  // keys in bins of lanes >= this lane
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t v = __shfl_down(S, off);
+            if (lane + off < kWave) S += v;
+        }
+        const uint64_t ge = __ballot(S >= need);
+        const int L = 63 - __builtin_clzll(ge);
+        uint32_t above = __shfl(S - s, L);
+        const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
+                                (uint32_t)__shfl(h3, L)};
+        int d = 4 * L;
+        for (int q = 3; q >= 0; --q) {
+            if (above + hb[q] >= need) {
+                d = 4 * L + q;
+                break;
+            }
+            above += hb[q];
+        }
+        need -= above;
+        prefix |= (uint64_t)d << shift;
+        pmask |= (uint64_t)255 << shift;
+        wave_lds_sync();
+    }
+    return prefix;
+}
+
+// Keeps the keys >= K of buf[0..n) in order; returns how many were kept.
+__device__ int wave_compact_ge(uint64_t* buf, int n, uint64_t K) {
+    const int lane = lane_id();
+    int w = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane;
+        const uint64_t key = i < n ? buf[i] : 0ull;
+        const bool keep = i < n && key >= K;
+        const uint64_t m = __ballot(keep);
+        if (keep) buf[w + rank_below(m)] = key;
+        w += __popcll(m);
+    }
+    return w;
+}
+
+// Per-(row, wave) candidate buffer of `cap` keys.  Appends the lanes'
+// candidates that beat the running threshold; when the next append could
+// overflow, shrinks the buffer to its k best and raises the threshold to
+// the k-th best.  Discarded keys have k better keys in the same buffer, so
+// the row's global top-k always survives.
+struct TopkBuf {
+    uint64_t* buf;
+    int cnt;
+    bool armed;
+    uint64_t theta;
+};
+
+__device__ __forceinline__ void topk_push(TopkBuf& B, bool valid, uint64_t key, int k, int cap, uint32_t* hist) {
+    if (B.cnt + kWave > cap) {
+        const uint64_t K = wave_kth_largest(B.buf, B.cnt, k, hist);
+        B.cnt = wave_compact_ge(B.buf, B.cnt, K);
+        B.theta = K;
+        B.armed = true;
+    }
+    const bool pred = valid && (!B.armed || key > B.theta);
+    const uint64_t m = __ballot(pred);
+    if (pred) B.buf[B.cnt + rank_below(m)] = key;
+    B.cnt += __popcll(m);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hash_hist1000(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
+                                                          ColArgs C, int64_t rb, int64_t rl,
+                                                          unsigned long long* __restrict__ hist) {
+    __shared__ GroupLds T;
+    __shared__ uint32_t bins[1000];
+    __shared__ float edges[1001];
+    for (int k = threadIdx.x; k < 1000; k += blockDim.x) bins[k] = 0u;
+    stage_edges(edges);
+    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
+    stage_group(T, G, hd);
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    int ma[siml::kGroupRows];
+    bool live[siml::kGroupRows];
+#pragma unroll
+    for (int r = 0; r < siml::kGroupRows; ++r) {
+        const int64_t row = (int64_t)hd.row0 + r;
+        live[r] = r < hd.nrows && row >= rb && row < rl;
+        ma[r] = live[r] ? rmag[row] : 0;
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t acc[4];
+        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
+        if (c >= C.n) continue;
+        const int mb = C.mag[c];
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r)
+            if (live[r]) atomicAdd(&bins[bin1000(tanimoto_mags(acc_row(acc, r), ma[r], mb), edges)], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 1000; k += blockDim.x)
+        if (bins[k]) atomicAdd(&hist[k], (unsigned long long)bins[k]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hist1000(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
+                                                     int64_t rowbase, unsigned long long* __restrict__ hist) {
+    extern __shared__ int2 lds_row[];
+    __shared__ uint32_t bins[1000];
+    __shared__ float edges[1001];
+    for (int k = threadIdx.x; k < 1000; k += blockDim.x) bins[k] = 0u;
+    stage_edges(edges);
+    const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
+    const int La = R.len[row];
+    const int ma = R.mag[row];
+    stage_row(lds_row, R.elems + R.off[row], La);
+    __syncthreads();
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        if (c >= C.n) continue;
+        const int isct = merge_isct(lds_row, La, C.tiles + C.toff[t] + lane, C.len[c]);
+        atomicAdd(&bins[bin1000(tanimoto_mags(isct, ma, C.mag[c]), edges)], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 1000; k += blockDim.x)
+        if (bins[k]) atomicAdd(&hist[k], (unsigned long long)bins[k]);
+}
+
+// Top-k candidates per (row, wave): buffers at ((row-rb)*nw + gw) * cap.
+__global__ __launch_bounds__(kBlock) void k_hash_topk(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
+                                                      ColArgs C, int64_t rb, int64_t rl, int k, int cap,
+                                                      uint64_t* __restrict__ buf, int32_t* __restrict__ bufcnt) {
+    __shared__ GroupLds T;
+    __shared__ uint32_t rhist[kBlock / kWave][256];
+    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
+    stage_group(T, G, hd);
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    uint32_t* hist = rhist[threadIdx.x / kWave];
+    int ma[siml::kGroupRows];
+    bool live[siml::kGroupRows];
+    TopkBuf B[siml::kGroupRows];
+#pragma unroll
+    for (int r = 0; r < siml::kGroupRows; ++r) {
+        const int64_t row = (int64_t)hd.row0 + r;
+        live[r] = r < hd.nrows && row >= rb && row < rl;
+        ma[r] = live[r] ? rmag[row] : 0;
+        B[r].buf = buf + ((live[r] ? row - rb : 0) * nw + gw) * (int64_t)cap;
+        B[r].cnt = 0;
+        B[r].armed = false;
+        B[r].theta = 0ull;
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t acc[4];
+        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
+        const bool valid = c < C.n;
+        const int mb = valid ? C.mag[c] : 0;
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r)
+            if (live[r]) topk_push(B[r], valid, topk_key(tanimoto_mags(acc_row(acc, r), ma[r], mb), valid ? col_of(C, c) : 0), k, cap, hist);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < siml::kGroupRows; ++r)
+            if (live[r]) bufcnt[((int64_t)hd.row0 + r - rb) * nw + gw] = B[r].cnt;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
+                                                 int64_t rowbase, int k, int cap, uint64_t* __restrict__ buf,
+                                                 int32_t* __restrict__ bufcnt) {
+    extern __shared__ int2 lds_row[];
+    __shared__ uint32_t rhist[kBlock / kWave][256];
+    const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
+    const int La = R.len[row];
+    const int ma = R.mag[row];
+    stage_row(lds_row, R.elems + R.off[row], La);
+    __syncthreads();
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    uint32_t* hist = rhist[threadIdx.x / kWave];
+    TopkBuf B{buf + ((row - rowbase) * nw + gw) * (int64_t)cap, 0, false, 0ull};
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        const bool valid = c < C.n;
+        float tv = 0.0f;
+        if (valid) tv = tanimoto_mags(merge_isct(lds_row, La, C.tiles + C.toff[t] + lane, C.len[c]), ma, C.mag[c]);
+        topk_push(B, valid, topk_key(tv, valid ? col_of(C, c) : 0), k, cap, hist);
+    }
+    if (lane == 0) bufcnt[(row - rowbase) * nw + gw] = B.cnt;
+}
+
+// Final top-k of each row from its nw candidate buffers: a workgroup radix
+// select of the k-th best key, then a bitonic sort of the (at most 256)
+// keys at or above it.  Output rows are padded with (-1, -inf).
+__global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restrict__ buf,
+                                                       const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
+                                                       int32_t* __restrict__ out_idx, float* __restrict__ out_t) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t sorted[kBlock];
+    __shared__ uint64_t s_prefix, s_pmask;
+    __shared__ uint32_t s_need, s_total, s_fill;
+    const int64_t rloc = blockIdx.x;
+    const uint64_t* rbuf = buf + rloc * nw * (int64_t)cap;
+    const int32_t* rcnt = bufcnt + rloc * nw;
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < nw; ++w) tot += (uint32_t)rcnt[w];
+        s_total = tot;
+        s_prefix = 0ull;
+        s_pmask = 0ull;
+        s_need = (uint32_t)k;
+        s_fill = 0u;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
+    uint64_t K = 0ull;  // every key is > 0
+    if (total > (uint32_t)k) {
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            hist[threadIdx.x] = 0u;
+            __syncthreads();
+            const uint64_t prefix = s_prefix, pmask = s_pmask;
+            for (int w = 0; w < nw; ++w) {
+                const int n = rcnt[w];
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    const uint64_t key = rbuf[(int64_t)w * cap + i];
+                    if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t above = 0, need = s_need;
+                int d = 0;
+                for (int b = 255; b >= 0; --b) {
+                    if (above + hist[b] >= need) {
+                        d = b;
+                        break;
+                    }
+                    above += hist[b];
+                }
+                s_need = need - above;
+                s_prefix = prefix | ((uint64_t)d << shift);
+                s_pmask = pmask | ((uint64_t)255 << shift);
+            }
+            __syncthreads();
+        }
+        K = s_prefix;
+    }
+    sorted[threadIdx.x] = 0ull;
+    __syncthreads();
+    for (int w = 0; w < nw; ++w) {
+        const int n = rcnt[w];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t key = rbuf[(int64_t)w * cap + i];
+            if (key >= K) sorted[atomicAdd(&s_fill, 1u)] = key;
+        }
+    }
+    __syncthreads();
+    // bitonic sort, descending, of kBlock keys (k <= kBlock)
+    for (int size = 2; size <= kBlock; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int i = threadIdx.x, j = i ^ stride;
+            if (j > i) {
+                const bool desc = (i & size) == 0;
+                const uint64_t a = sorted[i], b = sorted[j];
+                if (desc ? (a < b) : (a > b)) {
+                    sorted[i] = b;
+                    sorted[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t have = s_fill;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        int32_t ix = -1;
+        float tv = -__builtin_inff();
+        if ((uint32_t)i < have) {
+            const uint64_t key = sorted[i];
+            const uint32_t o = (uint32_t)(key >> 32);
+            tv = __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+            ix = (int32_t)(0xFFFFFFFFu - (uint32_t)key);
+        }
+        out_idx[rloc * k + i] = ix;
+        out_t[rloc * k + i] = tv;
     }
 }
 
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
 // per-row 101-bin histogram; per-workgroup LDS bins, flushed with global
 // atomics because several workgroups share a row.
-__global__ __launch_bounds__(kBlock) void k_hist_rows(RowArgs R, ColArgs C, int64_t rowbase, int32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kBlock) void k_hist_rows(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
+                                                      int64_t rowbase, int32_t* __restrict__ hist) {
     extern __shared__ int2 lds_row[];
     __shared__ int bins[101];
-    const int64_t row = rowbase + blockIdx.x;
+    const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
     const int La = R.len[row];
     const int ma = R.mag[row];
     for (int k = threadIdx.x; k < 101; k += blockDim.x) bins[k] = 0;
@@ -308,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_rows(RowArgs R, ColArgs C, int6
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 101; k += blockDim.x)
-        if (bins[k]) atomicAdd(&hist[(int64_t)blockIdx.x * 101 + k], bins[k]);
+        if (bins[k]) atomicAdd(&hist[(row - rowbase) * 101 + k], bins[k]);
 }
 
 // gpuGetNeighbors_multirow (kernels.py:393-475): magnitude-bound prefilter
@@ -317,11 +799,12 @@ __global__ __launch_bounds__(kBlock) void k_hist_rows(RowArgs R, ColArgs C, int6
 // and written in ascending column order.  Pass 1 (FILL = false) counts per
 // (row, wave); pass 2 writes at the scanned offsets.
 template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, int64_t rowbase, float lo, float hi,
+__global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
+                                                      int64_t rowbase, float lo, float hi,
                                                       int32_t* __restrict__ counts, const int64_t* __restrict__ offs,
                                                       int32_t* __restrict__ idx) {
     extern __shared__ int2 lds_row[];
-    const int64_t row = rowbase + blockIdx.x;
+    const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
     const int La = R.len[row];
     const int ma = R.mag[row];
     stage_row(lds_row, R.elems + R.off[row], La);
@@ -329,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, int6
     int64_t t0, t1, gw, nw;
     wave_tiles(C.ntiles, t0, t1, gw, nw);
     const int lane = lane_id();
-    const int64_t slot = (int64_t)blockIdx.x * nw + gw;
+    const int64_t slot = (row - rowbase) * nw + gw;
     int64_t pos = FILL ? offs[slot] : 0;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t c = t * kWave + lane;
@@ -344,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, int6
             }
         }
         const uint64_t m = __ballot(nb);
-        if (FILL && nb) idx[pos + rank_below(m)] = (int32_t)c;
+        if (FILL && nb) idx[pos + rank_below(m)] = (int32_t)col_of(C, c);
         pos += __popcll(m);
     }
     if (!FILL && lane == 0) counts[slot] = (int32_t)pos;
@@ -535,10 +1018,11 @@ struct ColSet {
     int32_t maxlen = 0;
     bool sorted_unique = true;
     bool counts_nonneg = true;
-    DevBuf tiles, toff, tlen, len, mag, csum;
+    DevBuf tiles, toff, tlen, len, mag, csum, perm;
+    bool identity = true;
     ColArgs args() const {
         return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
-                csum.as<int32_t>(), n, ntiles};
+                csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), n, ntiles};
     }
     void release() {
         tiles.release();
@@ -547,6 +1031,7 @@ struct ColSet {
         len.release();
         mag.release();
         csum.release();
+        perm.release();
         n = -1;
     }
 };
@@ -571,7 +1056,8 @@ struct siml_ctx {
     int64_t launches = 0, pairs = 0, pending_pairs = 0;
     std::string last_kernel = "none";
     int policy = SIML_POLICY_AUTO;
-    DevBuf rowlist;
+    int layout = SIML_LAYOUT_BLOCK;
+    DevBuf rowlist, sort_tmp, csr_dev;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
 };
 
@@ -681,17 +1167,45 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     return SIML_OK;
 }
 
+// Storage order of the query molecules.  NATURAL keeps the caller's order;
+// BLOCK sorts by length inside each block of kSortBlock consecutive columns,
+// so a tile's step count (its longest molecule) is close to its mean while a
+// wave's float32 outputs stay inside one 1 KiB span of the row.
+constexpr int64_t kSortBlock = 256;
+
+std::vector<int32_t> column_order(const SetInfo& info, int64_t n, int layout) {
+    std::vector<int32_t> pos((size_t)n);
+    for (int64_t c = 0; c < n; ++c) pos[(size_t)c] = (int32_t)c;
+    if (layout == SIML_LAYOUT_BLOCK)
+        for (int64_t b = 0; b < n; b += kSortBlock) {
+            auto first = pos.begin() + b, last = pos.begin() + std::min(n, b + kSortBlock);
+            std::stable_sort(first, last, [&](int32_t x, int32_t y) { return info.len[(size_t)x] < info.len[(size_t)y]; });
+        }
+    return pos;
+}
+
 int upload_cols(siml_ctx* ctx, const HostSet& s) {
     SetInfo info;
     TRY(validate(s, info, "set_qsmiles"));
+    if (s.n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "set_qsmiles: more than 2^31-1 molecules");
     TRY(set_device(ctx));
     ColSet& C = ctx->qry;
+    const std::vector<int32_t> order = column_order(info, s.n, ctx->layout);
+    bool identity = true;
+    for (int64_t p = 0; p < s.n && identity; ++p) identity = order[(size_t)p] == (int32_t)p;
+    std::vector<int32_t> plen((size_t)s.n), pmag((size_t)s.n), pcsum((size_t)s.n);
+    for (int64_t p = 0; p < s.n; ++p) {
+        const size_t c = (size_t)order[(size_t)p];
+        plen[(size_t)p] = info.len[c];
+        pmag[(size_t)p] = info.mag[c];
+        pcsum[(size_t)p] = info.csum[c];
+    }
     const int64_t ntiles = (s.n + kWave - 1) / kWave;
     std::vector<int32_t> tlen((size_t)std::max<int64_t>(ntiles, 1), 0);
     std::vector<int64_t> toff((size_t)ntiles + 1, 0);
     for (int64_t t = 0; t < ntiles; ++t) {
         int32_t m = 0;
-        for (int64_t c = t * kWave; c < std::min(s.n, (t + 1) * kWave); ++c) m = std::max(m, info.len[(size_t)c]);
+        for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) m = std::max(m, plen[(size_t)p]);
         tlen[(size_t)t] = m;
         toff[(size_t)t + 1] = toff[(size_t)t] + (int64_t)m * kWave;
     }
@@ -699,25 +1213,28 @@ int upload_cols(siml_ctx* ctx, const HostSet& s) {
     std::vector<int2> tiles((size_t)std::max<int64_t>(total, 1), make_int2(0, 0));
     parallel_for(ntiles, [&](int64_t a, int64_t b) {
         for (int64_t t = a; t < b; ++t)
-            for (int64_t c = t * kWave; c < std::min(s.n, (t + 1) * kWave); ++c) {
+            for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) {
+                const int64_t c = order[(size_t)p];
                 const int32_t* id = s.ids + s.start(c);
                 const int32_t* ct = s.counts + s.start(c);
-                int2* d = tiles.data() + toff[(size_t)t] + (c - t * kWave);
-                for (int32_t j = 0; j < info.len[(size_t)c]; ++j) d[(size_t)j * kWave] = make_int2(id[j], ct[j]);
+                int2* d = tiles.data() + toff[(size_t)t] + (p - t * kWave);
+                for (int32_t j = 0; j < plen[(size_t)p]; ++j) d[(size_t)j * kWave] = make_int2(id[j], ct[j]);
             }
     });
     TRY(upload(C.tiles, tiles.data(), (size_t)total, ctx->stream));
     TRY(upload(C.toff, toff.data(), (size_t)ntiles + 1, ctx->stream));
     TRY(upload(C.tlen, tlen.data(), (size_t)ntiles, ctx->stream));
-    TRY(upload(C.len, info.len.data(), (size_t)s.n, ctx->stream));
-    TRY(upload(C.mag, info.mag.data(), (size_t)s.n, ctx->stream));
-    TRY(upload(C.csum, info.csum.data(), (size_t)s.n, ctx->stream));
+    TRY(upload(C.len, plen.data(), (size_t)s.n, ctx->stream));
+    TRY(upload(C.mag, pmag.data(), (size_t)s.n, ctx->stream));
+    TRY(upload(C.csum, pcsum.data(), (size_t)s.n, ctx->stream));
+    TRY(upload(C.perm, order.data(), (size_t)s.n, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     C.n = s.n;
     C.ntiles = ntiles;
     C.maxlen = info.maxlen;
     C.sorted_unique = info.sorted_unique;
     C.counts_nonneg = info.counts_nonneg;
+    C.identity = identity;
     return SIML_OK;
 }
 
@@ -759,6 +1276,8 @@ void plan_rows(const siml_ctx* ctx, int64_t rb, int64_t rl, RowPlan& p) {
     auto b = std::lower_bound(R.merge_rows.begin(), R.merge_rows.end(), (int64_t)rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
     p.merge.assign(a, b);
 }
+
+int32_t* d_counts_of(siml_ctx* ctx) { return ctx->scratch.as<int32_t>(); }
 
 int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
     ctx->h_rowlist = rows;
@@ -852,6 +1371,8 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->res2.release();
     ctx->scratch.release();
     ctx->rowlist.release();
+    ctx->sort_tmp.release();
+    ctx->csr_dev.release();
     for (auto& pr : ctx->pending_ev) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
@@ -998,13 +1519,31 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     ctx->cols = 101;
     TRY(zero_fill_i32(ctx, ctx->res.as<int32_t>(), nr * 101));
     if (nr == 0 || nq == 0) return SIML_OK;
+    RowPlan plan;
+    plan_rows(ctx, rowbase, rowlimit, plan);
+    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
+    int32_t* hist = ctx->res.as<int32_t>();
     TimedRegion tr(ctx, nr * nq);
-    k_hist_rows<<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
-        ctx->ref.args(), ctx->qry.args(), rowbase, ctx->res.as<int32_t>());
-    TRY(launch_check("k_hist_rows"));
+    if (plan.all_merge) {
+        k_hist_rows<<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
+            ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, hist);
+        TRY(launch_check("k_hist_rows"));
+    } else {
+        const int64_t ng = plan.g1 - plan.g0;
+        if (ng > 0) {
+            k_hash_hist_rows<<<row_grid(ng, ctx->qry.ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, hist);
+            TRY(launch_check("k_hash_hist_rows"));
+        }
+        if (!plan.merge.empty()) {
+            k_hist_rows<<<row_grid((int64_t)plan.merge.size(), ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+            TRY(launch_check("k_hist_rows<rowlist>"));
+        }
+    }
     tr.done();
-    ctx->last_kernel = "hist_rows_merge";
+    ctx->last_kernel = plan.all_merge ? "hist_rows_merge" : "hist_rows_hash";
     return SIML_OK;
 }
 
@@ -1021,18 +1560,57 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
         TRY(ctx->res.reserve(256));
         return SIML_OK;
     }
+    RowPlan plan;
+    plan_rows(ctx, rowbase, rowlimit, plan);
+    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const dim3 grid = row_grid(nr, ctx->qry.ntiles, 8192, int64_t(1) << 24);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
+    const int64_t ng = plan.g1 - plan.g0;
+    const dim3 hgrid((unsigned)std::max<int64_t>(ng, 1), grid.y);
+    const dim3 mgrid((unsigned)std::max<size_t>(plan.merge.size(), 1), grid.y);
+    auto launch = [&](bool fill, const int64_t* offs, int32_t* idx) -> int {
+        if (plan.all_merge) {
+            if (fill)
+                k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr,
+                                                                            rowbase, lo, hi, nullptr, offs, idx);
+            else
+                k_neighbors<false><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr,
+                                                                             rowbase, lo, hi, d_counts_of(ctx), nullptr,
+                                                                             nullptr);
+            return launch_check("k_neighbors");
+        }
+        if (ng > 0) {
+            if (fill)
+                k_hash_neighbors<true><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
+                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, lo, hi,
+                    nullptr, offs, idx);
+            else
+                k_hash_neighbors<false><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
+                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, lo, hi,
+                    d_counts_of(ctx), nullptr, nullptr);
+            TRY(launch_check("k_hash_neighbors"));
+        }
+        if (!plan.merge.empty()) {
+            if (fill)
+                k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(),
+                                                                             ctx->rowlist.as<int32_t>(), rowbase, lo,
+                                                                             hi, nullptr, offs, idx);
+            else
+                k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(),
+                                                                              ctx->rowlist.as<int32_t>(), rowbase, lo,
+                                                                              hi, d_counts_of(ctx), nullptr, nullptr);
+            TRY(launch_check("k_neighbors<rowlist>"));
+        }
+        return SIML_OK;
+    };
     const int64_t nslots = nr * nw;
     const size_t counts_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
     TRY(ctx->scratch.reserve(counts_bytes + sizeof(int64_t) * (size_t)nslots));
     int32_t* d_counts = ctx->scratch.as<int32_t>();
     int64_t* d_offs = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + counts_bytes);
     TimedRegion tr(ctx, nr * nq);
-    k_neighbors<false><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), rowbase, lo, hi,
-                                                                   d_counts, nullptr, nullptr);
-    TRY(launch_check("k_neighbors<count>"));
+    TRY(launch(false, nullptr, nullptr));
     std::vector<int32_t> h_counts((size_t)nslots);
     HIP_TRY(hipMemcpyAsync(h_counts.data(), d_counts, sizeof(int32_t) * nslots, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1049,29 +1627,120 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     ctx->nnz = acc;
     HIP_TRY(hipMemcpyAsync(d_offs, h_offs.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(acc, 1)));
-    k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), rowbase, lo, hi,
-                                                                  nullptr, d_offs, ctx->res.as<int32_t>());
-    TRY(launch_check("k_neighbors<fill>"));
+    TRY(launch(true, d_offs, ctx->res.as<int32_t>()));
+    if (!ctx->qry.identity && acc > 0) {
+        // indices were written in storage order: sort each row ascending
+        TRY(upload(ctx->csr_dev, ctx->csr_rows.data(), (size_t)nr + 1, ctx->stream));
+        TRY(ctx->res2.reserve(sizeof(int32_t) * (size_t)acc));
+        size_t tmp_bytes = 0;
+        TRY(siml::sort_segments_i32(ctx->res.as<int32_t>(), ctx->res2.as<int32_t>(), acc, nr,
+                                    ctx->csr_dev.as<int64_t>(), nullptr, &tmp_bytes, ctx->stream));
+        TRY(ctx->sort_tmp.reserve(std::max<size_t>(tmp_bytes, 16)));
+        TRY(siml::sort_segments_i32(ctx->res.as<int32_t>(), ctx->res2.as<int32_t>(), acc, nr,
+                                    ctx->csr_dev.as<int64_t>(), ctx->sort_tmp.p, &tmp_bytes, ctx->stream));
+        std::swap(ctx->res, ctx->res2);
+    }
     tr.done();
-    // h_offs must outlive the async copy above.
+    // h_offs and csr_rows must outlive the async copies above.
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->last_kernel = "neighbors_merge";
+    ctx->last_kernel = plan.all_merge ? "neighbors_merge" : "neighbors_hash";
     return SIML_OK;
 }
 
 int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
-    (void)ctx;
-    (void)rowbase;
-    (void)rowlimit;
-    (void)k;
-    return fail(SIML_ESTATE, "siml_topk_async: not built yet");
+    TRY(check_rows(ctx, rowbase, rowlimit, "getTopK"));
+    if (k < 1 || k > SIML_MAX_TOPK) return fail(SIML_EINVAL, "getTopK: k=%d outside [1, %d]", k, SIML_MAX_TOPK);
+    if (ctx->qry.n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "getTopK: more than 2^31-1 query molecules");
+    TRY(set_device(ctx));
+    clear_result(ctx);
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(nr * k, 1)));
+    TRY(ctx->res2.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * k, 1)));
+    ctx->kind = SIML_RESULT_TOPK;
+    ctx->rows = nr;
+    ctx->cols = k;
+    if (nr == 0) return SIML_OK;
+    RowPlan plan;
+    plan_rows(ctx, rowbase, rowlimit, plan);
+    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
+    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
+    const int cap = (int)(((k + kWave) + kWave - 1) / kWave * kWave);
+    dim3 grid = row_grid(nr, ctx->qry.ntiles, 2048);
+    grid.y = std::min<unsigned>(grid.y, 64u);
+    const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
+    const int64_t nslots = nr * nw;
+    const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
+    TRY(ctx->scratch.reserve(buf_bytes + sizeof(int32_t) * (size_t)nslots));
+    uint64_t* d_buf = ctx->scratch.as<uint64_t>();
+    int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes);
+    const int64_t ng = plan.g1 - plan.g0;
+    TimedRegion tr(ctx, nr * nq);
+    if (plan.all_merge) {
+        k_topk<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, k, cap,
+                                                         d_buf, d_cnt);
+        TRY(launch_check("k_topk"));
+    } else {
+        if (ng > 0) {
+            k_hash_topk<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, k, cap,
+                d_buf, d_cnt);
+            TRY(launch_check("k_hash_topk"));
+        }
+        if (!plan.merge.empty()) {
+            k_topk<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
+            TRY(launch_check("k_topk<rowlist>"));
+        }
+    }
+    k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_buf, d_cnt, (int)nw, k, cap,
+                                                                     ctx->res.as<int32_t>(), ctx->res2.as<float>());
+    TRY(launch_check("k_topk_final"));
+    tr.done();
+    ctx->last_kernel = plan.all_merge ? "topk_merge" : "topk_hash";
+    return SIML_OK;
 }
 
 int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
-    (void)ctx;
-    (void)rowbase;
-    (void)rowlimit;
-    return fail(SIML_ESTATE, "siml_hist1000_async: not built yet");
+    TRY(check_rows(ctx, rowbase, rowlimit, "getGlobalHistogram"));
+    TRY(set_device(ctx));
+    clear_result(ctx);
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    TRY(ctx->res.reserve(sizeof(uint64_t) * 1000));
+    ctx->kind = SIML_RESULT_I64;
+    ctx->rows = 1;
+    ctx->cols = 1000;
+    unsigned long long* hist = ctx->res.as<unsigned long long>();
+    k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
+    TRY(launch_check("k_fill"));
+    if (nr == 0 || nq == 0) return SIML_OK;
+    RowPlan plan;
+    plan_rows(ctx, rowbase, rowlimit, plan);
+    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
+    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
+    dim3 grid = row_grid(nr, ctx->qry.ntiles, 2048);
+    // one workgroup counts at most 8 rows x its columns in uint32 LDS bins
+    const int64_t min_gy = (8 * kWave * ctx->qry.ntiles) / (int64_t(1) << 31) + 1;
+    grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
+    const int64_t ng = plan.g1 - plan.g0;
+    TimedRegion tr(ctx, nr * nq);
+    if (plan.all_merge) {
+        k_hist1000<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, hist);
+        TRY(launch_check("k_hist1000"));
+    } else {
+        if (ng > 0) {
+            k_hash_hist1000<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, hist);
+            TRY(launch_check("k_hash_hist1000"));
+        }
+        if (!plan.merge.empty()) {
+            k_hist1000<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+            TRY(launch_check("k_hist1000<rowlist>"));
+        }
+    }
+    tr.done();
+    ctx->last_kernel = plan.all_merge ? "hist1000_merge" : "hist1000_hash";
+    return SIML_OK;
 }
 
 int siml_query_done(siml_ctx* ctx, int* done) {
@@ -1158,22 +1827,42 @@ int siml_retrieve_csr(siml_ctx* ctx, int64_t* offsets, int32_t* indices) {
 }
 
 int siml_retrieve_topk(siml_ctx* ctx, int32_t* idx, float* t) {
-    (void)idx;
-    (void)t;
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    return fail(SIML_ESTATE, "no pending top-k result");
+    if (ctx->kind != SIML_RESULT_TOPK) return fail(SIML_ESTATE, "no pending top-k result");
+    TRY(set_device(ctx));
+    const size_t n = (size_t)(ctx->rows * ctx->cols);
+    if (n) {
+        if (!idx || !t) return fail(SIML_EINVAL, "null output buffer");
+        HIP_TRY(hipMemcpyAsync(idx, ctx->res.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(t, ctx->res2.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    clear_result(ctx);
+    return SIML_OK;
 }
 
 int siml_retrieve_i64(siml_ctx* ctx, int64_t* host) {
-    (void)host;
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    return fail(SIML_ESTATE, "no pending int64 result");
+    if (ctx->kind != SIML_RESULT_I64) return fail(SIML_ESTATE, "no pending int64 result");
+    if (!host) return fail(SIML_EINVAL, "null output buffer");
+    TRY(set_device(ctx));
+    HIP_TRY(hipMemcpyAsync(host, ctx->res.p, sizeof(int64_t) * (size_t)ctx->cols, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    clear_result(ctx);
+    return SIML_OK;
 }
 
 int siml_set_policy(siml_ctx* ctx, int policy) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
     if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE) return fail(SIML_EINVAL, "unknown policy %d", policy);
     ctx->policy = policy;
+    return SIML_OK;
+}
+
+int siml_set_layout(siml_ctx* ctx, int layout) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK) return fail(SIML_EINVAL, "unknown layout %d", layout);
+    ctx->layout = layout;
     return SIML_OK;
 }
 

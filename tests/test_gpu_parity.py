@@ -297,3 +297,77 @@ def test_mixed_hash_and_merge_rows(demo_lines):
     got = g.getMultipleRows(0, 100)
     assert g._h.last_kernel() == "rows_hash+merge"
     assert_same_f32(got, oracle.tanimoto_block(ref, qry, nthreads=0))
+
+
+@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+@pytest.mark.parametrize("k", [1, 8, 100, 256])
+def test_topk(demo_lines, policy, k):
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[:70])
+    qry = oracle.pack(demo_lines)
+    _set(g, ref, qry)
+    idx, t = g.getTopK(0, 70, k)
+    oi, ot = oracle.topk(ref, qry, k)
+    assert np.array_equal(idx, oi)
+    assert_same_f32(t, ot)
+
+
+def test_topk_more_than_columns_and_ties(demo_lines):
+    g = GPULingo(0)
+    ref = oracle.pack(demo_lines[:20])
+    qry = oracle.pack(demo_lines[:37] + [b"CCCC"] * 30)   # exact ties
+    _set(g, ref, qry)
+    idx, t = g.getTopK(3, 20, 100)
+    oi, ot = oracle.topk(tuple(x[3:20] for x in ref), qry, 100)
+    assert np.array_equal(idx, oi)
+    assert_same_f32(t, ot)
+    assert np.all(idx[:, 67:] == -1)
+
+
+@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+def test_global_histogram(demo_lines, demo_set, policy):
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[:333])
+    _set(g, ref, demo_set)
+    h = g.getGlobalHistogram(0, 333)
+    assert h.dtype == np.int64 and h.sum() == 333 * 4096
+    assert np.array_equal(h, oracle.hist1000(ref, demo_set))
+    h2 = g.getGlobalHistogram(100, 101)
+    assert np.array_equal(h2, oracle.hist1000(tuple(x[100:101] for x in ref), demo_set))
+
+
+@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+def test_hist_and_neighbors_both_policies(demo_lines, demo_set, policy):
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[1000:1150])
+    _set(g, ref, demo_set)
+    assert np.array_equal(g.getMultipleHistogrammedRows(3, 150), oracle.hist_rows101(
+        tuple(x[3:150] for x in ref), demo_set))
+    offs, idx = g.getNeighborsCSR(0, 150, 0.4)
+    lists, counts = oracle.neighbors(ref, demo_set, 0.4)
+    assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+
+
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK])
+def test_layouts_bit_identical(demo_lines, demo_set, layout):
+    """Column storage order (natural or length-sorted blocks) changes no bit
+    of any result, including neighbour order and top-k ties."""
+    g = GPULingo(0)
+    g._h.call("siml_set_layout", layout)
+    ref = oracle.pack(demo_lines[2000:2300])
+    qry = oracle.pack(demo_lines[:3000] + [b"CCCC"] * 40)
+    _set(g, ref, qry)
+    assert_same_f32(g.getMultipleRows(0, 300), oracle.tanimoto_block(ref, qry, nthreads=0))
+    assert_same_f32(g.getTanimotoRow(5)[0], oracle.single_row(ref, 5, qry))
+    offs, idx = g.getNeighborsCSR(0, 300, 0.2)
+    lists, counts = oracle.neighbors(ref, qry, 0.2)
+    assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+    ti, tt = g.getTopK(0, 300, 50)
+    oi, ot = oracle.topk(ref, qry, 50)
+    assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
+    assert np.array_equal(g.getGlobalHistogram(0, 300), oracle.hist1000(ref, qry))
+    g._h.call("siml_set_policy", _lib.POLICY_MERGE)
+    assert_same_f32(g.getMultipleRows(10, 40), oracle.tanimoto_block(tuple(x[10:40] for x in ref), qry, 0))
