@@ -52,19 +52,42 @@ def cpu_baseline(s, budget_s=10.0):
             "seconds": round(dt, 3)}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (hipMemcpy), the known-size
+    reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_ALLPAIRS, help="molecules (rows = columns)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
-                    help="auto: hashed multi-row kernels; merge: one-row-per-workgroup merge kernel")
+    ap.add_argument("--policy", choices=["auto", "hash8", "merge"], default="auto",
+                    help="auto: bit-sliced 28-row kernels; hash8: 8-row hashed kernels; "
+                         "merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural"], default="block",
                     help="query storage order: length-sorted 256-column blocks, or the caller's order")
     args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
@@ -80,7 +103,8 @@ def main():
     g = GPULingo(0)
     h = g._h
     h.call("siml_set_layout", _lib.LAYOUT_NATURAL if args.layout == "natural" else _lib.LAYOUT_BLOCK)
-    h.call("siml_set_policy", _lib.POLICY_MERGE if args.policy == "merge" else _lib.POLICY_AUTO)
+    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "hash8": _lib.POLICY_HASH8,
+                               "merge": _lib.POLICY_MERGE}[args.policy])
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
 

@@ -112,6 +112,18 @@ int siml_set_ref_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* counts
                         const int64_t* offsets, int64_t n);
 int siml_set_query_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* counts, const int32_t* mags,
                           const int64_t* offsets, int64_t n);
+/* Resident query databases (no counterpart in the reference).  A database
+ * is a ragged query set built once in device memory; any number of handles
+ * on the same device attach it as their query set and run concurrently on
+ * their own streams.  siml_db_destroy drops the caller's reference; the
+ * device memory is freed when no handle uses it any more.  layout is
+ * SIML_LAYOUT_NATURAL or SIML_LAYOUT_BLOCK. */
+typedef struct siml_db siml_db;
+int siml_db_create(int device, const int32_t* ids, const int32_t* counts, const int32_t* mags,
+                   const int64_t* offsets, int64_t n, int layout, siml_db** out);
+int siml_db_destroy(siml_db* db);
+int siml_db_size(siml_db* db, int64_t* n);
+int siml_attach_db(siml_ctx* ctx, siml_db* db);
 /* Sizes of the current sets (-1 when unset). */
 int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery);
 
@@ -164,12 +176,14 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
  * since the last reset. */
 int siml_set_timing(siml_ctx* ctx, int on);
 int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_t* pairs, int reset);
-/* Kernel policy: AUTO serves sorted-unique sets with the hashed multi-row
- * kernels (rows that do not qualify, and unsorted sets, use the merge
- * kernel); MERGE uses the merge kernel for everything.  Both give the same
- * bits. */
+/* Kernel policy: AUTO serves sorted-unique sets with the bit-sliced
+ * multi-row kernels (rows that do not qualify, and unsorted sets, use the
+ * merge kernel); HASH8 uses the 8-row hashed kernels instead of the
+ * bit-sliced ones; MERGE uses the merge kernel for everything.  All give the
+ * same bits. */
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
+#define SIML_POLICY_HASH8 2
 int siml_set_policy(siml_ctx* ctx, int policy);
 /* Storage order used by the next set_query: NATURAL keeps the caller's
  * order; BLOCK (default) sorts the query molecules by length inside blocks

@@ -120,6 +120,15 @@ class GPULingo:
         self.nquery = len(offsets) - 1
         self.qshape = (self.nquery, 0)
 
+    def attach_db(self, db):
+        """Use the resident database *db* (a LingoDB on the same device) as
+        the query set; several comparators may share one database."""
+        self._h.call("siml_attach_db", db.h)
+        self._db = db  # keep the database object alive while attached
+        self.nquery = len(db)
+        self.qshape = (self.nquery, 0)
+
+
     # -- helpers -----------------------------------------------------------
     def _rows(self, rowbase, rowlimit):
         rowbase, rowlimit = int(rowbase), int(rowlimit)
@@ -263,3 +272,38 @@ def _ragged(ids, counts, offsets, mags):
         if mags.size != offsets.size - 1:
             raise ValueError("magnitude vector length does not match offsets")
     return ids, counts, offsets, mags
+
+
+class LingoDB:
+    """A query (column) set built once in device memory and shared by any
+    number of comparators on the same device (no counterpart in the
+    reference; the reference re-uploads its query set per comparator,
+    GPULingo.py:245-314).  Input is ragged: molecule r owns
+    ids[offsets[r]:offsets[r+1]]."""
+
+    def __init__(self, ids, counts, offsets, mags=None, device=0, layout=_lib.LAYOUT_BLOCK):
+        import ctypes
+        ids, counts, offsets, mags = _ragged(ids, counts, offsets, mags)
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        ids_ = ids if ids.size else np.zeros(1, np.int32)
+        counts_ = counts if counts.size else np.zeros(1, np.int32)
+        _lib.check(self._lib.siml_db_create(int(device), _lib.i32p(ids_), _lib.i32p(counts_), _lib.i32p(mags),
+                                            _lib.i64p(offsets), len(offsets) - 1, int(layout), ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+        self.n = len(offsets) - 1
+
+    def __len__(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.siml_db_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

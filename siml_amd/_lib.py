@@ -22,7 +22,7 @@ SIML_ELIMIT = -5
 RESULT_NONE, RESULT_F32, RESULT_I32, RESULT_CSR, RESULT_TOPK, RESULT_I64 = range(6)
 MAX_ROW_LINGOS = 8192
 MAX_TOPK = 256
-POLICY_AUTO, POLICY_MERGE = 0, 1
+POLICY_AUTO, POLICY_MERGE, POLICY_HASH8 = 0, 1, 2
 LAYOUT_NATURAL, LAYOUT_BLOCK = 0, 1
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -53,6 +53,10 @@ SIGNATURES = {
     "siml_set_ref_ragged": (_int, [_vp, _i32p, _i32p, _i32p, _i64p, _i64]),
     "siml_set_query_ragged": (_int, [_vp, _i32p, _i32p, _i32p, _i64p, _i64]),
     "siml_set_sizes": (_int, [_vp, _i64p, _i64p]),
+    "siml_db_create": (_int, [_int, _i32p, _i32p, _i32p, _i64p, _i64, _int, ctypes.POINTER(_vp)]),
+    "siml_db_destroy": (_int, [_vp]),
+    "siml_db_size": (_int, [_vp, _i64p]),
+    "siml_attach_db": (_int, [_vp, _vp]),
     "siml_multirow_async": (_int, [_vp, _i64, _i64]),
     "siml_single_row_async": (_int, [_vp, _i64]),
     "siml_hist_rows_async": (_int, [_vp, _i64, _i64]),

@@ -21,11 +21,13 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "bitgroups.h"
 #include "groups.h"
 #include "siml_internal.h"
 
@@ -93,6 +95,9 @@ struct ColArgs {
     const int32_t* mag;
     const int32_t* csum;
     const int32_t* perm;
+    const int2* btiles;    // level stream (bit-sliced kernels)
+    const int64_t* btoff;
+    const int32_t* btlen;
     int64_t n;
     int64_t ntiles;
 };
@@ -766,6 +771,335 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Bit-sliced multi-row kernels (row groups of bitgroups.h)
+//
+// A workgroup stages its group's level table (kBitSlots x (level key, row
+// mask | t << 28)) in LDS.  Each lane streams one query molecule's levels;
+// every level is probed once and the matched 28-bit row mask is added to 28
+// vertical binary counters held as bit planes P[0..NB): blocks of 8 masks go
+// through a carry-save adder tree (xor3 / majority, one v_bitop3 each) and
+// the block's carry ripples into the upper planes.  A bit-plane transpose
+// then yields each row's intersection count.
+// ---------------------------------------------------------------------------
+
+struct BitGroupArgs {
+    const siml::BitGroupHdr* hdr;
+    const int32_t* rows;   // kBitRowStride row indices per group
+    const uint4* ent;      // (level key, word, slot, -) per entry
+    const int32_t* glist;  // the groups of this launch
+};
+
+struct BitHash {
+    uint32_t a1, b1, a2, b2;
+};
+
+__device__ __forceinline__ uint32_t bit_probe(const uint2* __restrict__ tab, const BitHash& hp, int2 e) {
+    const uint32_t y = (uint32_t)e.x, tt = (uint32_t)e.y;
+    const uint32_t yh = y >> 8;
+    const uint32_t h1 = (umul24u(y, hp.a1) + umul24u(yh, hp.b1)) >> (32 - siml::kBitSlotBits);
+    const uint32_t h2 = (umul24u(y, hp.a2) + umul24u(yh, hp.b2)) >> (32 - siml::kBitSlotBits);
+    const uint2 k1 = tab[h1], k2 = tab[h2];
+    const bool c1 = (k1.x == y) && ((k1.y ^ tt) < (1u << 28));
+    const bool c2 = (k2.x == y) && ((k2.y ^ tt) < (1u << 28));
+    return c1 ? k1.y : (c2 ? k2.y : 0u);
+}
+
+// Full adder on 32 independent bit lanes: s += a + b, carry out in c.
+__device__ __forceinline__ void full_add(uint32_t& s, uint32_t a, uint32_t b, uint32_t& c) {
+    c = (s & a) | (s & b) | (a & b);
+    s = s ^ a ^ b;
+}
+
+// Adds 8 masks to the counters P (carry-save tree, then ripple).
+template <int NB>
+__device__ __forceinline__ void csa8(uint32_t (&P)[NB], const uint32_t (&m)[8]) {
+    uint32_t c0, c1, c2, c3, d0, d1, e;
+    full_add(P[0], m[0], m[1], c0);
+    full_add(P[0], m[2], m[3], c1);
+    full_add(P[0], m[4], m[5], c2);
+    full_add(P[0], m[6], m[7], c3);
+    full_add(P[1], c0, c1, d0);
+    full_add(P[1], c2, c3, d1);
+    full_add(P[2], d0, d1, e);
+#pragma unroll
+    for (int k = 3; k < NB; ++k) {
+        const uint32_t t = P[k] & e;
+        P[k] ^= e;
+        e = t;
+    }
+}
+
+// Streams the Lt (a multiple of 8) levels of one query molecule; the next
+// block of 8 loads is issued before the current block is probed.
+template <int NB>
+__device__ __forceinline__ void bits_accumulate(const uint2* __restrict__ tab, const BitHash& hp,
+                                                const int2* __restrict__ p, int Lt, uint32_t (&P)[NB]) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) P[k] = 0u;
+    if (Lt <= 0) return;
+    int2 cur[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cur[q] = p[q * kWave];
+    for (int j = 0; j < Lt; j += 8) {
+        int2 nxt[8];
+        if (j + 8 < Lt) {
+            const int2* n = p + (size_t)(j + 8) * kWave;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) nxt[q] = n[q * kWave];
+        }
+        uint32_t m[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) m[q] = bit_probe(tab, hp, cur[q]);
+        csa8<NB>(P, m);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+    }
+}
+
+__device__ __forceinline__ void swap_bits(uint32_t& a, uint32_t& b, int sh, uint32_t mask) {
+    const uint32_t t = ((a >> sh) ^ b) & mask;
+    b ^= t;
+    a ^= t << sh;
+}
+
+// In-place transpose of the 8x8 bit blocks of P[0..8): afterwards byte q of
+// P[j] is the 8-bit value whose bit i was bit (8q + j) of the old P[i].
+__device__ __forceinline__ void transpose8(uint32_t* P) {
+    swap_bits(P[0], P[4], 4, 0x0F0F0F0Fu);
+    swap_bits(P[1], P[5], 4, 0x0F0F0F0Fu);
+    swap_bits(P[2], P[6], 4, 0x0F0F0F0Fu);
+    swap_bits(P[3], P[7], 4, 0x0F0F0F0Fu);
+    swap_bits(P[0], P[2], 2, 0x33333333u);
+    swap_bits(P[1], P[3], 2, 0x33333333u);
+    swap_bits(P[4], P[6], 2, 0x33333333u);
+    swap_bits(P[5], P[7], 2, 0x33333333u);
+    swap_bits(P[0], P[1], 1, 0x55555555u);
+    swap_bits(P[2], P[3], 1, 0x55555555u);
+    swap_bits(P[4], P[5], 1, 0x55555555u);
+    swap_bits(P[6], P[7], 1, 0x55555555u);
+}
+
+// Row r's count after transpose8 of each 8-plane half.
+template <int NB>
+__device__ __forceinline__ int count_of(const uint32_t (&P)[NB], int r) {
+    const int j = r & 7, sh = 8 * (r >> 3);
+    int v = (int)((P[j] >> sh) & 0xFFu);
+    if constexpr (NB == 16) v |= (int)((P[8 + j] >> sh) & 0xFFu) << 8;
+    return v;
+}
+
+struct BitRowInfo {
+    int32_t row[32];  // caller row index of group row r, -1 when not live
+    int32_t ma[32];
+    float maf[32];
+};
+
+// Epilogue policies: init (LDS), begin (per wave), apply (per row, tile),
+// end (per wave), finish (after the workgroup barrier).
+struct EpiF32 {
+    float* out;
+    int64_t opitch, rb;
+    struct Lds {
+        int unused;
+    };
+    __device__ void init(Lds&) const {}
+    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void apply(Lds&, int, int32_t row, bool valid, int64_t col, int, int, float t, int64_t,
+                          int64_t) const {
+        if (valid) out[(row - rb) * opitch + col] = t;
+    }
+    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void finish(Lds&, const BitRowInfo&) const {}
+};
+
+struct EpiHist101 {
+    int32_t* hist;
+    int64_t rb;
+    struct Lds {
+        int bins[siml::kBitRows][101];
+    };
+    __device__ void init(Lds& L) const {
+        for (int k = threadIdx.x; k < siml::kBitRows * 101; k += blockDim.x) (&L.bins[0][0])[k] = 0;
+    }
+    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void apply(Lds& L, int r, int32_t, bool valid, int64_t, int, int, float t, int64_t, int64_t) const {
+        if (valid) atomicAdd(&L.bins[r][bin101_f32(t)], 1);
+    }
+    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void finish(Lds& L, const BitRowInfo& ri) const {
+        for (int k = threadIdx.x; k < siml::kBitRows * 101; k += blockDim.x) {
+            const int r = k / 101, b = k - r * 101;
+            if (ri.row[r] >= 0 && L.bins[r][b]) atomicAdd(&hist[((int64_t)ri.row[r] - rb) * 101 + b], L.bins[r][b]);
+        }
+    }
+};
+
+struct EpiHist1000 {
+    unsigned long long* hist;
+    struct Lds {
+        uint32_t bins[1000];
+        float edges[1001];
+    };
+    __device__ void init(Lds& L) const {
+        for (int k = threadIdx.x; k < 1000; k += blockDim.x) L.bins[k] = 0u;
+        stage_edges(L.edges);
+    }
+    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void apply(Lds& L, int, int32_t, bool valid, int64_t, int, int, float t, int64_t, int64_t) const {
+        if (valid) atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
+    }
+    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void finish(Lds& L, const BitRowInfo&) const {
+        for (int k = threadIdx.x; k < 1000; k += blockDim.x)
+            if (L.bins[k]) atomicAdd(&hist[k], (unsigned long long)L.bins[k]);
+    }
+};
+
+// Neighbour count (FILL = false) / fill pass: one running position per
+// (wave, row) in LDS, ballot compaction in ascending storage order.
+template <bool FILL>
+struct EpiNbr {
+    float lo, hi;
+    int32_t* counts;
+    const int64_t* offs;
+    int32_t* idx;
+    int64_t rb;
+    struct Lds {
+        int64_t pos[kBlock / kWave][32];
+    };
+    __device__ void init(Lds&) const {}
+    __device__ void begin(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+        const int w = threadIdx.x / kWave, lane = lane_id();
+        if (lane < 32) L.pos[w][lane] = (FILL && ri.row[lane] >= 0) ? offs[((int64_t)ri.row[lane] - rb) * nw + gw] : 0;
+        wave_lds_sync();
+    }
+    __device__ void apply(Lds& L, int r, int32_t, bool valid, int64_t col, int ma, int mb, float t, int64_t,
+                          int64_t) const {
+        bool nb = false;
+        if (valid) {
+            const float bound = (float)min(ma, mb) / (float)max(ma, mb);
+            nb = bound >= lo && t >= lo && t < hi;
+        }
+        const uint64_t m = __ballot(nb);
+        if (m == 0ull) return;
+        const int w = threadIdx.x / kWave;
+        const int64_t pos = L.pos[w][r];
+        if (FILL && nb) idx[pos + rank_below(m)] = (int32_t)col;
+        wave_lds_sync();
+        if (lane_id() == 0) L.pos[w][r] = pos + __popcll(m);
+        wave_lds_sync();
+    }
+    __device__ void end(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+        const int w = threadIdx.x / kWave, lane = lane_id();
+        if (!FILL && lane < 32 && ri.row[lane] >= 0) counts[((int64_t)ri.row[lane] - rb) * nw + gw] = (int32_t)L.pos[w][lane];
+    }
+    __device__ void finish(Lds&, const BitRowInfo&) const {}
+};
+
+// Top-k candidates per (row, wave), as topk_push with the per-row state in
+// LDS.
+struct EpiTopk {
+    int k, cap;
+    uint64_t* buf;
+    int32_t* bufcnt;
+    int64_t rb;
+    struct Lds {
+        int32_t cnt[kBlock / kWave][32];
+        int32_t armed[kBlock / kWave][32];
+        uint64_t theta[kBlock / kWave][32];
+        uint32_t hist[kBlock / kWave][256];
+    };
+    __device__ void init(Lds&) const {}
+    __device__ void begin(Lds& L, const BitRowInfo&, int64_t, int64_t) const {
+        const int w = threadIdx.x / kWave, lane = lane_id();
+        if (lane < 32) {
+            L.cnt[w][lane] = 0;
+            L.armed[w][lane] = 0;
+            L.theta[w][lane] = 0ull;
+        }
+        wave_lds_sync();
+    }
+    __device__ void apply(Lds& L, int r, int32_t row, bool valid, int64_t col, int, int, float t, int64_t gw,
+                          int64_t nw) const {
+        const int w = threadIdx.x / kWave;
+        TopkBuf B{buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap, L.cnt[w][r], L.armed[w][r] != 0,
+                  L.theta[w][r]};
+        const int before = B.cnt;
+        topk_push(B, valid, topk_key(t, col), k, cap, L.hist[w]);
+        if (B.cnt != before || B.armed != (L.armed[w][r] != 0)) {
+            wave_lds_sync();
+            if (lane_id() == 0) {
+                L.cnt[w][r] = B.cnt;
+                L.armed[w][r] = B.armed ? 1 : 0;
+                L.theta[w][r] = B.theta;
+            }
+            wave_lds_sync();
+        }
+    }
+    __device__ void end(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+        const int w = threadIdx.x / kWave, lane = lane_id();
+        if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = L.cnt[w][lane];
+    }
+    __device__ void finish(Lds&, const BitRowInfo&) const {}
+};
+
+template <int NB, class Epi>
+__global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
+                                                 int64_t rb, int64_t rl, Epi epi) {
+    __shared__ uint2 tab[siml::kBitSlots];
+    __shared__ BitRowInfo ri;
+    __shared__ typename Epi::Lds el;
+    const int g = G.glist[blockIdx.x];
+    const siml::BitGroupHdr hd = G.hdr[g];
+    for (int s = threadIdx.x; s < siml::kBitSlots; s += blockDim.x) tab[s] = make_uint2(0u, 0u);
+    if (threadIdx.x < 32) {
+        const int r = threadIdx.x;
+        const int32_t row = r < hd.nrows ? G.rows[(int64_t)g * siml::kBitRowStride + r] : -1;
+        const bool live = row >= 0 && row >= rb && row < rl;
+        ri.row[r] = live ? row : -1;
+        const int32_t ma = live ? rmag[row] : 0;
+        ri.ma[r] = ma;
+        ri.maf[r] = (float)ma;
+    }
+    epi.init(el);
+    __syncthreads();
+    for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
+        const uint4 e = G.ent[hd.entry_off + k];
+        tab[e.z] = make_uint2(e.x, e.y);
+    }
+    __syncthreads();
+    const BitHash hp{hd.a1, hd.b1, hd.a2, hd.b2};
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    const int lane = lane_id();
+    epi.begin(el, ri, gw, nw);
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t c = t * kWave + lane;
+        uint32_t P[NB];
+        bits_accumulate<NB>(tab, hp, C.btiles + C.btoff[t] + lane, C.btlen[t], P);
+        transpose8(P);
+        if constexpr (NB == 16) transpose8(P + 8);
+        const bool valid = c < C.n;
+        const int mb = valid ? C.mag[c] : 0;
+        const float mbf = (float)mb;
+        const int64_t col = valid ? col_of(C, c) : 0;
+#pragma unroll
+        for (int r = 0; r < siml::kBitRows; ++r) {
+            const int32_t row = ri.row[r];
+            if (row < 0) continue;
+            const int ma = ri.ma[r];
+            const float fi = (float)count_of<NB>(P, r);
+            // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
+            const float tv = (ma == 0 || mb == 0) ? 0.0f : fi / ((ri.maf[r] + mbf) - fi);
+            epi.apply(el, r, row, valid, col, ma, mb, tv, gw, nw);
+        }
+    }
+    epi.end(el, ri, gw, nw);
+    __syncthreads();
+    epi.finish(el, ri);
+}
+
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
 // per-row 101-bin histogram; per-workgroup LDS bins, flushed with global
 // atomics because several workgroups share a row.
@@ -995,6 +1329,13 @@ struct RowSet {
     std::vector<siml::GroupHdr> h_hdr;
     std::vector<int32_t> merge_rows;  // rows served by the merge kernel
     GroupArgs gargs() const { return {g_hdr.as<siml::GroupHdr>(), g_ekey.as<int2>(), g_ecnt.as<uint4>()}; }
+    // bit-sliced row groups
+    DevBuf b_hdr, b_rows, b_ent;
+    std::vector<siml::BitGroupHdr> h_bhdr;
+    std::vector<int32_t> h_brows, bmerge_rows;
+    BitGroupArgs bargs(const int32_t* glist) const {
+        return {b_hdr.as<siml::BitGroupHdr>(), b_rows.as<int32_t>(), b_ent.as<uint4>(), glist};
+    }
     RowArgs args() const {
         return {elems.as<int2>(), off.as<int64_t>(), len.as<int32_t>(), mag.as<int32_t>(), csum.as<int32_t>()};
     }
@@ -1009,20 +1350,35 @@ struct RowSet {
         g_ecnt.release();
         h_hdr.clear();
         merge_rows.clear();
+        b_hdr.release();
+        b_rows.release();
+        b_ent.release();
+        h_bhdr.clear();
+        h_brows.clear();
+        bmerge_rows.clear();
         n = -1;
     }
 };
 
 struct ColSet {
+    ColSet() = default;
+    ColSet(const ColSet&) = delete;
+    ColSet& operator=(const ColSet&) = delete;
+    ~ColSet() { release(); }
     int64_t n = -1, ntiles = 0;
     int32_t maxlen = 0;
     bool sorted_unique = true;
     bool counts_nonneg = true;
     DevBuf tiles, toff, tlen, len, mag, csum, perm;
     bool identity = true;
+    // level stream for the bit-sliced kernels: (level key, t << 28) pairs in
+    // the same tiles/positions, step counts padded to a multiple of 8
+    DevBuf btiles, btoff, btlen;
+    bool levels_ok = false;
     ColArgs args() const {
         return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
-                csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), n, ntiles};
+                csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), btiles.as<int2>(), btoff.as<int64_t>(),
+                btlen.as<int32_t>(), n, ntiles};
     }
     void release() {
         tiles.release();
@@ -1032,6 +1388,9 @@ struct ColSet {
         mag.release();
         csum.release();
         perm.release();
+        btiles.release();
+        btoff.release();
+        btlen.release();
         n = -1;
     }
 };
@@ -1042,7 +1401,7 @@ struct siml_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     RowSet ref;
-    ColSet qry;
+    std::shared_ptr<ColSet> qry = std::make_shared<ColSet>();  // may be shared with a siml_db
     // pending result
     int kind = SIML_RESULT_NONE;
     int64_t rows = 0, cols = 0, nnz = 0;
@@ -1057,7 +1416,9 @@ struct siml_ctx {
     std::string last_kernel = "none";
     int policy = SIML_POLICY_AUTO;
     int layout = SIML_LAYOUT_BLOCK;
-    DevBuf rowlist, sort_tmp, csr_dev;
+    bool group_by_mag = false;
+    DevBuf rowlist, sort_tmp, csr_dev, glist;
+    std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
 };
 
@@ -1157,6 +1518,12 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     TRY(upload(R.g_hdr, gt.hdr.data(), gt.hdr.size(), ctx->stream));
     TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
     TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
+    siml::BitGroupInput bin{s.ids, s.counts, starts.data(), info.len.data(), info.mag.data(), s.n};
+    siml::BitGroupTables bt;
+    siml::build_bit_groups(bin, ctx->group_by_mag, bt, 0);
+    TRY(upload(R.b_hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
+    TRY(upload(R.b_rows, bt.rows.data(), bt.rows.size(), ctx->stream));
+    TRY(upload(R.b_ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     R.n = s.n;
     R.maxlen = info.maxlen;
@@ -1164,6 +1531,9 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     R.h_len = std::move(info.len);
     R.h_hdr = std::move(gt.hdr);
     R.merge_rows = std::move(gt.merge_rows);
+    R.h_bhdr = std::move(bt.hdr);
+    R.h_brows = std::move(bt.rows);
+    R.bmerge_rows = std::move(bt.merge_rows);
     return SIML_OK;
 }
 
@@ -1184,13 +1554,12 @@ std::vector<int32_t> column_order(const SetInfo& info, int64_t n, int layout) {
     return pos;
 }
 
-int upload_cols(siml_ctx* ctx, const HostSet& s) {
+// Builds a query set in device memory of the current device.
+int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
     SetInfo info;
     TRY(validate(s, info, "set_qsmiles"));
     if (s.n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "set_qsmiles: more than 2^31-1 molecules");
-    TRY(set_device(ctx));
-    ColSet& C = ctx->qry;
-    const std::vector<int32_t> order = column_order(info, s.n, ctx->layout);
+    const std::vector<int32_t> order = column_order(info, s.n, layout);
     bool identity = true;
     for (int64_t p = 0; p < s.n && identity; ++p) identity = order[(size_t)p] == (int32_t)p;
     std::vector<int32_t> plen((size_t)s.n), pmag((size_t)s.n), pcsum((size_t)s.n);
@@ -1221,14 +1590,54 @@ int upload_cols(siml_ctx* ctx, const HostSet& s) {
                 for (int32_t j = 0; j < plen[(size_t)p]; ++j) d[(size_t)j * kWave] = make_int2(id[j], ct[j]);
             }
     });
-    TRY(upload(C.tiles, tiles.data(), (size_t)total, ctx->stream));
-    TRY(upload(C.toff, toff.data(), (size_t)ntiles + 1, ctx->stream));
-    TRY(upload(C.tlen, tlen.data(), (size_t)ntiles, ctx->stream));
-    TRY(upload(C.len, plen.data(), (size_t)s.n, ctx->stream));
-    TRY(upload(C.mag, pmag.data(), (size_t)s.n, ctx->stream));
-    TRY(upload(C.csum, pcsum.data(), (size_t)s.n, ctx->stream));
-    TRY(upload(C.perm, order.data(), (size_t)s.n, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    TRY(upload(C.tiles, tiles.data(), (size_t)total, stream));
+    TRY(upload(C.toff, toff.data(), (size_t)ntiles + 1, stream));
+    TRY(upload(C.tlen, tlen.data(), (size_t)ntiles, stream));
+    TRY(upload(C.len, plen.data(), (size_t)s.n, stream));
+    TRY(upload(C.mag, pmag.data(), (size_t)s.n, stream));
+    TRY(upload(C.csum, pcsum.data(), (size_t)s.n, stream));
+    TRY(upload(C.perm, order.data(), (size_t)s.n, stream));
+    // level stream (bitgroups.h): every (id, t), t <= min(count, 15)
+    C.levels_ok = info.sorted_unique && info.counts_nonneg;
+    if (C.levels_ok) {
+        std::vector<int32_t> plv((size_t)s.n, 0);
+        for (int64_t p = 0; p < s.n; ++p) {
+            const int64_t c = order[(size_t)p];
+            const int32_t* ct = s.counts + s.start(c);
+            int32_t lv = 0;
+            for (int32_t j = 0; j < plen[(size_t)p]; ++j) lv += std::min(ct[j], siml::kBitMaxLevel);
+            plv[(size_t)p] = lv;
+        }
+        std::vector<int32_t> btlen((size_t)std::max<int64_t>(ntiles, 1), 0);
+        std::vector<int64_t> btoff((size_t)ntiles + 1, 0);
+        for (int64_t t = 0; t < ntiles; ++t) {
+            int32_t m = 0;
+            for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) m = std::max(m, plv[(size_t)p]);
+            m = (m + 7) & ~7;
+            btlen[(size_t)t] = m;
+            btoff[(size_t)t + 1] = btoff[(size_t)t] + (int64_t)m * kWave;
+        }
+        const int64_t btotal = btoff[(size_t)ntiles];
+        std::vector<int2> bt((size_t)std::max<int64_t>(btotal, 1), make_int2(0, 0));
+        parallel_for(ntiles, [&](int64_t a, int64_t b) {
+            for (int64_t t = a; t < b; ++t)
+                for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) {
+                    const int64_t c = order[(size_t)p];
+                    const int32_t* id = s.ids + s.start(c);
+                    const int32_t* ct = s.counts + s.start(c);
+                    int2* d = bt.data() + btoff[(size_t)t] + (p - t * kWave);
+                    size_t j = 0;
+                    for (int32_t k = 0; k < plen[(size_t)p]; ++k)
+                        for (int32_t lvl = 1; lvl <= std::min(ct[k], siml::kBitMaxLevel); ++lvl, ++j)
+                            d[j * kWave] = make_int2((int32_t)siml::level_key(id[k], (uint32_t)lvl), lvl << 28);
+                }
+        });
+        TRY(upload(C.btiles, bt.data(), (size_t)btotal, stream));
+        TRY(upload(C.btoff, btoff.data(), (size_t)ntiles + 1, stream));
+        TRY(upload(C.btlen, btlen.data(), (size_t)ntiles, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
     C.n = s.n;
     C.ntiles = ntiles;
     C.maxlen = info.maxlen;
@@ -1238,8 +1647,16 @@ int upload_cols(siml_ctx* ctx, const HostSet& s) {
     return SIML_OK;
 }
 
+int upload_cols(siml_ctx* ctx, const HostSet& s) {
+    TRY(set_device(ctx));
+    auto C = std::make_shared<ColSet>();
+    TRY(build_colset(ctx->stream, ctx->layout, s, *C));
+    ctx->qry = std::move(C);
+    return SIML_OK;
+}
+
 int check_rows(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, const char* what) {
-    if (ctx->ref.n < 0 || ctx->qry.n < 0)
+    if (ctx->ref.n < 0 || ctx->qry->n < 0)
         return fail(SIML_ESTATE, "%s: reference and query sets must both be set", what);
     if (rowbase < 0 || rowlimit > ctx->ref.n || rowlimit < rowbase)
         return fail(SIML_EINVAL, "%s: rows [%lld, %lld) outside [0, %lld)", what, (long long)rowbase,
@@ -1264,7 +1681,7 @@ struct RowPlan {
 
 void plan_rows(const siml_ctx* ctx, int64_t rb, int64_t rl, RowPlan& p) {
     const RowSet& R = ctx->ref;
-    const ColSet& C = ctx->qry;
+    const ColSet& C = *ctx->qry;
     p = RowPlan();
     if (ctx->policy == SIML_POLICY_MERGE || !C.sorted_unique || !C.counts_nonneg) return;
     p.all_merge = false;
@@ -1282,6 +1699,58 @@ int32_t* d_counts_of(siml_ctx* ctx) { return ctx->scratch.as<int32_t>(); }
 int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
     ctx->h_rowlist = rows;
     return upload(ctx->rowlist, ctx->h_rowlist.data(), ctx->h_rowlist.size(), ctx->stream);
+}
+
+// Bit-sliced plan of rows [rb, rl): the groups holding any of those rows,
+// the planes they need, and the rows left to the merge kernel.
+struct BitPlan {
+    bool use = false;
+    int nb = 8;
+    std::vector<int32_t> groups, merge;
+};
+
+void plan_bits(const siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p) {
+    const RowSet& R = ctx->ref;
+    p = BitPlan();
+    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok) return;
+    p.use = true;
+    for (size_t g = 0; g < R.h_bhdr.size(); ++g) {
+        bool hit = false;
+        for (int r = 0; r < R.h_bhdr[g].nrows && !hit; ++r) {
+            const int32_t row = R.h_brows[g * siml::kBitRowStride + (size_t)r];
+            hit = row >= rb && row < rl;
+        }
+        if (!hit) continue;
+        p.groups.push_back((int32_t)g);
+        p.nb = std::max(p.nb, R.h_bhdr[g].nbits);
+    }
+    auto a = std::lower_bound(R.bmerge_rows.begin(), R.bmerge_rows.end(), (int32_t)rb);
+    auto b = std::lower_bound(R.bmerge_rows.begin(), R.bmerge_rows.end(), rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
+    p.merge.assign(a, b);
+}
+
+int launch_check(const char* name);
+
+// Workgroups per bit-sliced launch: ~8 rounds of the 1024 that fit at once
+// (256 CUs x 4), so the last, partial round costs little.
+constexpr int64_t kBitsTargetWG = 8192;
+
+// Launches k_bits<NB, Epi> over the plan's groups with grid.y = gy.
+template <class Epi>
+int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_t rl, const Epi& epi,
+                const char* name) {
+    if (p.groups.empty()) return SIML_OK;
+    ctx->h_glist = p.groups;
+    TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
+    const dim3 grid((unsigned)p.groups.size(), gy);
+    const BitGroupArgs G = ctx->ref.bargs(ctx->glist.as<int32_t>());
+    if (p.nb == 8)
+        k_bits<8, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rb, rl,
+                                                              epi);
+    else
+        k_bits<16, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rb,
+                                                               rl, epi);
+    return launch_check(name);
 }
 
 // Grid for row-parallel kernels: grid.x = rows, grid.y splits the query
@@ -1366,12 +1835,13 @@ int siml_destroy(siml_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ref.release();
-    ctx->qry.release();
+    ctx->qry.reset();
     ctx->res.release();
     ctx->res2.release();
     ctx->scratch.release();
     ctx->rowlist.release();
     ctx->sort_tmp.release();
+    ctx->glist.release();
     ctx->csr_dev.release();
     for (auto& pr : ctx->pending_ev) {
         (void)hipEventDestroy(pr.first);
@@ -1441,10 +1911,75 @@ int siml_set_query_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* coun
     return upload_cols(ctx, s);
 }
 
+// ---------------------------------------------------------------------------
+// Resident databases shared by several handles
+// ---------------------------------------------------------------------------
+
+struct siml_db {
+    int device = 0;
+    std::shared_ptr<ColSet> set;
+};
+
+int siml_db_create(int device, const int32_t* ids, const int32_t* counts, const int32_t* mags, const int64_t* offsets,
+                   int64_t n, int layout, siml_db** out) {
+    if (!out) return fail(SIML_EINVAL, "siml_db_create: null output pointer");
+    *out = nullptr;
+    if (n > 0 && !offsets) return fail(SIML_EINVAL, "siml_db_create: null offsets");
+    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK)
+        return fail(SIML_EINVAL, "siml_db_create: unknown layout %d", layout);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) {
+        (void)hipGetLastError();
+        return fail(SIML_EDEVICE, "siml_db_create: device %d is not visible", device);
+    }
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HostSet s;
+    s.ids = ids;
+    s.counts = counts;
+    s.mags = mags;
+    s.offsets = offsets;
+    s.n = n;
+    auto set = std::make_shared<ColSet>();
+    const int rc = build_colset(st, layout, s, *set);
+    (void)hipStreamDestroy(st);
+    if (rc != SIML_OK) return rc;
+    siml_db* db = new siml_db();
+    db->device = device;
+    db->set = std::move(set);
+    *out = db;
+    return SIML_OK;
+}
+
+int siml_db_destroy(siml_db* db) {
+    if (!db) return SIML_OK;
+    (void)hipSetDevice(db->device);
+    delete db;  // the device buffers live on while a handle still uses them
+    return SIML_OK;
+}
+
+int siml_db_size(siml_db* db, int64_t* n) {
+    if (!db || !n) return fail(SIML_EINVAL, "null pointer");
+    *n = db->set->n;
+    return SIML_OK;
+}
+
+int siml_attach_db(siml_ctx* ctx, siml_db* db) {
+    if (!ctx || !db) return fail(SIML_EINVAL, "null pointer");
+    if (db->device != ctx->device)
+        return fail(SIML_EINVAL, "siml_attach_db: database on device %d, handle on device %d", db->device, ctx->device);
+    TRY(set_device(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    clear_result(ctx);
+    ctx->qry = db->set;
+    return SIML_OK;
+}
+
 int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery) {
     if (!ctx || !nref || !nquery) return fail(SIML_EINVAL, "null pointer");
     *nref = ctx->ref.n;
-    *nquery = ctx->qry.n;
+    *nquery = ctx->qry->n;
     return SIML_OK;
 }
 
@@ -1452,33 +1987,50 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleRows"));
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * nq, 1)));
     ctx->kind = SIML_RESULT_F32;
     ctx->rows = nr;
     ctx->cols = nq;
     if (nr == 0 || nq == 0) return SIML_OK;
+    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
+    BitPlan bp;
+    plan_bits(ctx, rowbase, rowlimit, bp);
+    if (bp.use) {
+        if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
+        TimedRegion tr(ctx, nr * nq);
+        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32{ctx->res.as<float>(), nq, rowbase}, "k_bits<f32>"));
+        if (!bp.merge.empty()) {
+            k_rows_f32<false><<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
+                                ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
+                                               ctx->res.as<float>(), nq);
+            TRY(launch_check("k_rows_f32<rowlist>"));
+        }
+        tr.done();
+        ctx->last_kernel = bp.merge.empty() ? "rows_bits" : (bp.groups.empty() ? "rows_merge" : "rows_bits+merge");
+        return SIML_OK;
+    }
     RowPlan plan;
     plan_rows(ctx, rowbase, rowlimit, plan);
     if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
-    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     TimedRegion tr(ctx, nr * nq);
     if (plan.all_merge) {
-        k_rows_f32<false><<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
-            ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, ctx->res.as<float>(), nq);
+        k_rows_f32<false><<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, ctx->res.as<float>(), nq);
         TRY(launch_check("k_rows_f32"));
         ctx->last_kernel = "rows_merge";
     } else {
         const int64_t ng = plan.g1 - plan.g0;
         if (ng > 0) {
-            k_hash_rows_f32<<<row_grid(ng, ctx->qry.ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit,
+            k_hash_rows_f32<<<row_grid(ng, ctx->qry->ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit,
                 ctx->res.as<float>(), nq);
             TRY(launch_check("k_hash_rows_f32"));
         }
         if (!plan.merge.empty()) {
-            k_rows_f32<false><<<row_grid((int64_t)plan.merge.size(), ctx->qry.ntiles), dim3(kBlock), lds,
-                                ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase,
+            k_rows_f32<false><<<row_grid((int64_t)plan.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
+                                ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
                                                ctx->res.as<float>(), nq);
             TRY(launch_check("k_rows_f32<rowlist>"));
         }
@@ -1492,7 +2044,7 @@ int siml_single_row_async(siml_ctx* ctx, int64_t row) {
     TRY(check_rows(ctx, row, row + 1, "getTanimotoRow"));
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nq = ctx->qry.n;
+    const int64_t nq = ctx->qry->n;
     TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nq, 1)));
     ctx->kind = SIML_RESULT_F32;
     ctx->rows = 1;
@@ -1500,8 +2052,8 @@ int siml_single_row_async(siml_ctx* ctx, int64_t row) {
     if (nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, row, row + 1));
     TimedRegion tr(ctx, nq);
-    k_rows_f32<true><<<row_grid(1, ctx->qry.ntiles, 2048), dim3(kBlock), lds, ctx->stream>>>(
-        ctx->ref.args(), ctx->qry.args(), nullptr, row, ctx->res.as<float>(), nq);
+    k_rows_f32<true><<<row_grid(1, ctx->qry->ntiles, 2048), dim3(kBlock), lds, ctx->stream>>>(
+        ctx->ref.args(), ctx->qry->args(), nullptr, row, ctx->res.as<float>(), nq);
     TRY(launch_check("k_rows_f32<single>"));
     tr.done();
     ctx->last_kernel = "single_row_merge";
@@ -1512,33 +2064,49 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleHistogrammedRows"));
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(nr * 101, 1)));
     ctx->kind = SIML_RESULT_I32;
     ctx->rows = nr;
     ctx->cols = 101;
     TRY(zero_fill_i32(ctx, ctx->res.as<int32_t>(), nr * 101));
     if (nr == 0 || nq == 0) return SIML_OK;
+    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
+    int32_t* hist = ctx->res.as<int32_t>();
+    BitPlan bp;
+    plan_bits(ctx, rowbase, rowlimit, bp);
+    if (bp.use) {
+        if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
+        TimedRegion tr(ctx, nr * nq);
+        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiHist101{hist, rowbase}, "k_bits<hist101>"));
+        if (!bp.merge.empty()) {
+            k_hist_rows<<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+            TRY(launch_check("k_hist_rows<rowlist>"));
+        }
+        tr.done();
+        ctx->last_kernel = "hist_rows_bits";
+        return SIML_OK;
+    }
     RowPlan plan;
     plan_rows(ctx, rowbase, rowlimit, plan);
     if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
-    const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    int32_t* hist = ctx->res.as<int32_t>();
     TimedRegion tr(ctx, nr * nq);
     if (plan.all_merge) {
-        k_hist_rows<<<row_grid(nr, ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
-            ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, hist);
+        k_hist_rows<<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, hist);
         TRY(launch_check("k_hist_rows"));
     } else {
         const int64_t ng = plan.g1 - plan.g0;
         if (ng > 0) {
-            k_hash_hist_rows<<<row_grid(ng, ctx->qry.ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, hist);
+            k_hash_hist_rows<<<row_grid(ng, ctx->qry->ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, hist);
             TRY(launch_check("k_hash_hist_rows"));
         }
         if (!plan.merge.empty()) {
-            k_hist_rows<<<row_grid((int64_t)plan.merge.size(), ctx->qry.ntiles), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+            k_hist_rows<<<row_grid((int64_t)plan.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
             TRY(launch_check("k_hist_rows<rowlist>"));
         }
     }
@@ -1551,7 +2119,7 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     TRY(check_rows(ctx, rowbase, rowlimit, "getNeighbors"));
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     ctx->kind = SIML_RESULT_CSR;
     ctx->rows = nr;
     ctx->cols = nq;
@@ -1560,22 +2128,49 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
         TRY(ctx->res.reserve(256));
         return SIML_OK;
     }
+    BitPlan bp;
+    plan_bits(ctx, rowbase, rowlimit, bp);
     RowPlan plan;
-    plan_rows(ctx, rowbase, rowlimit, plan);
-    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
+    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
+    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    const dim3 grid = row_grid(nr, ctx->qry.ntiles, 8192, int64_t(1) << 24);
+    dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
+    if (bp.use) {
+        // bit-sliced groups hold up to 28 rows: size grid.y by groups
+        const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        grid.y = (unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, (int64_t(1) << 24) / (nr * (kBlock / kWave))));
+    }
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t ng = plan.g1 - plan.g0;
     const dim3 hgrid((unsigned)std::max<int64_t>(ng, 1), grid.y);
-    const dim3 mgrid((unsigned)std::max<size_t>(plan.merge.size(), 1), grid.y);
+    const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y);
     auto launch = [&](bool fill, const int64_t* offs, int32_t* idx) -> int {
+        if (bp.use) {
+            if (fill)
+                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase},
+                                "k_bits<nbr fill>"));
+            else
+                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
+                                EpiNbr<false>{lo, hi, d_counts_of(ctx), nullptr, nullptr, rowbase}, "k_bits<nbr count>"));
+            if (!mrows.empty()) {
+                if (fill)
+                    k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
+                        ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, nullptr, offs, idx);
+                else
+                    k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
+                        ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, d_counts_of(ctx),
+                        nullptr, nullptr);
+                TRY(launch_check("k_neighbors<rowlist>"));
+            }
+            return SIML_OK;
+        }
         if (plan.all_merge) {
             if (fill)
-                k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr,
+                k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
                                                                             rowbase, lo, hi, nullptr, offs, idx);
             else
-                k_neighbors<false><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr,
+                k_neighbors<false><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
                                                                              rowbase, lo, hi, d_counts_of(ctx), nullptr,
                                                                              nullptr);
             return launch_check("k_neighbors");
@@ -1583,21 +2178,21 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
         if (ng > 0) {
             if (fill)
                 k_hash_neighbors<true><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
-                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, lo, hi,
+                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
                     nullptr, offs, idx);
             else
                 k_hash_neighbors<false><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
-                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, lo, hi,
+                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
                     d_counts_of(ctx), nullptr, nullptr);
             TRY(launch_check("k_hash_neighbors"));
         }
         if (!plan.merge.empty()) {
             if (fill)
-                k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(),
+                k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(),
                                                                              ctx->rowlist.as<int32_t>(), rowbase, lo,
                                                                              hi, nullptr, offs, idx);
             else
-                k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(),
+                k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(),
                                                                               ctx->rowlist.as<int32_t>(), rowbase, lo,
                                                                               hi, d_counts_of(ctx), nullptr, nullptr);
             TRY(launch_check("k_neighbors<rowlist>"));
@@ -1628,7 +2223,7 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     HIP_TRY(hipMemcpyAsync(d_offs, h_offs.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(acc, 1)));
     TRY(launch(true, d_offs, ctx->res.as<int32_t>()));
-    if (!ctx->qry.identity && acc > 0) {
+    if (!ctx->qry->identity && acc > 0) {
         // indices were written in storage order: sort each row ascending
         TRY(upload(ctx->csr_dev, ctx->csr_rows.data(), (size_t)nr + 1, ctx->stream));
         TRY(ctx->res2.reserve(sizeof(int32_t) * (size_t)acc));
@@ -1643,29 +2238,33 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     tr.done();
     // h_offs and csr_rows must outlive the async copies above.
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->last_kernel = plan.all_merge ? "neighbors_merge" : "neighbors_hash";
+    ctx->last_kernel = bp.use ? "neighbors_bits" : (plan.all_merge ? "neighbors_merge" : "neighbors_hash");
     return SIML_OK;
 }
 
 int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getTopK"));
     if (k < 1 || k > SIML_MAX_TOPK) return fail(SIML_EINVAL, "getTopK: k=%d outside [1, %d]", k, SIML_MAX_TOPK);
-    if (ctx->qry.n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "getTopK: more than 2^31-1 query molecules");
+    if (ctx->qry->n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "getTopK: more than 2^31-1 query molecules");
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(nr * k, 1)));
     TRY(ctx->res2.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * k, 1)));
     ctx->kind = SIML_RESULT_TOPK;
     ctx->rows = nr;
     ctx->cols = k;
     if (nr == 0) return SIML_OK;
+    BitPlan bp;
+    plan_bits(ctx, rowbase, rowlimit, bp);
     RowPlan plan;
-    plan_rows(ctx, rowbase, rowlimit, plan);
-    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
+    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
+    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const int cap = (int)(((k + kWave) + kWave - 1) / kWave * kWave);
-    dim3 grid = row_grid(nr, ctx->qry.ntiles, 2048);
+    dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
+                         bp.use ? kBitsTargetWG : 2048);
     grid.y = std::min<unsigned>(grid.y, 64u);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
@@ -1675,20 +2274,27 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes);
     const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
-    if (plan.all_merge) {
-        k_topk<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, k, cap,
+    if (bp.use) {
+        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase}, "k_bits<topk>"));
+        if (!mrows.empty()) {
+            k_topk<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
+            TRY(launch_check("k_topk<rowlist>"));
+        }
+    } else if (plan.all_merge) {
+        k_topk<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, k, cap,
                                                          d_buf, d_cnt);
         TRY(launch_check("k_topk"));
     } else {
         if (ng > 0) {
             k_hash_topk<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, k, cap,
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, k, cap,
                 d_buf, d_cnt);
             TRY(launch_check("k_hash_topk"));
         }
         if (!plan.merge.empty()) {
             k_topk<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
             TRY(launch_check("k_topk<rowlist>"));
         }
     }
@@ -1696,7 +2302,7 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
                                                                      ctx->res.as<int32_t>(), ctx->res2.as<float>());
     TRY(launch_check("k_topk_final"));
     tr.done();
-    ctx->last_kernel = plan.all_merge ? "topk_merge" : "topk_hash";
+    ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
     return SIML_OK;
 }
 
@@ -1704,7 +2310,7 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getGlobalHistogram"));
     TRY(set_device(ctx));
     clear_result(ctx);
-    const int64_t nr = rowlimit - rowbase, nq = ctx->qry.n;
+    const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     TRY(ctx->res.reserve(sizeof(uint64_t) * 1000));
     ctx->kind = SIML_RESULT_I64;
     ctx->rows = 1;
@@ -1713,33 +2319,44 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
     TRY(launch_check("k_fill"));
     if (nr == 0 || nq == 0) return SIML_OK;
+    BitPlan bp;
+    plan_bits(ctx, rowbase, rowlimit, bp);
     RowPlan plan;
-    plan_rows(ctx, rowbase, rowlimit, plan);
-    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
+    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
+    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    dim3 grid = row_grid(nr, ctx->qry.ntiles, 2048);
-    // one workgroup counts at most 8 rows x its columns in uint32 LDS bins
-    const int64_t min_gy = (8 * kWave * ctx->qry.ntiles) / (int64_t(1) << 31) + 1;
+    dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
+                         bp.use ? kBitsTargetWG : 2048);
+    // one workgroup counts at most 28 rows x its columns in uint32 LDS bins
+    const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
     const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
-    if (plan.all_merge) {
-        k_hist1000<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry.args(), nullptr, rowbase, hist);
+    if (bp.use) {
+        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiHist1000{hist}, "k_bits<hist1000>"));
+        if (!mrows.empty()) {
+            k_hist1000<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+            TRY(launch_check("k_hist1000<rowlist>"));
+        }
+    } else if (plan.all_merge) {
+        k_hist1000<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, hist);
         TRY(launch_check("k_hist1000"));
     } else {
         if (ng > 0) {
             k_hash_hist1000<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry.args(), rowbase, rowlimit, hist);
+                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, hist);
             TRY(launch_check("k_hash_hist1000"));
         }
         if (!plan.merge.empty()) {
             k_hist1000<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry.args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
             TRY(launch_check("k_hist1000<rowlist>"));
         }
     }
     tr.done();
-    ctx->last_kernel = plan.all_merge ? "hist1000_merge" : "hist1000_hash";
+    ctx->last_kernel = bp.use ? "hist1000_bits" : (plan.all_merge ? "hist1000_merge" : "hist1000_hash");
     return SIML_OK;
 }
 
@@ -1854,7 +2471,8 @@ int siml_retrieve_i64(siml_ctx* ctx, int64_t* host) {
 
 int siml_set_policy(siml_ctx* ctx, int policy) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE) return fail(SIML_EINVAL, "unknown policy %d", policy);
+    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE && policy != SIML_POLICY_HASH8)
+        return fail(SIML_EINVAL, "unknown policy %d", policy);
     ctx->policy = policy;
     return SIML_OK;
 }

@@ -257,7 +257,11 @@ def test_ragged_sets_equal_padded(gpu):
     assert_same_f32(got, oracle.tanimoto_block(tuple(x[:300] for x in pad), pad, 0))
 
 
-@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+KERNEL_NAMES = {_lib.POLICY_AUTO: "rows_bits", _lib.POLICY_HASH8: "rows_hash", _lib.POLICY_MERGE: "rows_merge"}
+POLICIES = [_lib.POLICY_AUTO, _lib.POLICY_HASH8, _lib.POLICY_MERGE]
+
+
+@pytest.mark.parametrize("policy", POLICIES)
 def test_policies_bit_identical(demo_lines, policy):
     g = GPULingo(0)
     g._h.call("siml_set_policy", policy)
@@ -268,14 +272,15 @@ def test_policies_bit_identical(demo_lines, policy):
     assert_same_f32(g.getMultipleRows(0, 777), want)
     assert_same_f32(g.getMultipleRows(5, 13), want[5:13])     # range inside one group
     assert_same_f32(g.getMultipleRows(250, 251), want[250:251])
-    name = g._h.last_kernel()
-    assert name == ("rows_hash" if policy == _lib.POLICY_AUTO else "rows_merge")
+    assert g._h.last_kernel() == KERNEL_NAMES[policy]
 
 
-def test_mixed_hash_and_merge_rows(demo_lines):
-    """Rows that cannot use the hashed kernel (count > 65535, unsorted ids,
-    too many ids) are served by the merge kernel in the same call."""
+@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_HASH8])
+def test_mixed_hash_and_merge_rows(demo_lines, policy):
+    """Rows that cannot use the multi-row kernels (count > 65535, unsorted
+    ids, too many ids) are served by the merge kernel in the same call."""
     g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
     ref = [x.copy() for x in oracle.pack(demo_lines[:100])]
     L, C, n, m = ref
     C[3, 0] = 70000                      # count beyond u16
@@ -295,11 +300,11 @@ def test_mixed_hash_and_merge_rows(demo_lines):
     qry = oracle.pack(demo_lines[:2000])
     _set(g, ref, qry)
     got = g.getMultipleRows(0, 100)
-    assert g._h.last_kernel() == "rows_hash+merge"
+    assert g._h.last_kernel() == ("rows_bits+merge" if policy == _lib.POLICY_AUTO else "rows_hash+merge")
     assert_same_f32(got, oracle.tanimoto_block(ref, qry, nthreads=0))
 
 
-@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+@pytest.mark.parametrize("policy", POLICIES)
 @pytest.mark.parametrize("k", [1, 8, 100, 256])
 def test_topk(demo_lines, policy, k):
     g = GPULingo(0)
@@ -325,7 +330,7 @@ def test_topk_more_than_columns_and_ties(demo_lines):
     assert np.all(idx[:, 67:] == -1)
 
 
-@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+@pytest.mark.parametrize("policy", POLICIES)
 def test_global_histogram(demo_lines, demo_set, policy):
     g = GPULingo(0)
     g._h.call("siml_set_policy", policy)
@@ -338,7 +343,7 @@ def test_global_histogram(demo_lines, demo_set, policy):
     assert np.array_equal(h2, oracle.hist1000(tuple(x[100:101] for x in ref), demo_set))
 
 
-@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_MERGE])
+@pytest.mark.parametrize("policy", POLICIES)
 def test_hist_and_neighbors_both_policies(demo_lines, demo_set, policy):
     g = GPULingo(0)
     g._h.call("siml_set_policy", policy)
@@ -371,3 +376,25 @@ def test_layouts_bit_identical(demo_lines, demo_set, layout):
     assert np.array_equal(g.getGlobalHistogram(0, 300), oracle.hist1000(ref, qry))
     g._h.call("siml_set_policy", _lib.POLICY_MERGE)
     assert_same_f32(g.getMultipleRows(10, 40), oracle.tanimoto_block(tuple(x[10:40] for x in ref), qry, 0))
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_repeated_lingos(demo_lines, policy):
+    """Multiplicities above 15 (long carbon chains): such rows take the
+    merge kernel, such columns have their levels above 15 dropped from the
+    level stream; every output stays bit-exact."""
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    chains = ["C" * n for n in (4, 5, 18, 19, 40, 80)] + ["CC(C)" + "C" * n + "O" for n in (10, 17, 30)]
+    ref = oracle.pack(chains + demo_lines[:60])
+    qry = oracle.pack(demo_lines[100:700] + chains)
+    _set(g, ref, qry)
+    n = ref[0].shape[0]
+    assert_same_f32(g.getMultipleRows(0, n), oracle.tanimoto_block(ref, qry, nthreads=0))
+    assert np.array_equal(g.getMultipleHistogrammedRows(0, n), oracle.hist_rows101(ref, qry))
+    ti, tt = g.getTopK(0, n, 20)
+    oi, ot = oracle.topk(ref, qry, 20)
+    assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
+    offs, idx = g.getNeighborsCSR(0, n, 0.3)
+    lists, counts = oracle.neighbors(ref, qry, 0.3)
+    assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
