@@ -1,7 +1,7 @@
 """Benchmark: Tanimoto pairs/s of the LINGO similarity path on one MI355X.
 
-Workload (one step): the full 32768 x 32768 float32 Tanimoto matrix of a
-seeded synthetic set (strings U[20,120] over 'CNOSPFcnos()=#[]@+-123456',
+Default workload (one step): the full 32768 x 32768 float32 Tanimoto matrix
+of a seeded synthetic set (strings U[20,120] over 'CNOSPFcnos()=#[]@+-123456',
 seed 2) against itself, rows = columns, via getMultipleRows(0, 32768).  The
 packed set is resident in HBM and the 4 GiB result stays in HBM; no host
 copies are timed.
@@ -12,6 +12,21 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 against the VALU peak 256 CU * 64 lanes * 2.4 GHz
   cpu_baseline  the CPU oracle (C restatement of _CPULingo.c, OpenMP over
                 rows) timed on a bounded sample of the same workload
+
+Other workloads (--workload), each one JSON line:
+  batched    1M-molecule resident database (lognormal lengths, median 45,
+             sigma 0.35, clipped [8, 256], seed 3); 2000 query sets with
+             geometric(1/128) sizes clipped [1, 4096]; top-100 per query
+             molecule.  "plain" runs one getTopK per set, "batched" runs all
+             sets as one launch sequence; both end with results on the host.
+  threshold  10M-molecule database (same law, seed 4) x 1024 queries that
+             are near duplicates of database molecules; complete CSR
+             neighbour lists for T >= 0.7 and the 1000-bin int64 histogram of
+             all 1.024e10 similarities.
+  serve      10M-molecule database; Poisson arrivals of query batches (sizes
+             geometric(1/32) clipped [1, 512], lognormal lengths), half top-50
+             and half threshold 0.8, continuous batching over 4 slots;
+             latency p50/p99 and throughput per arrival rate.
 """
 import argparse
 import json
@@ -70,28 +85,25 @@ def copy_calibration(nbytes):
     return 0
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="only run a device-to-device copy of this many bytes (counter calibration)")
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=N_ALLPAIRS, help="molecules (rows = columns)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--policy", choices=["auto", "hash8", "merge"], default="auto",
-                    help="auto: bit-sliced 28-row kernels; hash8: 8-row hashed kernels; "
-                         "merge: one-row-per-workgroup merge kernel")
-    ap.add_argument("--layout", choices=["block", "natural"], default="block",
-                    help="query storage order: length-sorted 256-column blocks, or the caller's order")
-    args = ap.parse_args()
-    if args.copy_bytes:
-        return copy_calibration(args.copy_bytes)
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-        return 2
+def configure(g, args):
+    from siml_amd import _lib
+    h = g._h
+    h.call("siml_set_layout", _lib.LAYOUT_NATURAL if args.layout == "natural" else _lib.LAYOUT_BLOCK)
+    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "hash8": _lib.POLICY_HASH8,
+                               "merge": _lib.POLICY_MERGE}[args.policy])
 
+
+def base_record(args, metric, value, unit, wall, config, dtype="int32 intersections, float32 division"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded SMILES-like strings, packed with the native packer)", "config": config}
+
+
+# ---------------------------------------------------------------------------
+# all-pairs 32768 x 32768 (default)
+# ---------------------------------------------------------------------------
+def workload_allpairs(args):
     from siml_amd import synth
     from siml_amd.GPULingo import GPULingo
 
@@ -99,12 +111,9 @@ def main():
     log(f"packing {n} synthetic strings (U[20,120], seed 2)")
     s = synth.packed_padded(n, 20, 120, seed=2)
     lens = s[2].ravel().astype(np.int64)
-    from siml_amd import _lib
     g = GPULingo(0)
     h = g._h
-    h.call("siml_set_layout", _lib.LAYOUT_NATURAL if args.layout == "natural" else _lib.LAYOUT_BLOCK)
-    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "hash8": _lib.POLICY_HASH8,
-                               "merge": _lib.POLICY_MERGE}[args.policy])
+    configure(g, args)
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
 
@@ -130,32 +139,300 @@ def main():
     ops = 4.0 * (2 * n * lens.sum())            # sum over pairs of 4*(La+Lb), rows == columns
     bytes_alg = 4.0 * n * n + 8.0 * 2 * lens.sum() + 16.0 * n
     achieved = ops / kernel_s
-    rec = {
-        "metric": "Tanimoto pairs/s (1 GPU, bit-exact)",
-        "value": value,
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32 merge, float32 division",
-        "data": "synthetic (seeded SMILES-like strings, packed with the native packer)",
-        "config": {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
-                   "rows": n, "columns": n, "seed": 2, "policy": args.policy,
-                   "layout": args.layout},
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact)", value, "pairs/s", wall,
+                      {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
+                       "rows": n, "columns": n, "seed": 2, "policy": args.policy, "layout": args.layout})
+    rec.update({
         "kernel": kernel,
         "kernel_ms_mean": kernel_s * 1e3,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": P_VALU / 1e12,
-                     "unit": "Tops/s (int32 lane-ops)", "frac": achieved / P_VALU, "traffic": None,
+                     "unit": "Tops/s (int32 lane-ops)", "frac": achieved / P_VALU, "traffic": args.traffic,
                      "alg_ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
                      "hbm_frac": bytes_alg / kernel_s / B_HBM},
-    }
+    })
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")
         rec["cpu_baseline"] = cpu_baseline(s, args.cpu_budget)
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# batched uneven query sets, top-k
+# ---------------------------------------------------------------------------
+def lognormal_set(n, seed):
+    from siml_amd import synth
+    return synth.packed_ragged(n, 8, 256, seed=seed, dist="lognormal", median=45.0, sigma=0.35)
+
+
+def oracle_topk_ragged(q, db, rows, k, chunk=1 << 20):
+    """Oracle top-k of query rows against a ragged database, column chunks
+    merged with the (T desc, index asc) order."""
+    import oracle
+    from siml_amd import synth
+    Q = synth.padded_columns(*q, 0, len(q[2]) - 1)
+    Q = tuple(x[rows] for x in Q)
+    n = len(db[2]) - 1
+    best_i = np.zeros((len(rows), 0), np.int32)
+    best_t = np.zeros((len(rows), 0), np.float32)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        D = synth.padded_columns(*db, c0, c1)
+        oi, ot = oracle.topk(Q, D, min(k, c1 - c0), nthreads=min(os.cpu_count() or 1, 16))
+        best_i = np.concatenate([best_i, oi + c0], axis=1)
+        best_t = np.concatenate([best_t, ot], axis=1)
+        order = np.lexsort((best_i, -best_t.astype(np.float64)), axis=1)[:, :k]
+        best_i = np.take_along_axis(best_i, order, 1)
+        best_t = np.take_along_axis(best_t, order, 1)
+    return best_i, best_t
+
+
+def workload_batched(args):
+    from siml_amd import synth
+    from siml_amd.GPULingo import GPULingo, LingoDB
+
+    ndb = args.db or 1_000_000
+    nsets, k = args.sets, 100
+    log(f"packing the {ndb}-molecule database (lognormal, seed 3)")
+    db_set = lognormal_set(ndb, 3)
+    sizes = synth.geometric_sizes(nsets, 1.0 / 128, 1, 4096, seed=3)
+    nq = int(sizes.sum())
+    log(f"packing {nq} query molecules in {nsets} sets")
+    q = lognormal_set(nq, 33)
+    bounds = np.zeros(nsets + 1, np.int64)
+    np.cumsum(sizes, out=bounds[1:])
+    sets = [synth.ragged_slice(*q, int(bounds[i]), int(bounds[i + 1])) for i in range(nsets)]
+    db = LingoDB(*db_set)
+    g = GPULingo(0)
+    configure(g, args)
+    g.attach_db(db)
+    h = g._h
+    # warm-up: one small set through the whole path
+    g.set_refsmiles_ragged(*sets[0])
+    g.getTopK(0, int(sizes[0]), k)
+    pairs = nq * ndb
+
+    h.call("siml_set_timing", 1)
+    h.kernel_stats(reset=True)
+    log("plain: one getTopK per query set")
+    t0 = time.perf_counter()
+    plain = []
+    for s in sets:
+        g.set_refsmiles_ragged(*s)
+        plain.append(g.getTopK(0, len(s[2]) - 1, k))
+    t_plain = time.perf_counter() - t0
+    kms_plain, n_plain, _ = h.kernel_stats(reset=True)
+    log(f"plain: {t_plain:.2f} s wall, kernels {kms_plain:.0f} ms over {n_plain} launches")
+
+    log("batched: all sets as one launch sequence")
+    t0 = time.perf_counter()
+    g.set_refsmiles_ragged(*q)
+    bi, bt = g.getTopK(0, nq, k)
+    t_batched = time.perf_counter() - t0
+    kms_b, n_b, _ = h.kernel_stats(reset=True)
+    kernel = h.last_kernel()
+    log(f"batched: {t_batched:.2f} s wall, kernels {kms_b:.0f} ms over {n_b} launches ({kernel})")
+
+    pi = np.concatenate([p[0] for p in plain])
+    pt = np.concatenate([p[1] for p in plain])
+    same = bool(np.array_equal(pi, bi) and np.array_equal(pt.view(np.int32), bt.view(np.int32)))
+    parity = None
+    if not args.no_cpu:
+        rng = np.random.default_rng(7)
+        rows = np.sort(rng.choice(nq, size=min(6, nq), replace=False))
+        log(f"oracle top-{k} of query rows {rows.tolist()} over the whole database")
+        oi, ot = oracle_topk_ragged(q, db_set, rows, k)
+        parity = bool(np.array_equal(oi, bi[rows]) and np.array_equal(ot.view(np.int32), bt[rows].view(np.int32)))
+    lens_db = np.diff(db_set[2]).astype(np.int64)
+    lens_q = np.diff(q[2]).astype(np.int64)
+    ops = 4.0 * (lens_q.sum() * ndb + lens_db.sum() * nq)
+    kernel_s = kms_b / 1e3
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), batched top-k", pairs / t_batched, "pairs/s",
+                      t_batched, {"workload": f"batched uneven query sets: {nsets} sets (geometric 1/128, clipped "
+                                              f"[1, 4096], {nq} molecules) x {ndb}-molecule resident database, "
+                                              f"top-{k}", "sets": nsets, "queries": nq, "database": ndb, "k": k,
+                                  "seed": 3})
+    rec.update({
+        "steps": 1, "warmup": 1,
+        "plain": {"wall_s": t_plain, "pairs_per_s": pairs / t_plain, "kernel_ms": kms_plain, "launches": n_plain},
+        "batched": {"wall_s": t_batched, "pairs_per_s": pairs / t_batched, "kernel_ms": kms_b, "launches": n_b},
+        "speedup_wall": t_plain / t_batched,
+        "speedup_kernel": kms_plain / max(kms_b, 1e-9),
+        "plain_equals_batched": same,
+        "oracle_rows_match": parity,
+        "kernel": kernel,
+        "roofline": {"bound": "valu", "achieved": ops / kernel_s / 1e12, "peak": P_VALU / 1e12,
+                     "unit": "Tops/s (int32 lane-ops)", "frac": ops / kernel_s / P_VALU, "traffic": None},
+    })
+    db.close()
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# threshold neighbours + global histogram
+# ---------------------------------------------------------------------------
+def near_duplicate_queries(nq, db_buf, db_offs, seed):
+    from siml_amd import compiler
+    rng = np.random.default_rng(seed)
+    picks = rng.choice(len(db_offs) - 1, size=nq, replace=False)
+    from siml_amd import synth
+    buf, offs = synth.mutated_strings(db_buf, db_offs, picks, 2, seed)
+    return compiler.pack_ragged_buffer(buf, offs)
+
+
+def workload_threshold(args):
+    from siml_amd import compiler, synth
+    from siml_amd.GPULingo import GPULingo, LingoDB
+    import oracle
+
+    ndb, nq, lo = args.db or 10_000_000, 1024, 0.7
+    log(f"packing the {ndb}-molecule database (lognormal, seed 4)")
+    buf, offs = synth.random_strings(ndb, 8, 256, seed=4, dist="lognormal", median=45.0, sigma=0.35)
+    db_set = compiler.pack_ragged_buffer(buf, offs)
+    q = near_duplicate_queries(nq, buf, offs, 44)
+    del buf
+    db = LingoDB(*db_set)
+    g = GPULingo(0)
+    configure(g, args)
+    g.attach_db(db)
+    g.set_refsmiles_ragged(*q)
+    h = g._h
+    g.getNeighborsCSR(0, 32, lo)
+    g.getGlobalHistogram(0, 32)
+    pairs = nq * ndb
+    h.call("siml_set_timing", 1)
+    h.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        roff, cols = g.getNeighborsCSR(0, nq, lo)
+    t_csr = (time.perf_counter() - t0) / args.steps
+    kms_csr, n_csr, _ = h.kernel_stats(reset=True)
+    k_csr = h.last_kernel()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hist = g.getGlobalHistogram(0, nq)
+    t_hist = (time.perf_counter() - t0) / args.steps
+    kms_hist, n_hist, _ = h.kernel_stats(reset=True)
+    k_hist = h.last_kernel()
+    log(f"CSR: {t_csr * 1e3:.1f} ms ({int(roff[-1])} neighbours); hist: {t_hist * 1e3:.1f} ms")
+    checks = {"hist_total": int(hist.sum()), "hist_total_ok": int(hist.sum()) == pairs, "nnz": int(roff[-1])}
+    if not args.no_cpu:
+        rows = [0, 1, nq // 2, nq - 1]
+        log(f"oracle neighbours and histogram of query rows {rows} over the whole database")
+        Q = tuple(x[rows] for x in synth.padded_columns(*q, 0, nq))
+        nb = [[] for _ in rows]
+        oh = np.zeros(1000, np.int64)
+        for c0 in range(0, ndb, 1 << 20):
+            c1 = min(ndb, c0 + (1 << 20))
+            D = synth.padded_columns(*db_set, c0, c1)
+            lst, _ = oracle.neighbors(Q, D, lo, nthreads=min(os.cpu_count() or 1, 16))
+            for i, l in enumerate(lst):
+                nb[i].append(l + c0)
+            oh += oracle.hist1000(Q, D)
+        ok_nb = all(np.array_equal(np.concatenate(nb[i]), cols[roff[r]:roff[r + 1]]) for i, r in enumerate(rows))
+        gh = np.zeros(1000, np.int64)
+        for r in rows:
+            gh += g.getGlobalHistogram(r, r + 1)
+        checks.update({"oracle_rows": rows, "neighbours_match": bool(ok_nb), "hist_rows_match": bool(np.array_equal(gh, oh))})
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), threshold CSR", pairs / t_csr, "pairs/s", t_csr * args.steps,
+                      {"workload": f"threshold neighbours T >= {lo} + 1000-bin histogram: {nq} near-duplicate "
+                                   f"queries x {ndb}-molecule resident database", "queries": nq, "database": ndb,
+                       "seed": 4})
+    rec.update({"csr": {"wall_ms": t_csr * 1e3, "pairs_per_s": pairs / t_csr, "kernel_ms": kms_csr / args.steps,
+                        "kernel": k_csr},
+                "hist1000": {"wall_ms": t_hist * 1e3, "pairs_per_s": pairs / t_hist,
+                             "kernel_ms": kms_hist / args.steps, "kernel": k_hist},
+                "checks": checks})
+    db.close()
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# serving loop
+# ---------------------------------------------------------------------------
+def workload_serve(args):
+    from siml_amd import synth
+    from siml_amd.GPULingo import LingoDB
+    from siml_amd.serving import Request, Server, run_poisson
+
+    ndb = args.db or 10_000_000
+    log(f"packing the {ndb}-molecule database (lognormal, seed 5)")
+    db_set = lognormal_set(ndb, 5)
+    db = LingoDB(*db_set)
+    nreq = args.requests
+    sizes = synth.geometric_sizes(nreq, 1.0 / 32, 1, 512, seed=5)
+    pool = lognormal_set(int(sizes.sum()), 55)
+    bounds = np.zeros(nreq + 1, np.int64)
+    np.cumsum(sizes, out=bounds[1:])
+    rates = [float(r) for r in args.rates.split(",")]
+    results = []
+    with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30) as srv:
+        # warm-up: one request of each kind
+        for op, prm in (("topk", 50), ("threshold", 0.8)):
+            s = synth.ragged_slice(*pool, 0, 4)
+            srv.submit(Request(-1, *s, op, prm)).event.wait()
+        for rate in rates:
+            reqs = []
+            for i in range(nreq):
+                s = synth.ragged_slice(*pool, int(bounds[i]), int(bounds[i + 1]))
+                op, prm = ("topk", 50) if i % 2 == 0 else ("threshold", 0.8)
+                reqs.append(Request(i, *s, op, prm))
+            b0 = srv.batches
+            wall, lat = run_poisson(srv, reqs, rate, seed=int(rate * 1000))
+            errs = [r.error for r in reqs if r.error is not None]
+            if errs:
+                raise errs[0]
+            pairs = int(sizes.sum()) * ndb
+            results.append({"rate_per_s": rate, "requests": nreq, "wall_s": wall, "batches": srv.batches - b0,
+                            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                            "max_ms": float(lat.max() * 1e3), "pairs_per_s": pairs / wall,
+                            "requests_per_s": nreq / wall})
+            log(f"rate {rate}/s: p50 {results[-1]['p50_ms']:.1f} ms, p99 {results[-1]['p99_ms']:.1f} ms, "
+                f"{results[-1]['pairs_per_s']:.3g} pairs/s")
+    best = max(results, key=lambda r: r["pairs_per_s"])
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), serving loop", best["pairs_per_s"], "pairs/s",
+                      best["wall_s"], {"workload": f"serving: {ndb}-molecule resident database, Poisson arrivals "
+                                                   f"of query batches (geometric 1/32 sizes clipped [1, 512]), "
+                                                   f"half top-50, half threshold 0.8", "database": ndb,
+                                       "requests": nreq, "slots": args.slots, "seed": 5})
+    rec.update({"steps": 1, "warmup": 1, "rates": results})
+    db.close()
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--workload", choices=["allpairs", "batched", "threshold", "serve"], default="allpairs")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=N_ALLPAIRS, help="all-pairs: molecules (rows = columns)")
+    ap.add_argument("--db", type=int, default=0, help="database molecules (batched/threshold/serve)")
+    ap.add_argument("--sets", type=int, default=2000, help="batched: query sets")
+    ap.add_argument("--requests", type=int, default=400, help="serve: requests per arrival rate")
+    ap.add_argument("--rates", default="10,20,40", help="serve: Poisson arrival rates, batches/s")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent comparators")
+    ap.add_argument("--max-rows", type=int, default=1024, help="serve: query molecules per launch")
+    ap.add_argument("--mem-budget", type=int, default=16, help="serve: admission budget, GiB")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="all-pairs: HBM bytes per launch from counter runs, reported as roofline.traffic")
+    ap.add_argument("--policy", choices=["auto", "hash8", "merge"], default="auto",
+                    help="auto: bit-sliced 28-row kernels; hash8: 8-row hashed kernels; "
+                         "merge: one-row-per-workgroup merge kernel")
+    ap.add_argument("--layout", choices=["block", "natural"], default="block",
+                    help="query storage order: length-sorted 256-column blocks, or the caller's order")
+    args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        return 2
+    rec = {"allpairs": workload_allpairs, "batched": workload_batched, "threshold": workload_threshold,
+           "serve": workload_serve}[args.workload](args)
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -116,14 +116,22 @@ int siml_set_query_ragged(siml_ctx* ctx, const int32_t* ids, const int32_t* coun
  * is a ragged query set built once in device memory; any number of handles
  * on the same device attach it as their query set and run concurrently on
  * their own streams.  siml_db_destroy drops the caller's reference; the
- * device memory is freed when no handle uses it any more.  layout is
- * SIML_LAYOUT_NATURAL or SIML_LAYOUT_BLOCK. */
+ * device memory is freed when no handle uses it any more.  layout is one
+ * of the SIML_LAYOUT_* storage orders below. */
 typedef struct siml_db siml_db;
 int siml_db_create(int device, const int32_t* ids, const int32_t* counts, const int32_t* mags,
                    const int64_t* offsets, int64_t n, int layout, siml_db** out);
 int siml_db_destroy(siml_db* db);
 int siml_db_size(siml_db* db, int64_t* n);
 int siml_attach_db(siml_ctx* ctx, siml_db* db);
+/* On-disk database (no counterpart in the reference, whose packed sets only
+ * note that zero padding "will allow efficient on-disk compressed storage",
+ * _ccompiler.cpp:354-355): siml_db_save writes the built device arrays
+ * (tiles, level stream, permutation, magnitudes); siml_db_load uploads them
+ * to `device` without rebuilding.  Files carry a format version; a file of
+ * another version is rejected with SIML_EINVAL. */
+int siml_db_save(siml_db* db, const char* path);
+int siml_db_load(int device, const char* path, siml_db** out);
 /* Sizes of the current sets (-1 when unset). */
 int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery);
 
@@ -180,19 +188,30 @@ int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_
  * multi-row kernels (rows that do not qualify, and unsorted sets, use the
  * merge kernel); HASH8 uses the 8-row hashed kernels instead of the
  * bit-sliced ones; MERGE uses the merge kernel for everything.  All give the
- * same bits. */
+ * same bits.  The multi-row tables are built by set_ref for the policy in
+ * force then; a reference set given under another policy is served by the
+ * merge kernel. */
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
 #define SIML_POLICY_HASH8 2
 int siml_set_policy(siml_ctx* ctx, int policy);
 /* Storage order used by the next set_query: NATURAL keeps the caller's
- * order; BLOCK (default) sorts the query molecules by length inside blocks
- * of 256 consecutive columns.  Results are identical; only speed differs. */
+ * order; BLOCK (default) sorts the query molecules by size inside blocks
+ * of 256 consecutive columns; SORTED sorts the whole set by size (best for
+ * top-k, neighbours and histograms; the float32 matrix is written with
+ * scattered stores).  Results are identical; only speed differs. */
 #define SIML_LAYOUT_NATURAL 0
 #define SIML_LAYOUT_BLOCK 1
+#define SIML_LAYOUT_SORTED 2
 int siml_set_layout(siml_ctx* ctx, int layout);
 /* Name of the kernel variant the last similarity launch used. */
 const char* siml_last_kernel(siml_ctx* ctx);
+/* Self-test of a short float32 division method of the kernels (no
+ * counterpart in the reference): compares it on the device with IEEE
+ * division for every integer numerator in [0, 65536) and denominator in
+ * [1, 131072); reports the number of differing quotients and the first
+ * (smallest denominator) pair, -1 when none. */
+int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t* first_n, int64_t* first_d);
 
 #ifdef __cplusplus
 }

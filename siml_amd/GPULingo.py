@@ -281,7 +281,7 @@ class LingoDB:
     GPULingo.py:245-314).  Input is ragged: molecule r owns
     ids[offsets[r]:offsets[r+1]]."""
 
-    def __init__(self, ids, counts, offsets, mags=None, device=0, layout=_lib.LAYOUT_BLOCK):
+    def __init__(self, ids, counts, offsets, mags=None, device=0, layout=_lib.LAYOUT_SORTED):
         import ctypes
         ids, counts, offsets, mags = _ragged(ids, counts, offsets, mags)
         self._lib = _lib.load()
@@ -293,6 +293,27 @@ class LingoDB:
         self.h = h
         self.device = int(device)
         self.n = len(offsets) - 1
+
+    @classmethod
+    def load(cls, path, device=0):
+        """A database saved with save(), uploaded without rebuilding."""
+        import ctypes
+        import os
+        self = cls.__new__(cls)
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.siml_db_load(int(device), os.fsencode(path), ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+        n = ctypes.c_int64()
+        _lib.check(self._lib.siml_db_size(h, ctypes.byref(n)))
+        self.n = n.value
+        return self
+
+    def save(self, path):
+        """Writes the built database (device arrays, format-versioned)."""
+        import os
+        _lib.check(self._lib.siml_db_save(self.h, os.fsencode(path)))
 
     def __len__(self):
         return self.n

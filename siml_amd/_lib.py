@@ -23,7 +23,7 @@ RESULT_NONE, RESULT_F32, RESULT_I32, RESULT_CSR, RESULT_TOPK, RESULT_I64 = range
 MAX_ROW_LINGOS = 8192
 MAX_TOPK = 256
 POLICY_AUTO, POLICY_MERGE, POLICY_HASH8 = 0, 1, 2
-LAYOUT_NATURAL, LAYOUT_BLOCK = 0, 1
+LAYOUT_NATURAL, LAYOUT_BLOCK, LAYOUT_SORTED = 0, 1, 2
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -57,6 +57,8 @@ SIGNATURES = {
     "siml_db_destroy": (_int, [_vp]),
     "siml_db_size": (_int, [_vp, _i64p]),
     "siml_attach_db": (_int, [_vp, _vp]),
+    "siml_db_save": (_int, [_vp, ctypes.c_char_p]),
+    "siml_db_load": (_int, [_int, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "siml_multirow_async": (_int, [_vp, _i64, _i64]),
     "siml_single_row_async": (_int, [_vp, _i64]),
     "siml_hist_rows_async": (_int, [_vp, _i64, _i64]),
@@ -77,6 +79,7 @@ SIGNATURES = {
     "siml_set_layout": (_int, [_vp, _int]),
     "siml_kernel_stats": (_int, [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _int]),
     "siml_last_kernel": (ctypes.c_char_p, [_vp]),
+    "siml_selftest_division": (_int, [_int, _int, _i64p, _i64p, _i64p]),
 }
 
 _lib = None

@@ -4,23 +4,29 @@
 //     min(ca, cb) = #{ t >= 1 : t <= ca and t <= cb },
 // so the intersection sum_x min(ca(x), cb(x)) of two Lingo multisets is the
 // size of the intersection of their LEVEL sets {(x, t) : 1 <= t <= c(x)}.
-// A level (x, t) is encoded as the pair
-//     y = x ^ ((t - 1) * 0x9E3779B9),  w = t << 28            (t in 1..15)
-// which is injective for fixed t.  The query (column) set is streamed as
-// these (y, w) pairs, levels above 15 dropped: no row of a group has them.
 //
+// Level keys.  A Lingo id x whose four bytes are all below 0x80 (four ASCII
+// characters, what the packer produces from SMILES) leaves bits 7, 15, 23
+// and 31 clear; the level (x, t), 1 <= t <= 15, is encoded by placing the
+// four bits of t there:
+//     key(x, t) = x | (t&1) << 7 | (t&2) << 14 | (t&4) << 21 | (t&8) << 28.
+// The map is injective on such (x, t) and never 0.  Levels with another x or
+// with t > 15 cannot occur in any grouped row (see below), so the column
+// stream omits them.
+//
+// Slots.  Every key has two candidate slots of a kBitSlots-slot table,
+// h1(key) and h2(key), fixed functions of the key (a 32-bit mix), so a
+// column's slot addresses are computed once when its level stream is built.
 // A group is up to kBitRows reference rows whose level union has at most
-// kBitMaxKeys levels.  Each level of the union occupies one slot of a
-// two-choice cuckoo table of kBitSlots slots, at h1(y) or h2(y) with
-//     h_i(y) = ((umul24(y, a_i) + umul24(y >> 8, b_i)) mod 2^32) >> (32 - kBitSlotBits),
-// holding the pair (y, mask | w), where bit r of mask (r < 28) says that row
-// r of the group has the level.  A probe with (y, w) matches a slot iff the
-// ids are equal and the top 4 bits equal t; empty slots are (0, 0), which no
-// probe (t >= 1) matches.  Summing the matched masks as 28 vertical binary
-// counters gives every row's intersection with the column at once.
+// kBitMaxKeys keys; each key of the union sits in one of its two slots
+// (cuckoo placement) holding (key, mask), bit r of mask saying that row r of
+// the group has the level.  Empty slots hold (0, 0).  Probing a column level
+// reads both candidate slots and takes the mask of the one whose key is
+// equal; summing the masks as 28 vertical binary counters gives every row's
+// intersection with the column at once.
 //
-// Rows qualify when their ids are strictly ascending, every count is in
-// [0, 15] and the row alone fits in a group; other rows use the merge
+// Rows qualify when their ids are strictly ascending and ASCII, every count
+// is in [0, 15] and the row alone fits in a group; other rows use the merge
 // kernel.
 #pragma once
 
@@ -35,31 +41,48 @@ constexpr int kBitSlotBits = 12;
 constexpr int kBitSlots = 1 << kBitSlotBits;  // 4096 slots x 8 B = 32 KiB of LDS
 constexpr int kBitMaxKeys = 1800;             // load factor <= 0.44
 constexpr int kBitMaxLevel = 15;
-constexpr uint32_t kLevelMul = 0x9E3779B9u;
 
-inline uint32_t level_key(int32_t x, uint32_t t) { return (uint32_t)x ^ ((t - 1u) * kLevelMul); }
+inline bool ascii_id(int32_t x) { return ((uint32_t)x & 0x80808080u) == 0u; }
+
+inline uint32_t level_key(int32_t x, uint32_t t) {
+    return (uint32_t)x | ((t & 1u) << 7) | ((t & 2u) << 14) | ((t & 4u) << 21) | ((t & 8u) << 28);
+}
+
+// The two candidate slots of a key (distinct).
+inline void key_slots(uint32_t key, uint32_t& h1, uint32_t& h2) {
+    uint32_t h = key * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    h *= 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h1 = h >> (32 - kBitSlotBits);
+    h2 = (h >> 4) & (kBitSlots - 1);
+    if (h2 == h1) h2 ^= 1u;
+}
+
+// Second word of a column level: the byte offsets of both slots in the
+// table (8-byte slots), h1 in the low half, h2 in the high half.
+inline uint32_t slot_word(uint32_t key) {
+    uint32_t h1, h2;
+    key_slots(key, h1, h2);
+    return (h1 * 8u) | ((h2 * 8u) << 16);
+}
 
 struct BitGroupHdr {
     int32_t nrows;      // 1..kBitRows
     int32_t nkeys;      // levels in the union
     int32_t nbits;      // bit planes the counts need: 8 or 16
     int32_t pad0;
-    uint32_t a1, b1;    // multipliers of h1 (24-bit)
-    uint32_t a2, b2;    // multipliers of h2 (24-bit)
-    int64_t entry_off;  // first entry of the group in the entry arrays
+    int64_t entry_off;  // first entry of the group in the entry array
 };
 
 struct BitEntry {
-    uint32_t y;     // level key
-    uint32_t word;  // row mask | t << 28
+    uint32_t key;   // level key
+    uint32_t mask;  // rows of the group holding the level
     int32_t slot;
     int32_t pad;
 };
-
-inline uint32_t umul24_host(uint32_t u, uint32_t v) { return (uint32_t)((uint64_t)(u & 0xFFFFFFu) * (v & 0xFFFFFFu)); }
-inline uint32_t bit_hash(uint32_t y, uint32_t a, uint32_t b) {
-    return (umul24_host(y, a) + umul24_host(y >> 8, b)) >> (32 - kBitSlotBits);
-}
 
 struct BitGroupInput {
     const int32_t* ids;
@@ -80,7 +103,7 @@ struct BitGroupTables {
 
 // Consecutive rows form groups; with by_mag the eligible rows are first
 // ordered by (magnitude, index) so that a group's rows have close
-// magnitudes.
+// magnitudes.  A group whose keys cannot be placed is split in two.
 void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out, int nthreads);
 
 }  // namespace siml

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -103,6 +104,21 @@ struct ColArgs {
 };
 
 __device__ __forceinline__ int64_t col_of(const ColArgs& C, int64_t p) { return C.perm ? (int64_t)C.perm[p] : p; }
+
+// Candidate short divisions for an integer numerator n in [0, 2^16) and an
+// integer denominator d in [1, 2^17), the operands of every Tanimoto ratio
+// with small magnitudes.  Method 1: q0 = n * rcp(d), then one residual
+// correction (Markstein's quotient step, q0 + (n - q0 d) rcp(d) with fused
+// multiply-adds).  Method 2 first refines the reciprocal by one
+// Newton-Raphson step.  siml_selftest_division compares a method with the
+// IEEE quotient on every (n, d) of that domain.
+__device__ __forceinline__ float short_div(float n, float d, int method) {
+    float r = __builtin_amdgcn_rcpf(d);
+    if (method == 2) r = __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+    const float q0 = n * r;
+    const float e = __builtin_fmaf(-q0, d, n);
+    return __builtin_fmaf(e, r, q0);
+}
 
 // multisetTanimoto3_mags' float expression (_CPULingo.c:55,70).
 __device__ __forceinline__ float tanimoto_mags(int isct, int ma, int mb) {
@@ -775,34 +791,31 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
 // Bit-sliced multi-row kernels (row groups of bitgroups.h)
 //
 // A workgroup stages its group's level table (kBitSlots x (level key, row
-// mask | t << 28)) in LDS.  Each lane streams one query molecule's levels;
-// every level is probed once and the matched 28-bit row mask is added to 28
-// vertical binary counters held as bit planes P[0..NB): blocks of 8 masks go
-// through a carry-save adder tree (xor3 / majority, one v_bitop3 each) and
-// the block's carry ripples into the upper planes.  A bit-plane transpose
-// then yields each row's intersection count.
+// mask)) in LDS.  Each lane streams one query molecule's levels as (key,
+// slot word) pairs, two levels per 16-byte load; a level reads its two
+// candidate slots at the precomputed byte offsets and keeps the mask of the
+// slot whose key is equal.  The masks are added to 28 vertical binary
+// counters held as bit planes P[0..NB): blocks of 8 masks go through a
+// carry-save adder tree (xor3 / majority, one v_bitop3 each) and the
+// block's carry ripples into the upper planes.  A bit-plane transpose then
+// yields each row's intersection count as one byte (two for NB = 16).
 // ---------------------------------------------------------------------------
 
 struct BitGroupArgs {
     const siml::BitGroupHdr* hdr;
     const int32_t* rows;   // kBitRowStride row indices per group
-    const uint4* ent;      // (level key, word, slot, -) per entry
+    const uint4* ent;      // (key, mask, slot, -) per entry
     const int32_t* glist;  // the groups of this launch
 };
 
-struct BitHash {
-    uint32_t a1, b1, a2, b2;
-};
-
-__device__ __forceinline__ uint32_t bit_probe(const uint2* __restrict__ tab, const BitHash& hp, int2 e) {
-    const uint32_t y = (uint32_t)e.x, tt = (uint32_t)e.y;
-    const uint32_t yh = y >> 8;
-    const uint32_t h1 = (umul24u(y, hp.a1) + umul24u(yh, hp.b1)) >> (32 - siml::kBitSlotBits);
-    const uint32_t h2 = (umul24u(y, hp.a2) + umul24u(yh, hp.b2)) >> (32 - siml::kBitSlotBits);
-    const uint2 k1 = tab[h1], k2 = tab[h2];
-    const bool c1 = (k1.x == y) && ((k1.y ^ tt) < (1u << 28));
-    const bool c2 = (k2.x == y) && ((k2.y ^ tt) < (1u << 28));
-    return c1 ? k1.y : (c2 ? k2.y : 0u);
+// Row mask of the slot holding `key` (0 when neither candidate does); sw
+// holds the byte offsets of both candidate slots.
+__device__ __forceinline__ uint32_t bit_probe(const uint2* tab, uint32_t key, uint32_t sw) {
+    const char* base = reinterpret_cast<const char*>(tab);
+    const uint2 k1 = *reinterpret_cast<const uint2*>(base + (sw & 0xFFFFu));
+    const uint2 k2 = *reinterpret_cast<const uint2*>(base + (sw >> 16));
+    const uint32_t m2 = (k2.x == key) ? k2.y : 0u;
+    return (k1.x == key) ? k1.y : m2;
 }
 
 // Full adder on 32 independent bit lanes: s += a + b, carry out in c.
@@ -830,31 +843,45 @@ __device__ __forceinline__ void csa8(uint32_t (&P)[NB], const uint32_t (&m)[8]) 
     }
 }
 
-// Streams the Lt (a multiple of 8) levels of one query molecule; the next
-// block of 8 loads is issued before the current block is probed.
+// Probes the 8 levels held in v[0..4) and adds their masks.
 template <int NB>
-__device__ __forceinline__ void bits_accumulate(const uint2* __restrict__ tab, const BitHash& hp,
-                                                const int2* __restrict__ p, int Lt, uint32_t (&P)[NB]) {
+__device__ __forceinline__ void probe8(const uint2* tab, const int4 (&v)[4], uint32_t (&P)[NB]) {
+    uint32_t m[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        m[2 * q] = bit_probe(tab, (uint32_t)v[q].x, (uint32_t)v[q].y);
+        m[2 * q + 1] = bit_probe(tab, (uint32_t)v[q].z, (uint32_t)v[q].w);
+    }
+    csa8<NB>(P, m);
+}
+
+__device__ __forceinline__ void load8(int4 (&v)[4], const int4* __restrict__ p) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = p[q * kWave];
+}
+
+// Streams the Lt (a multiple of 8) levels of one query molecule; p is the
+// lane's first 16-byte pair.  Two register sets alternate, so the loads of
+// the next 8 levels are in flight while the current 8 are probed.
+template <int NB>
+__device__ __forceinline__ void bits_accumulate(const uint2* tab, const int4* __restrict__ p, int Lt,
+                                                uint32_t (&P)[NB]) {
 #pragma unroll
     for (int k = 0; k < NB; ++k) P[k] = 0u;
-    if (Lt <= 0) return;
-    int2 cur[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) cur[q] = p[q * kWave];
-    for (int j = 0; j < Lt; j += 8) {
-        int2 nxt[8];
-        if (j + 8 < Lt) {
-            const int2* n = p + (size_t)(j + 8) * kWave;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) nxt[q] = n[q * kWave];
-        }
-        uint32_t m[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = bit_probe(tab, hp, cur[q]);
-        csa8<NB>(P, m);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+    const int nblk = Lt >> 3;
+    if (nblk <= 0) return;
+    int4 a[4], b[4];
+    load8(a, p);
+    int k = 0;
+    for (; k + 2 <= nblk; k += 2) {
+        load8(b, p + (size_t)(k + 1) * 4 * kWave);
+        probe8<NB>(tab, a, P);
+        // may read the 8 levels after this molecule's last: the stream
+        // carries one spare block at its end, so this stays in bounds
+        load8(a, p + (size_t)(k + 2) * 4 * kWave);
+        probe8<NB>(tab, b, P);
     }
+    if (k < nblk) probe8<NB>(tab, a, P);
 }
 
 __device__ __forceinline__ void swap_bits(uint32_t& a, uint32_t& b, int sh, uint32_t mask) {
@@ -880,13 +907,15 @@ __device__ __forceinline__ void transpose8(uint32_t* P) {
     swap_bits(P[6], P[7], 1, 0x55555555u);
 }
 
-// Row r's count after transpose8 of each 8-plane half.
+// Row r's count (as float) after transpose8 of each 8-plane half.
 template <int NB>
-__device__ __forceinline__ int count_of(const uint32_t (&P)[NB], int r) {
+__device__ __forceinline__ float count_of(const uint32_t (&P)[NB], int r) {
     const int j = r & 7, sh = 8 * (r >> 3);
-    int v = (int)((P[j] >> sh) & 0xFFu);
-    if constexpr (NB == 16) v |= (int)((P[8 + j] >> sh) & 0xFFu) << 8;
-    return v;
+    if constexpr (NB == 8) {
+        return (float)((P[j] >> sh) & 0xFFu);
+    } else {
+        return (float)(((P[j] >> sh) & 0xFFu) | (((P[8 + j] >> sh) & 0xFFu) << 8));
+    }
 }
 
 struct BitRowInfo {
@@ -898,6 +927,7 @@ struct BitRowInfo {
 // Epilogue policies: init (LDS), begin (per wave), apply (per row, tile),
 // end (per wave), finish (after the workgroup barrier).
 struct EpiF32 {
+    static constexpr bool kBlockMode = false;
     float* out;
     int64_t opitch, rb;
     struct Lds {
@@ -913,7 +943,26 @@ struct EpiF32 {
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
 
+// Float32 matrix written a 256-column block at a time: each wave computes
+// the 4 tiles of a block, stages a row's 256 values in LDS at their column
+// offsets inside the block and stores them as one contiguous 1 KiB run.
+// Needs a layout that keeps columns inside their block, out and opitch
+// 16-byte aligned.
+struct EpiF32Block {
+    static constexpr bool kBlockMode = true;
+    float* out;
+    int64_t opitch, rb;
+    struct Lds {
+        float stage[kBlock / kWave][5 * kWave];
+    };
+    __device__ void init(Lds&) const {}
+    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void finish(Lds&, const BitRowInfo&) const {}
+};
+
 struct EpiHist101 {
+    static constexpr bool kBlockMode = false;
     int32_t* hist;
     int64_t rb;
     struct Lds {
@@ -936,6 +985,7 @@ struct EpiHist101 {
 };
 
 struct EpiHist1000 {
+    static constexpr bool kBlockMode = false;
     unsigned long long* hist;
     struct Lds {
         uint32_t bins[1000];
@@ -960,6 +1010,7 @@ struct EpiHist1000 {
 // (wave, row) in LDS, ballot compaction in ascending storage order.
 template <bool FILL>
 struct EpiNbr {
+    static constexpr bool kBlockMode = false;
     float lo, hi;
     int32_t* counts;
     const int64_t* offs;
@@ -1000,6 +1051,7 @@ struct EpiNbr {
 // Top-k candidates per (row, wave), as topk_push with the per-row state in
 // LDS.
 struct EpiTopk {
+    static constexpr bool kBlockMode = false;
     int k, cap;
     uint64_t* buf;
     int32_t* bufcnt;
@@ -1069,35 +1121,97 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         tab[e.z] = make_uint2(e.x, e.y);
     }
     __syncthreads();
-    const BitHash hp{hd.a1, hd.b1, hd.a2, hd.b2};
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
     const int lane = lane_id();
-    epi.begin(el, ri, gw, nw);
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t P[NB];
-        bits_accumulate<NB>(tab, hp, C.btiles + C.btoff[t] + lane, C.btlen[t], P);
-        transpose8(P);
-        if constexpr (NB == 16) transpose8(P + 8);
-        const bool valid = c < C.n;
-        const int mb = valid ? C.mag[c] : 0;
-        const float mbf = (float)mb;
-        const int64_t col = valid ? col_of(C, c) : 0;
+    const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
+    if constexpr (Epi::kBlockMode) {
+        const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
+        const int64_t gw = (int64_t)blockIdx.y * wpb + w, nw = (int64_t)gridDim.y * wpb;
+        const int64_t nblocks = (C.ntiles + 3) / 4;
+        const int64_t b0 = (nblocks * gw) / nw, b1 = (nblocks * (gw + 1)) / nw;
+        float* stage = el.stage[w];  // 256 column slots + 64 for lanes past the end
+        for (int64_t blk = b0; blk < b1; ++blk) {
+            uint32_t Q[4][NB];
+            float mbf[4];
+            bool zb[4];
+            int loc[4];
 #pragma unroll
-        for (int r = 0; r < siml::kBitRows; ++r) {
-            const int32_t row = ri.row[r];
-            if (row < 0) continue;
-            const int ma = ri.ma[r];
-            const float fi = (float)count_of<NB>(P, r);
-            // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
-            const float tv = (ma == 0 || mb == 0) ? 0.0f : fi / ((ri.maf[r] + mbf) - fi);
-            epi.apply(el, r, row, valid, col, ma, mb, tv, gw, nw);
+            for (int q = 0; q < 4; ++q) {
+                const int64_t t = blk * 4 + q;
+                const int64_t c = t * kWave + lane;
+                if (t < C.ntiles) {
+                    bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], Q[q]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) Q[q][k] = 0u;
+                }
+                transpose8(Q[q]);
+                if constexpr (NB == 16) transpose8(Q[q] + 8);
+                const bool valid = c < C.n;
+                const int mb = valid ? C.mag[c] : 0;
+                mbf[q] = (float)mb;
+                zb[q] = mb == 0;
+                loc[q] = valid ? (int)(col_of(C, c) - blk * 4 * kWave) : 4 * kWave + lane;
+            }
+            const int64_t cbase = blk * 4 * kWave;
+            const bool full = cbase + 4 * kWave <= C.n;
+#pragma unroll
+            for (int r = 0; r < siml::kBitRows; ++r) {
+                const int32_t row = ri.row[r];
+                if (row < 0) continue;
+                const bool za = ri.ma[r] == 0;
+                const float maf = ri.maf[r];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float fi = count_of<NB>(Q[q], r);
+                    // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
+                    const float d = fi / ((maf + mbf[q]) - fi);
+                    stage[loc[q]] = (za || zb[q]) ? 0.0f : d;
+                }
+                wave_lds_sync();
+                float* orow = epi.out + (int64_t)(row - epi.rb) * epi.opitch + cbase;
+                if (full) {
+                    reinterpret_cast<float4*>(orow)[lane] = reinterpret_cast<const float4*>(stage)[lane];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int o = 4 * lane + i;
+                        if (cbase + o < C.n) orow[o] = stage[o];
+                    }
+                }
+                wave_lds_sync();
+            }
         }
+        return;
+    } else {
+        int64_t t0, t1, gw, nw;
+        wave_tiles(C.ntiles, t0, t1, gw, nw);
+        epi.begin(el, ri, gw, nw);
+        for (int64_t t = t0; t < t1; ++t) {
+            const int64_t c = t * kWave + lane;
+            uint32_t P[NB];
+            bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
+            transpose8(P);
+            if constexpr (NB == 16) transpose8(P + 8);
+            const bool valid = c < C.n;
+            const int mb = valid ? C.mag[c] : 0;
+            const float mbf = (float)mb;
+            const int64_t col = valid ? col_of(C, c) : 0;
+#pragma unroll
+            for (int r = 0; r < siml::kBitRows; ++r) {
+                const int32_t row = ri.row[r];
+                if (row < 0) continue;
+                const int ma = ri.ma[r];
+                const float fi = count_of<NB>(P, r);
+                // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
+                const float d = fi / ((ri.maf[r] + mbf) - fi);
+                const float tv = (ma == 0 || mb == 0) ? 0.0f : d;
+                epi.apply(el, r, row, valid, col, ma, mb, tv, gw, nw);
+            }
+        }
+        epi.end(el, ri, gw, nw);
+        __syncthreads();
+        epi.finish(el, ri);
     }
-    epi.end(el, ri, gw, nw);
-    __syncthreads();
-    epi.finish(el, ri);
 }
 
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
@@ -1197,6 +1311,33 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+
+// Exhaustive comparison of short_div(method) with IEEE division over
+// n in [0, 65536) x d in [1, 131072): one thread per d, grid.y splits n.
+__global__ __launch_bounds__(kBlock) void k_div_check(int method, unsigned long long* bad,
+                                                      unsigned long long* first) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x + 1u;
+    const uint32_t n0 = blockIdx.y * 4096u, n1 = n0 + 4096u;
+    uint32_t cnt = 0;
+    unsigned long long f = ~0ull;
+    if (d < 131072u) {
+        const float df = (float)d;
+        for (uint32_t n = n0; n < n1; ++n) {
+            const float nf = (float)n;
+            const float a = short_div(nf, df, method);
+            const float b = nf / df;
+            if (__float_as_uint(a) != __float_as_uint(b)) {
+                ++cnt;
+                const unsigned long long key = ((unsigned long long)d << 32) | n;
+                f = key < f ? key : f;
+            }
+        }
+    }
+    if (cnt) {
+        atomicAdd(bad, (unsigned long long)cnt);
+        atomicMin(first, f);
+    }
+}
 
 struct DevBuf {
     void* p = nullptr;
@@ -1328,6 +1469,7 @@ struct RowSet {
     DevBuf g_hdr, g_ekey, g_ecnt;
     std::vector<siml::GroupHdr> h_hdr;
     std::vector<int32_t> merge_rows;  // rows served by the merge kernel
+    bool hash_built = false, bits_built = false;
     GroupArgs gargs() const { return {g_hdr.as<siml::GroupHdr>(), g_ekey.as<int2>(), g_ecnt.as<uint4>()}; }
     // bit-sliced row groups
     DevBuf b_hdr, b_rows, b_ent;
@@ -1356,6 +1498,7 @@ struct RowSet {
         h_bhdr.clear();
         h_brows.clear();
         bmerge_rows.clear();
+        hash_built = bits_built = false;
         n = -1;
     }
 };
@@ -1366,13 +1509,15 @@ struct ColSet {
     ColSet& operator=(const ColSet&) = delete;
     ~ColSet() { release(); }
     int64_t n = -1, ntiles = 0;
+    int64_t tiles_total = 0, btiles_total = 0;  // int2 elements in tiles / btiles
     int32_t maxlen = 0;
+    int layout = 0;
     bool sorted_unique = true;
     bool counts_nonneg = true;
     DevBuf tiles, toff, tlen, len, mag, csum, perm;
     bool identity = true;
-    // level stream for the bit-sliced kernels: (level key, t << 28) pairs in
-    // the same tiles/positions, step counts padded to a multiple of 8
+    bool blocked = true;  // perm keeps every column inside its 256-column block
+    // level stream of the bit-sliced kernels (build_colset)
     DevBuf btiles, btoff, btlen;
     bool levels_ok = false;
     ColArgs args() const {
@@ -1405,6 +1550,7 @@ struct siml_ctx {
     // pending result
     int kind = SIML_RESULT_NONE;
     int64_t rows = 0, cols = 0, nnz = 0;
+    int64_t dpitch = 0;  // row pitch (elements) of a matrix result in res
     DevBuf res, res2, scratch;
     std::vector<int64_t> csr_rows;  // host row offsets of a CSR result
     // timing
@@ -1509,22 +1655,29 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     TRY(upload(R.len, info.len.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.mag, info.mag.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.csum, info.csum.data(), (size_t)s.n, ctx->stream));
-    // row groups and their cuckoo tables for the hashed kernels
+    // multi-row tables of the policy in force: bit-sliced groups (AUTO) or
+    // 8-row cuckoo groups (HASH8)
     std::vector<int64_t> starts((size_t)std::max<int64_t>(s.n, 1));
     for (int64_t r = 0; r < s.n; ++r) starts[(size_t)r] = s.start(r);
-    siml::GroupInput gin{s.ids, s.counts, starts.data(), info.len.data(), info.csum.data(), s.n};
     siml::GroupTables gt;
-    siml::build_groups(gin, gt, 0);
-    TRY(upload(R.g_hdr, gt.hdr.data(), gt.hdr.size(), ctx->stream));
-    TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
-    TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
-    siml::BitGroupInput bin{s.ids, s.counts, starts.data(), info.len.data(), info.mag.data(), s.n};
+    if (ctx->policy == SIML_POLICY_HASH8) {
+        siml::GroupInput gin{s.ids, s.counts, starts.data(), info.len.data(), info.csum.data(), s.n};
+        siml::build_groups(gin, gt, 0);
+        TRY(upload(R.g_hdr, gt.hdr.data(), gt.hdr.size(), ctx->stream));
+        TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
+        TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
+    }
     siml::BitGroupTables bt;
-    siml::build_bit_groups(bin, ctx->group_by_mag, bt, 0);
-    TRY(upload(R.b_hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
-    TRY(upload(R.b_rows, bt.rows.data(), bt.rows.size(), ctx->stream));
-    TRY(upload(R.b_ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
+    if (ctx->policy == SIML_POLICY_AUTO) {
+        siml::BitGroupInput bin{s.ids, s.counts, starts.data(), info.len.data(), info.mag.data(), s.n};
+        siml::build_bit_groups(bin, ctx->group_by_mag, bt, 0);
+        TRY(upload(R.b_hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
+        TRY(upload(R.b_rows, bt.rows.data(), bt.rows.size(), ctx->stream));
+        TRY(upload(R.b_ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    R.hash_built = ctx->policy == SIML_POLICY_HASH8;
+    R.bits_built = ctx->policy == SIML_POLICY_AUTO;
     R.n = s.n;
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
@@ -1538,20 +1691,31 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
 }
 
 // Storage order of the query molecules.  NATURAL keeps the caller's order;
-// BLOCK sorts by length inside each block of kSortBlock consecutive columns,
-// so a tile's step count (its longest molecule) is close to its mean while a
-// wave's float32 outputs stay inside one 1 KiB span of the row.
+// BLOCK sorts by level count inside each block of kSortBlock consecutive
+// columns, so a tile's step count (its longest molecule) is close to its mean
+// while a wave's float32 outputs stay inside one 1 KiB span of the row;
+// SORTED sorts the whole set by level count (outputs that carry their column
+// index -- neighbours, top-k, histograms -- do not care about order).
 constexpr int64_t kSortBlock = 256;
 
-std::vector<int32_t> column_order(const SetInfo& info, int64_t n, int layout) {
+std::vector<int32_t> column_order(const std::vector<int32_t>& weight, int64_t n, int layout) {
     std::vector<int32_t> pos((size_t)n);
     for (int64_t c = 0; c < n; ++c) pos[(size_t)c] = (int32_t)c;
+    auto by_weight = [&](int32_t x, int32_t y) { return weight[(size_t)x] < weight[(size_t)y]; };
     if (layout == SIML_LAYOUT_BLOCK)
-        for (int64_t b = 0; b < n; b += kSortBlock) {
-            auto first = pos.begin() + b, last = pos.begin() + std::min(n, b + kSortBlock);
-            std::stable_sort(first, last, [&](int32_t x, int32_t y) { return info.len[(size_t)x] < info.len[(size_t)y]; });
-        }
+        for (int64_t b = 0; b < n; b += kSortBlock)
+            std::stable_sort(pos.begin() + b, pos.begin() + std::min(n, b + kSortBlock), by_weight);
+    if (layout == SIML_LAYOUT_SORTED) std::stable_sort(pos.begin(), pos.end(), by_weight);
     return pos;
+}
+
+// Levels of one query molecule that a grouped row can hold (bitgroups.h):
+// ASCII ids, t <= 15.
+inline int32_t stream_levels(const int32_t* id, const int32_t* ct, int32_t L) {
+    int32_t lv = 0;
+    for (int32_t j = 0; j < L; ++j)
+        if (siml::ascii_id(id[j]) && ct[j] > 0) lv += std::min(ct[j], siml::kBitMaxLevel);
+    return lv;
 }
 
 // Builds a query set in device memory of the current device.
@@ -1559,15 +1723,23 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
     SetInfo info;
     TRY(validate(s, info, "set_qsmiles"));
     if (s.n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "set_qsmiles: more than 2^31-1 molecules");
-    const std::vector<int32_t> order = column_order(info, s.n, layout);
+    const bool levels_ok = info.sorted_unique && info.counts_nonneg;
+    std::vector<int32_t> clv((size_t)s.n, 0);
+    parallel_for(s.n, [&](int64_t a, int64_t b) {
+        for (int64_t c = a; c < b; ++c)
+            clv[(size_t)c] = levels_ok ? stream_levels(s.ids + s.start(c), s.counts + s.start(c), info.len[(size_t)c])
+                                       : info.len[(size_t)c];
+    });
+    const std::vector<int32_t> order = column_order(clv, s.n, layout);
     bool identity = true;
     for (int64_t p = 0; p < s.n && identity; ++p) identity = order[(size_t)p] == (int32_t)p;
-    std::vector<int32_t> plen((size_t)s.n), pmag((size_t)s.n), pcsum((size_t)s.n);
+    std::vector<int32_t> plen((size_t)s.n), pmag((size_t)s.n), pcsum((size_t)s.n), plv((size_t)s.n);
     for (int64_t p = 0; p < s.n; ++p) {
         const size_t c = (size_t)order[(size_t)p];
         plen[(size_t)p] = info.len[c];
         pmag[(size_t)p] = info.mag[c];
         pcsum[(size_t)p] = info.csum[c];
+        plv[(size_t)p] = clv[c];
     }
     const int64_t ntiles = (s.n + kWave - 1) / kWave;
     std::vector<int32_t> tlen((size_t)std::max<int64_t>(ntiles, 1), 0);
@@ -1591,23 +1763,20 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
             }
     });
     TRY(upload(C.tiles, tiles.data(), (size_t)total, stream));
+    C.tiles_total = total;
     TRY(upload(C.toff, toff.data(), (size_t)ntiles + 1, stream));
     TRY(upload(C.tlen, tlen.data(), (size_t)ntiles, stream));
     TRY(upload(C.len, plen.data(), (size_t)s.n, stream));
     TRY(upload(C.mag, pmag.data(), (size_t)s.n, stream));
     TRY(upload(C.csum, pcsum.data(), (size_t)s.n, stream));
     TRY(upload(C.perm, order.data(), (size_t)s.n, stream));
-    // level stream (bitgroups.h): every (id, t), t <= min(count, 15)
-    C.levels_ok = info.sorted_unique && info.counts_nonneg;
+    // level stream (bitgroups.h): (key, slot word) of every level a grouped
+    // row can hold; level j of lane l of tile t is int2 element
+    // btoff[t] + (j / 2) * 128 + 2 * l + (j & 1), so one 16-byte load per
+    // lane reads two levels; step counts are padded to a multiple of 8 with
+    // (0, 0), which matches only empty slots.
+    C.levels_ok = levels_ok;
     if (C.levels_ok) {
-        std::vector<int32_t> plv((size_t)s.n, 0);
-        for (int64_t p = 0; p < s.n; ++p) {
-            const int64_t c = order[(size_t)p];
-            const int32_t* ct = s.counts + s.start(c);
-            int32_t lv = 0;
-            for (int32_t j = 0; j < plen[(size_t)p]; ++j) lv += std::min(ct[j], siml::kBitMaxLevel);
-            plv[(size_t)p] = lv;
-        }
         std::vector<int32_t> btlen((size_t)std::max<int64_t>(ntiles, 1), 0);
         std::vector<int64_t> btoff((size_t)ntiles + 1, 0);
         for (int64_t t = 0; t < ntiles; ++t) {
@@ -1618,24 +1787,32 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
             btoff[(size_t)t + 1] = btoff[(size_t)t] + (int64_t)m * kWave;
         }
         const int64_t btotal = btoff[(size_t)ntiles];
-        std::vector<int2> bt((size_t)std::max<int64_t>(btotal, 1), make_int2(0, 0));
+        // one spare block of 8 levels x 64 lanes after the last tile: the
+        // kernels' prefetch of the next block may read it
+        const int64_t bspare = 8 * kWave;
+        std::vector<int2> bt((size_t)(btotal + bspare), make_int2(0, 0));
         parallel_for(ntiles, [&](int64_t a, int64_t b) {
             for (int64_t t = a; t < b; ++t)
                 for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) {
                     const int64_t c = order[(size_t)p];
                     const int32_t* id = s.ids + s.start(c);
                     const int32_t* ct = s.counts + s.start(c);
-                    int2* d = bt.data() + btoff[(size_t)t] + (p - t * kWave);
+                    int2* d = bt.data() + btoff[(size_t)t] + 2 * (p - t * kWave);
                     size_t j = 0;
-                    for (int32_t k = 0; k < plen[(size_t)p]; ++k)
-                        for (int32_t lvl = 1; lvl <= std::min(ct[k], siml::kBitMaxLevel); ++lvl, ++j)
-                            d[j * kWave] = make_int2((int32_t)siml::level_key(id[k], (uint32_t)lvl), lvl << 28);
+                    for (int32_t k = 0; k < plen[(size_t)p]; ++k) {
+                        if (!siml::ascii_id(id[k])) continue;
+                        for (int32_t lvl = 1; lvl <= std::min(ct[k], siml::kBitMaxLevel); ++lvl, ++j) {
+                            const uint32_t key = siml::level_key(id[k], (uint32_t)lvl);
+                            d[(j >> 1) * 2 * kWave + (j & 1)] =
+                                make_int2((int32_t)key, (int32_t)siml::slot_word(key));
+                        }
+                    }
                 }
         });
-        TRY(upload(C.btiles, bt.data(), (size_t)btotal, stream));
+        TRY(upload(C.btiles, bt.data(), (size_t)(btotal + bspare), stream));
+        C.btiles_total = btotal + bspare;
         TRY(upload(C.btoff, btoff.data(), (size_t)ntiles + 1, stream));
         TRY(upload(C.btlen, btlen.data(), (size_t)ntiles, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
     }
     HIP_TRY(hipStreamSynchronize(stream));
     C.n = s.n;
@@ -1644,6 +1821,8 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
     C.sorted_unique = info.sorted_unique;
     C.counts_nonneg = info.counts_nonneg;
     C.identity = identity;
+    C.blocked = layout != SIML_LAYOUT_SORTED;
+    C.layout = layout;
     return SIML_OK;
 }
 
@@ -1683,7 +1862,7 @@ void plan_rows(const siml_ctx* ctx, int64_t rb, int64_t rl, RowPlan& p) {
     const RowSet& R = ctx->ref;
     const ColSet& C = *ctx->qry;
     p = RowPlan();
-    if (ctx->policy == SIML_POLICY_MERGE || !C.sorted_unique || !C.counts_nonneg) return;
+    if (ctx->policy == SIML_POLICY_MERGE || !C.sorted_unique || !C.counts_nonneg || !R.hash_built) return;
     p.all_merge = false;
     const auto& H = R.h_hdr;
     p.g0 = std::partition_point(H.begin(), H.end(), [&](const siml::GroupHdr& h) { return h.row0 + h.nrows <= rb; }) -
@@ -1712,7 +1891,7 @@ struct BitPlan {
 void plan_bits(const siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p) {
     const RowSet& R = ctx->ref;
     p = BitPlan();
-    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok) return;
+    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok || !R.bits_built) return;
     p.use = true;
     for (size_t g = 0; g < R.h_bhdr.size(); ++g) {
         bool hit = false;
@@ -1774,7 +1953,7 @@ int launch_check(const char* name) {
 
 void clear_result(siml_ctx* ctx) {
     ctx->kind = SIML_RESULT_NONE;
-    ctx->rows = ctx->cols = ctx->nnz = 0;
+    ctx->rows = ctx->cols = ctx->nnz = ctx->dpitch = 0;
     ctx->csr_rows.clear();
 }
 
@@ -1784,6 +1963,45 @@ int zero_fill_i32(siml_ctx* ctx, int32_t* p, int64_t n) {
     k_fill<int32_t><<<dim3((unsigned)blocks), dim3(256), 0, ctx->stream>>>(p, n, 0);
     return launch_check("k_fill");
 }
+
+// On-disk form of a built database: a header, then the device arrays of the
+// column set in a fixed order, each at a 64-byte aligned offset.  Loading
+// uploads the arrays as they are, skipping the host build (ordering,
+// tiling, level stream).
+constexpr char kDbMagic[8] = {'S', 'I', 'M', 'L', 'D', 'B', 0, 1};
+constexpr int32_t kDbVersion = 2;  // level keys and slot words of bitgroups.h
+
+struct DbFileHeader {
+    char magic[8];
+    int32_t version, layout;
+    int64_t n, ntiles, tiles_total, btiles_total;
+    int32_t maxlen;
+    uint8_t identity, blocked, levels_ok, sorted_unique, counts_nonneg, pad[3];
+};
+
+struct DbArray {
+    DevBuf* buf;
+    size_t bytes;
+};
+
+std::vector<DbArray> db_arrays(ColSet& C) {
+    const size_t n = (size_t)std::max<int64_t>(C.n, 0), nt = (size_t)C.ntiles;
+    std::vector<DbArray> a = {{&C.tiles, sizeof(int2) * (size_t)C.tiles_total},
+                              {&C.toff, sizeof(int64_t) * (nt + 1)},
+                              {&C.tlen, sizeof(int32_t) * nt},
+                              {&C.len, sizeof(int32_t) * n},
+                              {&C.mag, sizeof(int32_t) * n},
+                              {&C.csum, sizeof(int32_t) * n},
+                              {&C.perm, sizeof(int32_t) * n}};
+    if (C.levels_ok) {
+        a.push_back({&C.btiles, sizeof(int2) * (size_t)C.btiles_total});
+        a.push_back({&C.btoff, sizeof(int64_t) * (nt + 1)});
+        a.push_back({&C.btlen, sizeof(int32_t) * nt});
+    }
+    return a;
+}
+
+size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
 
 }  // namespace
 
@@ -1925,7 +2143,7 @@ int siml_db_create(int device, const int32_t* ids, const int32_t* counts, const 
     if (!out) return fail(SIML_EINVAL, "siml_db_create: null output pointer");
     *out = nullptr;
     if (n > 0 && !offsets) return fail(SIML_EINVAL, "siml_db_create: null offsets");
-    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK)
+    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK && layout != SIML_LAYOUT_SORTED)
         return fail(SIML_EINVAL, "siml_db_create: unknown layout %d", layout);
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) {
@@ -1965,6 +2183,100 @@ int siml_db_size(siml_db* db, int64_t* n) {
     return SIML_OK;
 }
 
+int siml_db_save(siml_db* db, const char* path) {
+    if (!db || !path) return fail(SIML_EINVAL, "siml_db_save: null argument");
+    HIP_TRY(hipSetDevice(db->device));
+    ColSet& C = *db->set;
+    DbFileHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kDbMagic, 8);
+    h.version = kDbVersion;
+    h.layout = C.layout;
+    h.n = C.n;
+    h.ntiles = C.ntiles;
+    h.tiles_total = C.tiles_total;
+    h.btiles_total = C.btiles_total;
+    h.maxlen = C.maxlen;
+    h.identity = C.identity;
+    h.blocked = C.blocked;
+    h.levels_ok = C.levels_ok;
+    h.sorted_unique = C.sorted_unique;
+    h.counts_nonneg = C.counts_nonneg;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(SIML_EINVAL, "siml_db_save: cannot open %s for writing", path);
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    auto put = [&](const void* p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes; };
+    const char zeros[64] = {0};
+    size_t pos = sizeof(h);
+    if (!put(&h, sizeof(h))) return fail(SIML_EINVAL, "siml_db_save: write to %s failed", path);
+    std::vector<char> host;
+    for (const DbArray& a : db_arrays(C)) {
+        const size_t pad = align64(pos) - pos;
+        if (!put(zeros, pad)) return fail(SIML_EINVAL, "siml_db_save: write to %s failed", path);
+        pos += pad;
+        host.resize(a.bytes);
+        if (a.bytes) HIP_TRY(hipMemcpy(host.data(), a.buf->p, a.bytes, hipMemcpyDeviceToHost));
+        if (!put(host.data(), a.bytes)) return fail(SIML_EINVAL, "siml_db_save: write to %s failed", path);
+        pos += a.bytes;
+    }
+    if (std::fflush(f) != 0) return fail(SIML_EINVAL, "siml_db_save: write to %s failed", path);
+    return SIML_OK;
+}
+
+int siml_db_load(int device, const char* path, siml_db** out) {
+    if (!out || !path) return fail(SIML_EINVAL, "siml_db_load: null argument");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(SIML_EINVAL, "siml_db_load: cannot open %s", path);
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    DbFileHeader h;
+    if (std::fread(&h, 1, sizeof(h), f) != sizeof(h) || std::memcmp(h.magic, kDbMagic, 8) != 0)
+        return fail(SIML_EINVAL, "siml_db_load: %s is not a database file", path);
+    if (h.version != kDbVersion)
+        return fail(SIML_EINVAL, "siml_db_load: %s has format version %d, this build reads %d", path, h.version,
+                    kDbVersion);
+    if (h.n < 0 || h.ntiles != (h.n + kWave - 1) / kWave || h.tiles_total < 0 || h.btiles_total < 0 ||
+        (h.layout != SIML_LAYOUT_NATURAL && h.layout != SIML_LAYOUT_BLOCK && h.layout != SIML_LAYOUT_SORTED))
+        return fail(SIML_EINVAL, "siml_db_load: %s has an inconsistent header", path);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) {
+        (void)hipGetLastError();
+        return fail(SIML_EDEVICE, "siml_db_load: device %d is not visible", device);
+    }
+    HIP_TRY(hipSetDevice(device));
+    auto set = std::make_shared<ColSet>();
+    ColSet& C = *set;
+    C.n = h.n;
+    C.ntiles = h.ntiles;
+    C.tiles_total = h.tiles_total;
+    C.btiles_total = h.btiles_total;
+    C.maxlen = h.maxlen;
+    C.layout = h.layout;
+    C.identity = h.identity;
+    C.blocked = h.blocked;
+    C.levels_ok = h.levels_ok;
+    C.sorted_unique = h.sorted_unique;
+    C.counts_nonneg = h.counts_nonneg;
+    size_t pos = sizeof(h);
+    std::vector<char> host;
+    for (const DbArray& a : db_arrays(C)) {
+        const size_t next = align64(pos);
+        if (std::fseek(f, (long)next, SEEK_SET) != 0) return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
+        pos = next;
+        host.resize(std::max<size_t>(a.bytes, 1));
+        if (a.bytes && std::fread(host.data(), 1, a.bytes, f) != a.bytes)
+            return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
+        TRY(a.buf->reserve(a.bytes));
+        if (a.bytes) HIP_TRY(hipMemcpy(a.buf->p, host.data(), a.bytes, hipMemcpyHostToDevice));
+        pos += a.bytes;
+    }
+    siml_db* db = new siml_db();
+    db->device = device;
+    db->set = std::move(set);
+    *out = db;
+    return SIML_OK;
+}
+
 int siml_attach_db(siml_ctx* ctx, siml_db* db) {
     if (!ctx || !db) return fail(SIML_EINVAL, "null pointer");
     if (db->device != ctx->device)
@@ -1988,10 +2300,14 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(set_device(ctx));
     clear_result(ctx);
     const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
-    TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * nq, 1)));
+    // device row pitch: a multiple of 4 floats, so a row's 256-column
+    // blocks are 16-byte aligned
+    const int64_t dp = (nq + 3) & ~int64_t(3);
+    TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * dp, 1)));
     ctx->kind = SIML_RESULT_F32;
     ctx->rows = nr;
     ctx->cols = nq;
+    ctx->dpitch = dp;
     if (nr == 0 || nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     BitPlan bp;
@@ -2000,11 +2316,15 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
         const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
-        TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32{ctx->res.as<float>(), nq, rowbase}, "k_bits<f32>"));
+        if (ctx->qry->blocked)
+            TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32Block{ctx->res.as<float>(), dp, rowbase},
+                            "k_bits<f32 block>"));
+        else
+            TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32{ctx->res.as<float>(), dp, rowbase}, "k_bits<f32>"));
         if (!bp.merge.empty()) {
             k_rows_f32<false><<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
                                 ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
-                                               ctx->res.as<float>(), nq);
+                                               ctx->res.as<float>(), dp);
             TRY(launch_check("k_rows_f32<rowlist>"));
         }
         tr.done();
@@ -2017,7 +2337,7 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TimedRegion tr(ctx, nr * nq);
     if (plan.all_merge) {
         k_rows_f32<false><<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
-            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, ctx->res.as<float>(), nq);
+            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, ctx->res.as<float>(), dp);
         TRY(launch_check("k_rows_f32"));
         ctx->last_kernel = "rows_merge";
     } else {
@@ -2025,13 +2345,13 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
         if (ng > 0) {
             k_hash_rows_f32<<<row_grid(ng, ctx->qry->ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
                 ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit,
-                ctx->res.as<float>(), nq);
+                ctx->res.as<float>(), dp);
             TRY(launch_check("k_hash_rows_f32"));
         }
         if (!plan.merge.empty()) {
             k_rows_f32<false><<<row_grid((int64_t)plan.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
                                 ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
-                                               ctx->res.as<float>(), nq);
+                                               ctx->res.as<float>(), dp);
             TRY(launch_check("k_rows_f32<rowlist>"));
         }
         ctx->last_kernel = plan.merge.empty() ? "rows_hash" : (ng > 0 ? "rows_hash+merge" : "rows_merge");
@@ -2049,6 +2369,7 @@ int siml_single_row_async(siml_ctx* ctx, int64_t row) {
     ctx->kind = SIML_RESULT_F32;
     ctx->rows = 1;
     ctx->cols = nq;
+    ctx->dpitch = nq;
     if (nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, row, row + 1));
     TimedRegion tr(ctx, nq);
@@ -2069,6 +2390,7 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     ctx->kind = SIML_RESULT_I32;
     ctx->rows = nr;
     ctx->cols = 101;
+    ctx->dpitch = 101;
     TRY(zero_fill_i32(ctx, ctx->res.as<int32_t>(), nr * 101));
     if (nr == 0 || nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
@@ -2396,7 +2718,7 @@ static int retrieve_matrix(siml_ctx* ctx, void* host, int64_t pitch, int want, s
     TRY(set_device(ctx));
     if (ctx->rows > 0 && ctx->cols > 0) {
         if (!host) return fail(SIML_EINVAL, "null host buffer");
-        HIP_TRY(hipMemcpy2DAsync(host, esz * pitch, ctx->res.p, esz * ctx->cols, esz * ctx->cols, ctx->rows,
+        HIP_TRY(hipMemcpy2DAsync(host, esz * pitch, ctx->res.p, esz * ctx->dpitch, esz * ctx->cols, ctx->rows,
                                  hipMemcpyDeviceToHost, ctx->stream));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2421,9 +2743,9 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
     if (pitch < ctx->cols) return fail(SIML_EINVAL, "host pitch smaller than columns");
     TRY(set_device(ctx));
     if (r1 > r0 && ctx->cols > 0)
-        HIP_TRY(hipMemcpy2DAsync(host, sizeof(float) * pitch, ctx->res.as<float>() + r0 * ctx->cols,
-                                 sizeof(float) * ctx->cols, sizeof(float) * ctx->cols, r1 - r0, hipMemcpyDeviceToHost,
-                                 ctx->stream));
+        HIP_TRY(hipMemcpy2DAsync(host, sizeof(float) * pitch, ctx->res.as<float>() + r0 * ctx->dpitch,
+                                 sizeof(float) * ctx->dpitch, sizeof(float) * ctx->cols, r1 - r0,
+                                 hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
 }
@@ -2479,7 +2801,7 @@ int siml_set_policy(siml_ctx* ctx, int policy) {
 
 int siml_set_layout(siml_ctx* ctx, int layout) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK) return fail(SIML_EINVAL, "unknown layout %d", layout);
+    if (layout != SIML_LAYOUT_NATURAL && layout != SIML_LAYOUT_BLOCK && layout != SIML_LAYOUT_SORTED) return fail(SIML_EINVAL, "unknown layout %d", layout);
     ctx->layout = layout;
     return SIML_OK;
 }
@@ -2568,6 +2890,27 @@ int siml_row_hist101_f32(int device, const float* data, int64_t rows, int64_t co
 int siml_row_hist101_f64(int device, const double* data, int64_t rows, int64_t cols, int64_t pitch,
                          uint32_t* hist) {
     return row_hist_common(device, data, sizeof(double), rows, cols, pitch, hist);
+}
+
+int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t* first_n, int64_t* first_d) {
+    if (!mismatches || !first_n || !first_d) return fail(SIML_EINVAL, "null pointer");
+    if (method != 1 && method != 2) return fail(SIML_EINVAL, "unknown division method %d", method);
+    siml_ctx* ctx = nullptr;
+    TRY(shared_ctx(device, &ctx));
+    TRY(set_device(ctx));
+    TRY(ctx->scratch.reserve(2 * sizeof(unsigned long long)));
+    unsigned long long* d = ctx->scratch.as<unsigned long long>();
+    k_fill<unsigned long long><<<dim3(1), dim3(64), 0, ctx->stream>>>(d, 1, 0ull);
+    k_fill<unsigned long long><<<dim3(1), dim3(64), 0, ctx->stream>>>(d + 1, 1, ~0ull);
+    k_div_check<<<dim3(131072 / kBlock, 16), dim3(kBlock), 0, ctx->stream>>>(method, d, d + 1);
+    TRY(launch_check("k_div_check"));
+    unsigned long long h[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *mismatches = (int64_t)h[0];
+    *first_n = h[0] ? (int64_t)(h[1] & 0xFFFFFFFFull) : -1;
+    *first_d = h[0] ? (int64_t)(h[1] >> 32) : -1;
+    return SIML_OK;
 }
 
 }  // extern "C"

@@ -64,3 +64,53 @@ def ragged_to_padded(ids, counts, offsets, mags):
         L[row, col] = ids
         C[row, col] = counts
     return L, C, lens.reshape(-1, 1), np.asarray(mags, np.int32).reshape(-1, 1)
+
+
+def ragged_slice(ids, counts, offsets, mags, a, b):
+    """Molecules [a, b) of a ragged set as a new ragged set."""
+    o0, o1 = int(offsets[a]), int(offsets[b])
+    return ids[o0:o1], counts[o0:o1], offsets[a:b + 1] - o0, mags[a:b]
+
+
+def ragged_concat(parts):
+    """One ragged set holding the molecules of each (ids, counts, offsets,
+    mags) part in order."""
+    ids = np.concatenate([p[0] for p in parts]).astype(np.int32, copy=False)
+    counts = np.concatenate([p[1] for p in parts]).astype(np.int32, copy=False)
+    lens = np.concatenate([np.diff(p[2]) for p in parts])
+    offs = np.zeros(len(lens) + 1, np.int64)
+    np.cumsum(lens, out=offs[1:])
+    mags = np.concatenate([p[3] for p in parts]).astype(np.int32, copy=False)
+    return ids, counts, offs, mags
+
+
+def geometric_sizes(n, p, lo, hi, seed):
+    """n set sizes drawn from a geometric law with success probability p,
+    clipped to [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.clip(rng.geometric(p, size=n), lo, hi).astype(np.int64)
+
+
+def mutated_strings(buf, offs, picks, edits, seed):
+    """(buffer, offsets) of copies of strings `picks` of (buf, offs), each
+    with `edits` random characters replaced over the alphabet: near
+    duplicates of database molecules."""
+    rng = np.random.default_rng(seed)
+    src = np.frombuffer(buf, dtype=np.uint8)
+    out, lens = [], []
+    for i in picks:
+        s = src[offs[i]:offs[i + 1]].copy()
+        if s.size:
+            pos = rng.integers(0, s.size, size=edits)
+            s[pos] = ALPHABET[rng.integers(0, ALPHABET.size, size=edits)]
+        out.append(s)
+        lens.append(s.size)
+    o = np.zeros(len(lens) + 1, np.int64)
+    np.cumsum(lens, out=o[1:])
+    return (np.concatenate(out).tobytes() if out else b""), o
+
+
+def padded_columns(ids, counts, offsets, mags, a, b):
+    """cSMILEStoMatrices-layout (lingos, counts, lengths, mags) of molecules
+    [a, b) of a ragged set."""
+    return ragged_to_padded(*ragged_slice(ids, counts, offsets, mags, a, b))

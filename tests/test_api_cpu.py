@@ -49,3 +49,21 @@ def test_row_histogram_argument_errors():
         rowHistogram101(np.zeros((2, 2), np.int32))
     with pytest.raises(ValueError, match="dimension 2"):
         rowHistogram101(np.zeros(4, np.float32))
+
+
+def test_db_load_rejects_bad_files(tmp_path):
+    """siml_db_load validates the file before touching a device."""
+    from siml_amd.GPULingo import LingoDB
+    missing = tmp_path / "missing.db"
+    with pytest.raises(ValueError, match="cannot open"):
+        LingoDB.load(str(missing))
+    bad = tmp_path / "bad.db"
+    bad.write_bytes(b"not a database file at all" * 4)
+    with pytest.raises(ValueError, match="not a database file"):
+        LingoDB.load(str(bad))
+    old = tmp_path / "old.db"
+    hdr = bytearray(b"SIMLDB\x00\x01" + b"\x00" * 64)
+    hdr[8:12] = (1).to_bytes(4, "little")
+    old.write_bytes(bytes(hdr))
+    with pytest.raises(ValueError, match="format version"):
+        LingoDB.load(str(old))

@@ -356,7 +356,7 @@ def test_hist_and_neighbors_both_policies(demo_lines, demo_set, policy):
     assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
 
 
-@pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK])
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK, _lib.LAYOUT_SORTED])
 def test_layouts_bit_identical(demo_lines, demo_set, layout):
     """Column storage order (natural or length-sorted blocks) changes no bit
     of any result, including neighbour order and top-k ties."""
@@ -398,3 +398,15 @@ def test_repeated_lingos(demo_lines, policy):
     offs, idx = g.getNeighborsCSR(0, n, 0.3)
     lists, counts = oracle.neighbors(ref, qry, 0.3)
     assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", [1, 2])
+def test_short_division_exhaustive(method):
+    """The short division candidates against IEEE float32 division on every
+    integer numerator in [0, 65536) and denominator in [1, 131072)."""
+    import ctypes
+    lib = _lib.load()
+    bad, fn, fd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.siml_selftest_division(0, method, ctypes.byref(bad), ctypes.byref(fn), ctypes.byref(fd)))
+    print(f"method {method}: {bad.value} mismatches, first n={fn.value} d={fd.value}", flush=True)

@@ -1,0 +1,153 @@
+"""Resident database, batched top-k and the continuous-batching serving
+loop on the GPU: results are bit-identical to the oracle and do not depend
+on what a request is batched with."""
+import numpy as np
+import pytest
+
+import oracle
+from siml_amd import _lib, compiler, synth
+from siml_amd.GPULingo import GPULingo, LingoDB
+from siml_amd.serving import Request, Server
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.int32)
+
+
+@pytest.fixture(scope="module")
+def db_and_queries():
+    buf, offs = synth.random_strings(20000, 8, 256, seed=9, dist="lognormal")
+    db_set = compiler.pack_ragged_buffer(buf, offs)
+    rng = np.random.default_rng(10)
+    picks = rng.choice(20000, size=600, replace=False)
+    qb, qo = synth.mutated_strings(buf, offs, picks, 3, seed=11)
+    q = compiler.pack_ragged_buffer(qb, qo)
+    return db_set, q
+
+
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK, _lib.LAYOUT_SORTED])
+def test_resident_db_matches_oracle(db_and_queries, layout):
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set, layout=layout)
+    g = GPULingo(0)
+    g.attach_db(db)
+    g.set_refsmiles_ragged(*synth.ragged_slice(*q, 0, 40))
+    Q = synth.padded_columns(*q, 0, 40)
+    D = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+    want = oracle.tanimoto_block(Q, D, nthreads=0)
+    got = g.getMultipleRows(0, 40)
+    assert np.array_equal(bits(got), bits(want))
+    ti, tt = g.getTopK(0, 40, 25)
+    oi, ot = oracle.topk(Q, D, 25)
+    assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
+    offs, idx = g.getNeighborsCSR(0, 40, 0.5)
+    lists, counts = oracle.neighbors(Q, D, 0.5)
+    assert counts.sum() > 0
+    assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+    assert np.array_equal(g.getGlobalHistogram(0, 40), oracle.hist1000(Q, D))
+    db.close()
+
+
+def test_batched_equals_per_set(db_and_queries):
+    """One launch over concatenated uneven query sets gives each set the
+    same top-k as running the set alone."""
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set, layout=_lib.LAYOUT_SORTED)
+    g = GPULingo(0)
+    g.attach_db(db)
+    sizes = [1, 2, 27, 28, 29, 5, 100, 3, 57, 1]
+    bounds = np.concatenate([[0], np.cumsum(sizes)])
+    per = []
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        g.set_refsmiles_ragged(*synth.ragged_slice(*q, int(a), int(b)))
+        per.append(g.getTopK(0, int(b - a), 30))
+    g.set_refsmiles_ragged(*synth.ragged_slice(*q, 0, int(bounds[-1])))
+    bi, bt = g.getTopK(0, int(bounds[-1]), 30)
+    assert np.array_equal(np.concatenate([p[0] for p in per]), bi)
+    assert np.array_equal(bits(np.concatenate([p[1] for p in per])), bits(bt))
+    db.close()
+
+
+def _requests(q, sizes):
+    reqs, o = [], 0
+    for i, n in enumerate(sizes):
+        op, prm = ("topk", 20) if i % 2 == 0 else ("threshold", 0.5)
+        reqs.append(Request(i, *synth.ragged_slice(*q, o, o + n), op, prm))
+        o += n
+    return reqs
+
+
+def test_serving_alone_equals_mix(db_and_queries):
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set, layout=_lib.LAYOUT_SORTED)
+    sizes = [1, 5, 17, 40, 3, 64, 2, 100, 33, 8, 12, 70, 1, 1, 9]
+    alone = _requests(q, sizes)
+    with Server(db, slots=1) as srv:
+        for r in alone:
+            srv.submit(r).event.wait()
+            assert r.error is None
+    mix = _requests(q, sizes)
+    with Server(db, slots=3, max_rows=128) as srv:
+        for r in mix:
+            srv.submit(r)
+        for r in mix:
+            r.event.wait()
+            assert r.error is None
+        batches = srv.batches
+    print(f"{len(mix)} requests served in {batches} batches", flush=True)
+    assert batches < len(mix)
+    for a, b in zip(alone, mix):
+        assert np.array_equal(a.result[0], b.result[0])
+        if a.op == "topk":
+            assert np.array_equal(bits(a.result[1]), bits(b.result[1]))
+        else:
+            assert np.array_equal(a.result[1], b.result[1])
+    # and against the oracle for a few requests
+    D = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+    for r in (mix[0], mix[1], mix[7]):
+        Q = synth.ragged_to_padded(r.ids, r.counts, r.offsets, r.mags)
+        if r.op == "topk":
+            oi, ot = oracle.topk(Q, D, 20)
+            assert np.array_equal(r.result[0], oi) and np.array_equal(bits(r.result[1]), bits(ot))
+        else:
+            lists, counts = oracle.neighbors(Q, D, 0.5)
+            assert np.array_equal(np.diff(r.result[0]), counts)
+            assert np.array_equal(r.result[1], np.concatenate(lists) if lists else np.zeros(0, np.int32))
+    db.close()
+
+
+def test_serving_admission_budget(db_and_queries):
+    """A memory budget smaller than two requests still serves every request
+    (one at a time)."""
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set, layout=_lib.LAYOUT_SORTED)
+    reqs = _requests(q, [30, 30, 30, 30])
+    with Server(db, slots=2, max_rows=64, mem_budget=1) as srv:
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            r.event.wait()
+            assert r.error is None and r.result is not None
+    db.close()
+
+
+def test_db_save_load_roundtrip(db_and_queries, tmp_path):
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set, layout=_lib.LAYOUT_BLOCK)
+    path = str(tmp_path / "db.siml")
+    db.save(path)
+    db2 = LingoDB.load(path)
+    assert len(db2) == len(db)
+    g1, g2 = GPULingo(0), GPULingo(0)
+    g1.attach_db(db)
+    g2.attach_db(db2)
+    sub = synth.ragged_slice(*q, 100, 160)
+    g1.set_refsmiles_ragged(*sub)
+    g2.set_refsmiles_ragged(*sub)
+    assert np.array_equal(bits(g1.getMultipleRows(0, 60)), bits(g2.getMultipleRows(0, 60)))
+    a, b = g1.getTopK(0, 60, 10), g2.getTopK(0, 60, 10)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(bits(a[1]), bits(b[1]))
+    db.close()
+    db2.close()
