@@ -99,6 +99,7 @@ struct ColArgs {
     const int2* btiles;    // level stream (bit-sliced kernels)
     const int64_t* btoff;
     const int32_t* btlen;
+    const int2* tmag;      // (min, max) magnitude of the molecules of each tile
     int64_t n;
     int64_t ntiles;
 };
@@ -479,27 +480,38 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The k-th largest of the n > k distinct keys in buf[0..n), found by one
-// wave with an MSB-first 8-bit radix select; hist is 256 wave-private LDS
-// counters.  Called in wave-uniform control flow.
-__device__ uint64_t wave_kth_largest(const uint64_t* buf, int n, int k, uint32_t* hist) {
+// Keys one lane holds while a wave re-selects its candidate buffer: the
+// buffer capacity is at most kTopkRegs * 64.
+constexpr int kTopkRegs = 6;
+
+// Replaces the n > k distinct keys of buf[0..n) by the k largest (in any
+// order) and returns the k-th largest.  One wave: the keys are read into
+// registers once, an MSB-first 8-bit radix select over them finds the k-th
+// largest (hist: 256 wave-private LDS counters), and the keys not below it
+// are written back.  Called in wave-uniform control flow.
+__device__ uint64_t wave_keep_topk(uint64_t* buf, int n, int k, uint32_t* hist) {
     const int lane = lane_id();
+    uint64_t v[kTopkRegs];
+#pragma unroll
+    for (int j = 0; j < kTopkRegs; ++j) {
+        const int i = j * kWave + lane;
+        v[j] = i < n ? buf[i] : 0ull;  // 0 is below every key
+    }
     uint64_t prefix = 0, pmask = 0;
     uint32_t need = (uint32_t)k;
     for (int shift = 56; shift >= 0; shift -= 8) {
         for (int b = lane; b < 256; b += kWave) hist[b] = 0u;
         wave_lds_sync();
-        for (int i = lane; i < n; i += kWave) {
-            const uint64_t key = buf[i];
-            if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
-        }
+#pragma unroll
+        for (int j = 0; j < kTopkRegs; ++j)
+            if (v[j] != 0ull && (v[j] & pmask) == prefix) atomicAdd(&hist[(uint32_t)(v[j] >> shift) & 255u], 1u);
         wave_lds_sync();
         const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
         const uint32_t s = h0 + h1 + h2 + h3;
         uint32_t S = s;  // keys in bins of lanes >= this lane
         for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t v = __shfl_down(S, off);
-            if (lane + off < kWave) S += v;
+            const uint32_t t = __shfl_down(S, off);
+            if (lane + off < kWave) S += t;
         }
         const uint64_t ge = __ballot(S >= need);
         const int L = 63 - __builtin_clzll(ge);
@@ -519,29 +531,20 @@ __device__ uint64_t wave_kth_largest(const uint64_t* buf, int n, int k, uint32_t
         pmask |= (uint64_t)255 << shift;
         wave_lds_sync();
     }
+    // the keys >= prefix are exactly the k largest (keys are distinct)
+    int w = 0;
+#pragma unroll
+    for (int j = 0; j < kTopkRegs; ++j) {
+        const bool keep = v[j] >= prefix && v[j] != 0ull;
+        const uint64_t m = __ballot(keep);
+        if (keep) buf[w + rank_below(m)] = v[j];
+        w += __popcll(m);
+    }
     return prefix;
 }
 
-// Keeps the keys >= K of buf[0..n) in order; returns how many were kept.
-__device__ int wave_compact_ge(uint64_t* buf, int n, uint64_t K) {
-    const int lane = lane_id();
-    int w = 0;
-    for (int base = 0; base < n; base += kWave) {
-        const int i = base + lane;
-        const uint64_t key = i < n ? buf[i] : 0ull;
-        const bool keep = i < n && key >= K;
-        const uint64_t m = __ballot(keep);
-        if (keep) buf[w + rank_below(m)] = key;
-        w += __popcll(m);
-    }
-    return w;
-}
-
-// Per-(row, wave) candidate buffer of `cap` keys.  Appends the lanes'
-// candidates that beat the running threshold; when the next append could
-// overflow, shrinks the buffer to its k best and raises the threshold to
-// the k-th best.  Discarded keys have k better keys in the same buffer, so
-// the row's global top-k always survives.
+// Candidate buffer of one (row, wave): up to cap keys; once armed, only
+// keys above theta (the k-th largest seen so far) are added.
 struct TopkBuf {
     uint64_t* buf;
     int cnt;
@@ -551,9 +554,8 @@ struct TopkBuf {
 
 __device__ __forceinline__ void topk_push(TopkBuf& B, bool valid, uint64_t key, int k, int cap, uint32_t* hist) {
     if (B.cnt + kWave > cap) {
-        const uint64_t K = wave_kth_largest(B.buf, B.cnt, k, hist);
-        B.cnt = wave_compact_ge(B.buf, B.cnt, K);
-        B.theta = K;
+        B.theta = wave_keep_topk(B.buf, B.cnt, k, hist);
+        B.cnt = k;
         B.armed = true;
     }
     const bool pred = valid && (!B.armed || key > B.theta);
@@ -694,19 +696,17 @@ __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int
 // Final top-k of each row from its nw candidate buffers: a workgroup radix
 // select of the k-th best key, then a bitonic sort of the (at most 256)
 // keys at or above it.  Output rows are padded with (-1, -inf).
-__global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restrict__ buf,
-                                                       const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
-                                                       int32_t* __restrict__ out_idx, float* __restrict__ out_t) {
+// Workgroup radix select over the candidate segments [s0, s1) of one row
+// (segment w holds cnt[w] keys at buf + w * cap): leaves in `sorted` the
+// min(k, total) largest keys (unordered, rest 0) and returns their number.
+__device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32_t* __restrict__ cnt, int s0,
+                                   int s1, int cap, int k, uint64_t* sorted) {
     __shared__ uint32_t hist[256];
-    __shared__ uint64_t sorted[kBlock];
     __shared__ uint64_t s_prefix, s_pmask;
     __shared__ uint32_t s_need, s_total, s_fill;
-    const int64_t rloc = blockIdx.x;
-    const uint64_t* rbuf = buf + rloc * nw * (int64_t)cap;
-    const int32_t* rcnt = bufcnt + rloc * nw;
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int w = 0; w < nw; ++w) tot += (uint32_t)rcnt[w];
+        for (int w = s0; w < s1; ++w) tot += (uint32_t)cnt[w];
         s_total = tot;
         s_prefix = 0ull;
         s_pmask = 0ull;
@@ -721,27 +721,44 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
             hist[threadIdx.x] = 0u;
             __syncthreads();
             const uint64_t prefix = s_prefix, pmask = s_pmask;
-            for (int w = 0; w < nw; ++w) {
-                const int n = rcnt[w];
+            for (int w = s0; w < s1; ++w) {
+                const int n = cnt[w];
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                    const uint64_t key = rbuf[(int64_t)w * cap + i];
+                    const uint64_t key = buf[(int64_t)w * cap + i];
                     if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t above = 0, need = s_need;
-                int d = 0;
-                for (int b = 255; b >= 0; --b) {
-                    if (above + hist[b] >= need) {
-                        d = b;
+            if (threadIdx.x < kWave) {
+                // wave 0 finds the digit: suffix sums over 4 bins per lane
+                const int lane = threadIdx.x;
+                const uint32_t need = s_need;
+                const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                               h3 = hist[4 * lane + 3];
+                const uint32_t sl = h0 + h1 + h2 + h3;
+                uint32_t S = sl;
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const uint32_t t = __shfl_down(S, off);
+                    if (lane + off < kWave) S += t;
+                }
+                const uint64_t ge = __ballot(S >= need);
+                const int L = 63 - __builtin_clzll(ge);
+                uint32_t above = __shfl(S - sl, L);
+                const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
+                                        (uint32_t)__shfl(h3, L)};
+                int d = 4 * L;
+                for (int q = 3; q >= 0; --q) {
+                    if (above + hb[q] >= need) {
+                        d = 4 * L + q;
                         break;
                     }
-                    above += hist[b];
+                    above += hb[q];
                 }
-                s_need = need - above;
-                s_prefix = prefix | ((uint64_t)d << shift);
-                s_pmask = pmask | ((uint64_t)255 << shift);
+                if (lane == 0) {
+                    s_need = need - above;
+                    s_prefix = prefix | ((uint64_t)d << shift);
+                    s_pmask = pmask | ((uint64_t)255 << shift);
+                }
             }
             __syncthreads();
         }
@@ -749,14 +766,40 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
     }
     sorted[threadIdx.x] = 0ull;
     __syncthreads();
-    for (int w = 0; w < nw; ++w) {
-        const int n = rcnt[w];
+    for (int w = s0; w < s1; ++w) {
+        const int n = cnt[w];
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t key = rbuf[(int64_t)w * cap + i];
+            const uint64_t key = buf[(int64_t)w * cap + i];
             if (key >= K) sorted[atomicAdd(&s_fill, 1u)] = key;
         }
     }
     __syncthreads();
+    return s_fill;
+}
+
+// First stage of the per-row selection when a row has many candidate
+// buffers: workgroup (row, s) reduces buffers [s * per, (s + 1) * per) to
+// their k largest keys.
+__global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restrict__ buf,
+                                                       const int32_t* __restrict__ bufcnt, int nw, int cap, int k,
+                                                       int per, uint64_t* __restrict__ out, int32_t* __restrict__ outcnt) {
+    __shared__ uint64_t sorted[kBlock];
+    const int64_t r = blockIdx.x;
+    const int s = blockIdx.y, ns = gridDim.y;
+    const int s0 = s * per, s1 = min(nw, s0 + per);
+    const uint32_t have = wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, s0, s1, cap, k, sorted);
+    uint64_t* o = out + (r * ns + s) * (int64_t)k;
+    for (int i = threadIdx.x; i < (int)have; i += blockDim.x) o[i] = sorted[i];
+    if (threadIdx.x == 0) outcnt[r * ns + s] = (int32_t)have;
+}
+
+__global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restrict__ buf,
+                                                       const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
+                                                       int32_t* __restrict__ out_idx, float* __restrict__ out_t) {
+    __shared__ uint64_t sorted[kBlock];
+    const int64_t rloc = blockIdx.x;
+    const uint32_t have =
+        wg_select_topk(buf + rloc * nw * (int64_t)cap, bufcnt + rloc * nw, 0, nw, cap, k, sorted);
     // bitonic sort, descending, of kBlock keys (k <= kBlock)
     for (int size = 2; size <= kBlock; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -772,7 +815,6 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
             __syncthreads();
         }
     }
-    const uint32_t have = s_fill;
     for (int i = threadIdx.x; i < k; i += blockDim.x) {
         int32_t ix = -1;
         float tv = -__builtin_inff();
@@ -922,24 +964,27 @@ struct BitRowInfo {
     int32_t row[32];  // caller row index of group row r, -1 when not live
     int32_t ma[32];
     float maf[32];
+    int32_t amin, amax;  // magnitude range of the live rows
 };
 
 // Epilogue policies: init (LDS), begin (per wave), apply (per row, tile),
 // end (per wave), finish (after the workgroup barrier).
 struct EpiF32 {
+    static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     float* out;
     int64_t opitch, rb;
     struct Lds {
         int unused;
     };
+    struct Reg {};
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
-    __device__ void apply(Lds&, int, int32_t row, bool valid, int64_t col, int, int, float t, int64_t,
+    __device__ void begin(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void apply(Lds&, Reg&, int, int32_t row, bool valid, int64_t col, int, int, float t, int64_t,
                           int64_t) const {
         if (valid) out[(row - rb) * opitch + col] = t;
     }
-    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void end(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
 
@@ -949,33 +994,50 @@ struct EpiF32 {
 // Needs a layout that keeps columns inside their block, out and opitch
 // 16-byte aligned.
 struct EpiF32Block {
+    static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = true;
     float* out;
     int64_t opitch, rb;
     struct Lds {
         float stage[kBlock / kWave][5 * kWave];
     };
+    struct Reg {};
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
-    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
 
+// 101-bin rows: pairs in bin 0 (T * 100 < 1, the bulk of unrelated pairs)
+// are counted in per-lane registers, the rest with LDS atomics.
 struct EpiHist101 {
+    static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     int32_t* hist;
     int64_t rb;
     struct Lds {
         int bins[siml::kBitRows][101];
     };
+    struct Reg {
+        uint32_t zero[siml::kBitRows];
+    };
     __device__ void init(Lds& L) const {
         for (int k = threadIdx.x; k < siml::kBitRows * 101; k += blockDim.x) (&L.bins[0][0])[k] = 0;
     }
-    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
-    __device__ void apply(Lds& L, int r, int32_t, bool valid, int64_t, int, int, float t, int64_t, int64_t) const {
-        if (valid) atomicAdd(&L.bins[r][bin101_f32(t)], 1);
+    __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
+#pragma unroll
+        for (int r = 0; r < siml::kBitRows; ++r) R.zero[r] = 0u;
     }
-    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void apply(Lds& L, Reg& R, int r, int32_t, bool valid, int64_t, int, int, float t, int64_t,
+                          int64_t) const {
+        const bool z = !(t * 100.0f >= 1.0f);
+        R.zero[r] += (valid && z) ? 1u : 0u;
+        const bool nz = valid && !z;
+        if (__ballot(nz) != 0ull && nz) atomicAdd(&L.bins[r][bin101_f32(t)], 1);
+    }
+    __device__ void end(Lds& L, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
+#pragma unroll
+        for (int r = 0; r < siml::kBitRows; ++r)
+            if (ri.row[r] >= 0 && R.zero[r]) atomicAdd(&L.bins[r][0], (int)R.zero[r]);
+    }
     __device__ void finish(Lds& L, const BitRowInfo& ri) const {
         for (int k = threadIdx.x; k < siml::kBitRows * 101; k += blockDim.x) {
             const int r = k / 101, b = k - r * 101;
@@ -984,22 +1046,34 @@ struct EpiHist101 {
     }
 };
 
+// 1000-bin histogram of the whole block: bin-0 pairs (T < fl32(1/1000)) in
+// per-lane registers, the rest with LDS atomics.
 struct EpiHist1000 {
+    static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     unsigned long long* hist;
     struct Lds {
         uint32_t bins[1000];
         float edges[1001];
     };
+    struct Reg {
+        uint32_t zero;
+    };
     __device__ void init(Lds& L) const {
         for (int k = threadIdx.x; k < 1000; k += blockDim.x) L.bins[k] = 0u;
         stage_edges(L.edges);
     }
-    __device__ void begin(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
-    __device__ void apply(Lds& L, int, int32_t, bool valid, int64_t, int, int, float t, int64_t, int64_t) const {
-        if (valid) atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
+    __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const { R.zero = 0u; }
+    __device__ void apply(Lds& L, Reg& R, int, int32_t, bool valid, int64_t, int, int, float t, int64_t,
+                          int64_t) const {
+        const bool z = !(t >= L.edges[1]);
+        R.zero += (valid && z) ? 1u : 0u;
+        const bool nz = valid && !z;
+        if (__ballot(nz) != 0ull && nz) atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
     }
-    __device__ void end(Lds&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void end(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
+        if (R.zero) atomicAdd(&L.bins[0], R.zero);
+    }
     __device__ void finish(Lds& L, const BitRowInfo&) const {
         for (int k = threadIdx.x; k < 1000; k += blockDim.x)
             if (L.bins[k]) atomicAdd(&hist[k], (unsigned long long)L.bins[k]);
@@ -1007,41 +1081,66 @@ struct EpiHist1000 {
 };
 
 // Neighbour count (FILL = false) / fill pass: one running position per
-// (wave, row) in LDS, ballot compaction in ascending storage order.
+// (wave, row) in LDS, ballot compaction in ascending storage order.  The
+// bound prefilter of gpuGetNeighbors_multirow (kernels.py:451-455) is
+// evaluated only where lo <= T < hi already holds; the result is the same
+// conjunction.
 template <bool FILL>
 struct EpiNbr {
+    static constexpr bool kPrune = true;
     static constexpr bool kBlockMode = false;
     float lo, hi;
     int32_t* counts;
     const int64_t* offs;
     int32_t* idx;
     int64_t rb;
+    // count pass only: the first bcap indices of each (row, wave) slot are
+    // also kept in buf (slot-major), so that without overflow no fill pass
+    // is needed
+    int32_t* buf;
+    int bcap;
     struct Lds {
         int64_t pos[kBlock / kWave][32];
     };
+    struct Reg {};
+    // True when no pair of the group's rows (magnitudes [amin, amax]) with
+    // the tile's columns (magnitudes tm) can pass the bound prefilter:
+    // fl(min/max) is monotone in both magnitudes, so the closest pair of
+    // the two ranges bounds it.
+    __device__ bool skip_tile(int2 tm, int amin, int amax) const {
+        if (!(lo > 0.0f) || amin < 1 || tm.x < 1) return false;
+        if (tm.y < amin) return (float)tm.y / (float)amin < lo;
+        if (amax < tm.x) return (float)amax / (float)tm.x < lo;
+        return false;
+    }
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void begin(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (lane < 32) L.pos[w][lane] = (FILL && ri.row[lane] >= 0) ? offs[((int64_t)ri.row[lane] - rb) * nw + gw] : 0;
         wave_lds_sync();
     }
-    __device__ void apply(Lds& L, int r, int32_t, bool valid, int64_t col, int ma, int mb, float t, int64_t,
-                          int64_t) const {
-        bool nb = false;
-        if (valid) {
+    __device__ void apply(Lds& L, Reg&, int r, int32_t row, bool valid, int64_t col, int ma, int mb, float t,
+                          int64_t gw, int64_t nw) const {
+        bool nb = valid && t >= lo && t < hi;
+        if (__ballot(nb) == 0ull) return;
+        if (nb) {
             const float bound = (float)min(ma, mb) / (float)max(ma, mb);
-            nb = bound >= lo && t >= lo && t < hi;
+            nb = bound >= lo;
         }
         const uint64_t m = __ballot(nb);
         if (m == 0ull) return;
         const int w = threadIdx.x / kWave;
         const int64_t pos = L.pos[w][r];
         if (FILL && nb) idx[pos + rank_below(m)] = (int32_t)col;
+        if (!FILL && buf && nb) {
+            const int64_t j = pos + rank_below(m);
+            if (j < bcap) buf[(((int64_t)row - rb) * nw + gw) * bcap + j] = (int32_t)col;
+        }
         wave_lds_sync();
         if (lane_id() == 0) L.pos[w][r] = pos + __popcll(m);
         wave_lds_sync();
     }
-    __device__ void end(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void end(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (!FILL && lane < 32 && ri.row[lane] >= 0) counts[((int64_t)ri.row[lane] - rb) * nw + gw] = (int32_t)L.pos[w][lane];
     }
@@ -1049,8 +1148,11 @@ struct EpiNbr {
 };
 
 // Top-k candidates per (row, wave), as topk_push with the per-row state in
-// LDS.
+// LDS.  A pair whose ordered T is below the high word of the row's
+// threshold key cannot enter, so most tiles are rejected with one compare;
+// lane r of Reg::thr holds that high word for row r.
 struct EpiTopk {
+    static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     int k, cap;
     uint64_t* buf;
@@ -1062,23 +1164,30 @@ struct EpiTopk {
         uint64_t theta[kBlock / kWave][32];
         uint32_t hist[kBlock / kWave][256];
     };
+    struct Reg {
+        uint32_t thr;
+    };
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, const BitRowInfo&, int64_t, int64_t) const {
+    __device__ void begin(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (lane < 32) {
             L.cnt[w][lane] = 0;
             L.armed[w][lane] = 0;
             L.theta[w][lane] = 0ull;
         }
+        R.thr = 0u;
         wave_lds_sync();
     }
-    __device__ void apply(Lds& L, int r, int32_t row, bool valid, int64_t col, int, int, float t, int64_t gw,
-                          int64_t nw) const {
+    __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
+                          int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave;
+        const uint64_t key = topk_key(t, col);
+        const uint32_t thr = (uint32_t)__builtin_amdgcn_readlane((int)R.thr, r);
+        if (__ballot(valid && (uint32_t)(key >> 32) >= thr) == 0ull) return;
         TopkBuf B{buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap, L.cnt[w][r], L.armed[w][r] != 0,
                   L.theta[w][r]};
         const int before = B.cnt;
-        topk_push(B, valid, topk_key(t, col), k, cap, L.hist[w]);
+        topk_push(B, valid, key, k, cap, L.hist[w]);
         if (B.cnt != before || B.armed != (L.armed[w][r] != 0)) {
             wave_lds_sync();
             if (lane_id() == 0) {
@@ -1086,17 +1195,32 @@ struct EpiTopk {
                 L.armed[w][r] = B.armed ? 1 : 0;
                 L.theta[w][r] = B.theta;
             }
+            if (lane_id() == r) R.thr = B.armed ? (uint32_t)(B.theta >> 32) : 0u;
             wave_lds_sync();
         }
     }
-    __device__ void end(Lds& L, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void end(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = L.cnt[w][lane];
     }
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
 
-template <int NB, class Epi>
+// Tanimoto of one pair from its intersection count fi, the row magnitude
+// maf and the column magnitude mbf, as multisetTanimoto3_mags
+// (_CPULingo.c:55,70): I / (((float)ma + mb) - I), 0 when a magnitude is 0.
+// FAST (set by the host when every magnitude is in [0, 65536) and at least
+// the molecule's count sum) takes short_div method 1: then fi < 2^16 and the
+// denominator of every pair with both magnitudes non-zero is an integer in
+// [1, 2^17), where short_div equals IEEE division (siml_selftest_division).
+template <bool FAST>
+__device__ __forceinline__ float pair_tanimoto(float fi, float maf, float mbf, bool zero) {
+    const float u = (maf + mbf) - fi;
+    const float d = FAST ? short_div(fi, u, 1) : fi / u;
+    return zero ? 0.0f : d;
+}
+
+template <int NB, bool FAST, class Epi>
 __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
                                                  int64_t rb, int64_t rl, Epi epi) {
     __shared__ uint2 tab[siml::kBitSlots];
@@ -1116,6 +1240,16 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
     }
     epi.init(el);
     __syncthreads();
+    if (Epi::kPrune && threadIdx.x == 0) {
+        int lo = INT32_MAX, hi = INT32_MIN;
+        for (int r = 0; r < 32; ++r)
+            if (ri.row[r] >= 0) {
+                lo = min(lo, ri.ma[r]);
+                hi = max(hi, ri.ma[r]);
+            }
+        ri.amin = lo;
+        ri.amax = hi;
+    }
     for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
         const uint4 e = G.ent[hd.entry_off + k];
         tab[e.z] = make_uint2(e.x, e.y);
@@ -1161,12 +1295,8 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
                 const bool za = ri.ma[r] == 0;
                 const float maf = ri.maf[r];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float fi = count_of<NB>(Q[q], r);
-                    // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
-                    const float d = fi / ((maf + mbf[q]) - fi);
-                    stage[loc[q]] = (za || zb[q]) ? 0.0f : d;
-                }
+                for (int q = 0; q < 4; ++q)
+                    stage[loc[q]] = pair_tanimoto<FAST>(count_of<NB>(Q[q], r), maf, mbf[q], za || zb[q]);
                 wave_lds_sync();
                 float* orow = epi.out + (int64_t)(row - epi.rb) * epi.opitch + cbase;
                 if (full) {
@@ -1181,12 +1311,15 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
                 wave_lds_sync();
             }
         }
-        return;
     } else {
+        typename Epi::Reg R;
         int64_t t0, t1, gw, nw;
         wave_tiles(C.ntiles, t0, t1, gw, nw);
-        epi.begin(el, ri, gw, nw);
+        epi.begin(el, R, ri, gw, nw);
         for (int64_t t = t0; t < t1; ++t) {
+            if constexpr (Epi::kPrune) {
+                if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
+            }
             const int64_t c = t * kWave + lane;
             uint32_t P[NB];
             bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
@@ -1201,14 +1334,11 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
                 const int32_t row = ri.row[r];
                 if (row < 0) continue;
                 const int ma = ri.ma[r];
-                const float fi = count_of<NB>(P, r);
-                // multisetTanimoto3_mags' expression (_CPULingo.c:55,70)
-                const float d = fi / ((ri.maf[r] + mbf) - fi);
-                const float tv = (ma == 0 || mb == 0) ? 0.0f : d;
-                epi.apply(el, r, row, valid, col, ma, mb, tv, gw, nw);
+                const float tv = pair_tanimoto<FAST>(count_of<NB>(P, r), ri.maf[r], mbf, ma == 0 || mb == 0);
+                epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw);
             }
         }
-        epi.end(el, ri, gw, nw);
+        epi.end(el, R, ri, gw, nw);
         __syncthreads();
         epi.finish(el, ri);
     }
@@ -1312,6 +1442,19 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
 // Host side
 // ---------------------------------------------------------------------------
 
+// Copies the buffered neighbour indices of every (row, wave) slot to its
+// CSR position (no slot holds more than bcap).
+__global__ __launch_bounds__(kBlock) void k_nbr_compact(const int32_t* __restrict__ buf,
+                                                        const int32_t* __restrict__ counts,
+                                                        const int64_t* __restrict__ offs, int64_t nslots, int bcap,
+                                                        int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots * bcap) return;
+    const int64_t slot = i / bcap;
+    const int j = (int)(i - slot * bcap);
+    if (j < counts[slot]) out[offs[slot] + j] = buf[i];
+}
+
 // Exhaustive comparison of short_div(method) with IEEE division over
 // n in [0, 65536) x d in [1, 131072): one thread per d, grid.y splits n.
 __global__ __launch_bounds__(kBlock) void k_div_check(int method, unsigned long long* bad,
@@ -1384,6 +1527,8 @@ struct SetInfo {
     int32_t maxlen = 0;
     bool sorted_unique = true;
     bool counts_nonneg = true;
+    // every magnitude in [0, 65536) and at least its molecule's count sum
+    bool mags_small = true;
 };
 
 template <class F>
@@ -1424,6 +1569,7 @@ int validate(const HostSet& s, SetInfo& info, const char* what) {
     info.csum.resize((size_t)s.n);
     std::vector<char> unsorted((size_t)std::max<int64_t>(s.n, 1), 0);
     std::vector<char> negative((size_t)std::max<int64_t>(s.n, 1), 0);
+    std::vector<char> big((size_t)std::max<int64_t>(s.n, 1), 0);
     parallel_for(s.n, [&](int64_t a, int64_t b) {
         for (int64_t r = a; r < b; ++r) {
             const int32_t L = s.length(r);
@@ -1439,8 +1585,10 @@ int validate(const HostSet& s, SetInfo& info, const char* what) {
             if (neg) negative[(size_t)r] = 1;
             info.len[(size_t)r] = L;
             info.csum[(size_t)r] = (int32_t)sum;
-            info.mag[(size_t)r] = s.mags ? s.mags[r] : (int32_t)sum;
+            const int32_t mag = s.mags ? s.mags[r] : (int32_t)sum;
+            info.mag[(size_t)r] = mag;
             if (!ok) unsorted[(size_t)r] = 1;
+            if (neg || mag < 0 || mag >= 65536 || (int64_t)mag < (int64_t)sum) big[(size_t)r] = 1;
         }
     });
     info.maxlen = 0;
@@ -1448,6 +1596,7 @@ int validate(const HostSet& s, SetInfo& info, const char* what) {
         info.maxlen = std::max(info.maxlen, info.len[(size_t)r]);
         if (unsorted[(size_t)r]) info.sorted_unique = false;
         if (negative[(size_t)r]) info.counts_nonneg = false;
+        if (big[(size_t)r]) info.mags_small = false;
     }
     return SIML_OK;
 }
@@ -1469,15 +1618,30 @@ struct RowSet {
     DevBuf g_hdr, g_ekey, g_ecnt;
     std::vector<siml::GroupHdr> h_hdr;
     std::vector<int32_t> merge_rows;  // rows served by the merge kernel
-    bool hash_built = false, bits_built = false;
+    bool hash_built = false;
+    bool mags_small = false;
     GroupArgs gargs() const { return {g_hdr.as<siml::GroupHdr>(), g_ekey.as<int2>(), g_ecnt.as<uint4>()}; }
-    // bit-sliced row groups
-    DevBuf b_hdr, b_rows, b_ent;
-    std::vector<siml::BitGroupHdr> h_bhdr;
-    std::vector<int32_t> h_brows, bmerge_rows;
-    BitGroupArgs bargs(const int32_t* glist) const {
-        return {b_hdr.as<siml::BitGroupHdr>(), b_rows.as<int32_t>(), b_ent.as<uint4>(), glist};
-    }
+    // bit-sliced row groups: [0] consecutive rows (built by set_ref under the
+    // AUTO policy), [1] rows ordered by magnitude (built on first use by the
+    // neighbour search, whose tile pruning needs narrow magnitude bands)
+    struct BitGrouping {
+        bool built = false;
+        DevBuf hdr, rows, ent;
+        std::vector<siml::BitGroupHdr> h_hdr;
+        std::vector<int32_t> h_rows, merge_rows;
+        void release() {
+            hdr.release();
+            rows.release();
+            ent.release();
+            h_hdr.clear();
+            h_rows.clear();
+            merge_rows.clear();
+            built = false;
+        }
+    } bits[2];
+    // host copy of the ragged set, for groupings built after set_ref
+    std::vector<int32_t> h_ids, h_counts, h_mag, h_csum;
+    std::vector<int64_t> h_start;
     RowArgs args() const {
         return {elems.as<int2>(), off.as<int64_t>(), len.as<int32_t>(), mag.as<int32_t>(), csum.as<int32_t>()};
     }
@@ -1492,13 +1656,14 @@ struct RowSet {
         g_ecnt.release();
         h_hdr.clear();
         merge_rows.clear();
-        b_hdr.release();
-        b_rows.release();
-        b_ent.release();
-        h_bhdr.clear();
-        h_brows.clear();
-        bmerge_rows.clear();
-        hash_built = bits_built = false;
+        bits[0].release();
+        bits[1].release();
+        h_ids.clear();
+        h_counts.clear();
+        h_mag.clear();
+        h_csum.clear();
+        h_start.clear();
+        hash_built = false;
         n = -1;
     }
 };
@@ -1514,7 +1679,8 @@ struct ColSet {
     int layout = 0;
     bool sorted_unique = true;
     bool counts_nonneg = true;
-    DevBuf tiles, toff, tlen, len, mag, csum, perm;
+    bool mags_small = false;
+    DevBuf tiles, toff, tlen, len, mag, csum, perm, tmag;
     bool identity = true;
     bool blocked = true;  // perm keeps every column inside its 256-column block
     // level stream of the bit-sliced kernels (build_colset)
@@ -1523,7 +1689,7 @@ struct ColSet {
     ColArgs args() const {
         return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
                 csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), btiles.as<int2>(), btoff.as<int64_t>(),
-                btlen.as<int32_t>(), n, ntiles};
+                btlen.as<int32_t>(), tmag.as<int2>(), n, ntiles};
     }
     void release() {
         tiles.release();
@@ -1533,6 +1699,7 @@ struct ColSet {
         mag.release();
         csum.release();
         perm.release();
+        tmag.release();
         btiles.release();
         btoff.release();
         btlen.release();
@@ -1562,7 +1729,6 @@ struct siml_ctx {
     std::string last_kernel = "none";
     int policy = SIML_POLICY_AUTO;
     int layout = SIML_LAYOUT_BLOCK;
-    bool group_by_mag = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
@@ -1630,6 +1796,27 @@ int collect_timing(siml_ctx* ctx) {
     return SIML_OK;
 }
 
+// Builds bit-sliced grouping `which` of the reference set (0: consecutive
+// rows, 1: rows ordered by magnitude) unless it exists.
+int ensure_bits(siml_ctx* ctx, int which) {
+    RowSet& R = ctx->ref;
+    RowSet::BitGrouping& B = R.bits[which];
+    if (B.built) return SIML_OK;
+    siml::BitGroupInput bin{R.h_ids.data(), R.h_counts.data(), R.h_start.data(), R.h_len.data(), R.h_mag.data(),
+                            R.n};
+    siml::BitGroupTables bt;
+    siml::build_bit_groups(bin, which == 1, bt, 0);
+    TRY(upload(B.hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
+    TRY(upload(B.rows, bt.rows.data(), bt.rows.size(), ctx->stream));
+    TRY(upload(B.ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    B.h_hdr = std::move(bt.hdr);
+    B.h_rows = std::move(bt.rows);
+    B.merge_rows = std::move(bt.merge_rows);
+    B.built = true;
+    return SIML_OK;
+}
+
 int upload_rows(siml_ctx* ctx, const HostSet& s) {
     SetInfo info;
     TRY(validate(s, info, "set_refsmiles"));
@@ -1667,26 +1854,27 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
         TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
         TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
     }
-    siml::BitGroupTables bt;
-    if (ctx->policy == SIML_POLICY_AUTO) {
-        siml::BitGroupInput bin{s.ids, s.counts, starts.data(), info.len.data(), info.mag.data(), s.n};
-        siml::build_bit_groups(bin, ctx->group_by_mag, bt, 0);
-        TRY(upload(R.b_hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
-        TRY(upload(R.b_rows, bt.rows.data(), bt.rows.size(), ctx->stream));
-        TRY(upload(R.b_ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
+    R.bits[0].release();
+    R.bits[1].release();
+    R.h_ids.resize((size_t)total);
+    R.h_counts.resize((size_t)total);
+    for (int64_t i = 0; i < total; ++i) {
+        R.h_ids[(size_t)i] = el[(size_t)i].x;
+        R.h_counts[(size_t)i] = el[(size_t)i].y;
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    R.h_start.assign(off.begin(), off.end());
+    R.h_mag = info.mag;
+    R.h_csum = info.csum;
     R.hash_built = ctx->policy == SIML_POLICY_HASH8;
-    R.bits_built = ctx->policy == SIML_POLICY_AUTO;
+    R.mags_small = info.mags_small;
     R.n = s.n;
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
     R.h_len = std::move(info.len);
     R.h_hdr = std::move(gt.hdr);
     R.merge_rows = std::move(gt.merge_rows);
-    R.h_bhdr = std::move(bt.hdr);
-    R.h_brows = std::move(bt.rows);
-    R.bmerge_rows = std::move(bt.merge_rows);
+    if (ctx->policy == SIML_POLICY_AUTO) TRY(ensure_bits(ctx, 0));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
 }
 
@@ -1770,6 +1958,16 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
     TRY(upload(C.mag, pmag.data(), (size_t)s.n, stream));
     TRY(upload(C.csum, pcsum.data(), (size_t)s.n, stream));
     TRY(upload(C.perm, order.data(), (size_t)s.n, stream));
+    std::vector<int2> tmag((size_t)std::max<int64_t>(ntiles, 1), make_int2(0, 0));
+    for (int64_t t = 0; t < ntiles; ++t) {
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (int64_t p = t * kWave; p < std::min(s.n, (t + 1) * kWave); ++p) {
+            lo = std::min(lo, pmag[(size_t)p]);
+            hi = std::max(hi, pmag[(size_t)p]);
+        }
+        tmag[(size_t)t] = make_int2(lo, hi);
+    }
+    TRY(upload(C.tmag, tmag.data(), (size_t)ntiles, stream));
     // level stream (bitgroups.h): (key, slot word) of every level a grouped
     // row can hold; level j of lane l of tile t is int2 element
     // btoff[t] + (j / 2) * 128 + 2 * l + (j & 1), so one 16-byte load per
@@ -1820,6 +2018,7 @@ int build_colset(hipStream_t stream, int layout, const HostSet& s, ColSet& C) {
     C.maxlen = info.maxlen;
     C.sorted_unique = info.sorted_unique;
     C.counts_nonneg = info.counts_nonneg;
+    C.mags_small = info.mags_small;
     C.identity = identity;
     C.blocked = layout != SIML_LAYOUT_SORTED;
     C.layout = layout;
@@ -1885,30 +2084,45 @@ int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
 struct BitPlan {
     bool use = false;
     int nb = 8;
+    int which = 0;  // grouping used
     std::vector<int32_t> groups, merge;
 };
 
-void plan_bits(const siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p) {
-    const RowSet& R = ctx->ref;
+// `by_mag` asks for the magnitude-ordered grouping (built here on first
+// use).  Nothing is planned (use = false) when the policy is not AUTO, the
+// query set has no level stream, or the reference set was given under
+// another policy.
+int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false) {
+    RowSet& R = ctx->ref;
     p = BitPlan();
-    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok || !R.bits_built) return;
+    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok || !R.bits[0].built) return SIML_OK;
+    p.which = by_mag ? 1 : 0;
+    TRY(ensure_bits(ctx, p.which));
+    const RowSet::BitGrouping& B = R.bits[p.which];
     p.use = true;
-    for (size_t g = 0; g < R.h_bhdr.size(); ++g) {
+    for (size_t g = 0; g < B.h_hdr.size(); ++g) {
         bool hit = false;
-        for (int r = 0; r < R.h_bhdr[g].nrows && !hit; ++r) {
-            const int32_t row = R.h_brows[g * siml::kBitRowStride + (size_t)r];
+        for (int r = 0; r < B.h_hdr[g].nrows && !hit; ++r) {
+            const int32_t row = B.h_rows[g * siml::kBitRowStride + (size_t)r];
             hit = row >= rb && row < rl;
         }
         if (!hit) continue;
         p.groups.push_back((int32_t)g);
-        p.nb = std::max(p.nb, R.h_bhdr[g].nbits);
+        p.nb = std::max(p.nb, B.h_hdr[g].nbits);
     }
-    auto a = std::lower_bound(R.bmerge_rows.begin(), R.bmerge_rows.end(), (int32_t)rb);
-    auto b = std::lower_bound(R.bmerge_rows.begin(), R.bmerge_rows.end(), rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
+    auto a = std::lower_bound(B.merge_rows.begin(), B.merge_rows.end(), (int32_t)rb);
+    auto b = std::lower_bound(B.merge_rows.begin(), B.merge_rows.end(), rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
     p.merge.assign(a, b);
+    return SIML_OK;
 }
 
 int launch_check(const char* name);
+
+// Candidate buffers reduced by one workgroup of the first top-k stage.
+constexpr int64_t kTopkStage = 16;
+
+// Neighbour indices kept per (row, wave) slot by the count pass.
+constexpr int kNbrCap = 16;
 
 // Workgroups per bit-sliced launch: ~8 rounds of the 1024 that fit at once
 // (256 CUs x 4), so the last, partial round costs little.
@@ -1922,13 +2136,20 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
     ctx->h_glist = p.groups;
     TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
     const dim3 grid((unsigned)p.groups.size(), gy);
-    const BitGroupArgs G = ctx->ref.bargs(ctx->glist.as<int32_t>());
-    if (p.nb == 8)
-        k_bits<8, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rb, rl,
-                                                              epi);
+    const RowSet::BitGrouping& B = ctx->ref.bits[p.which];
+    const BitGroupArgs G{B.hdr.as<siml::BitGroupHdr>(), B.rows.as<int32_t>(), B.ent.as<uint4>(),
+                         ctx->glist.as<int32_t>()};
+    const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
+    const int32_t* rmag = ctx->ref.mag.as<int32_t>();
+    const ColArgs C = ctx->qry->args();
+    if (p.nb == 8 && fast)
+        k_bits<8, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+    else if (p.nb == 8)
+        k_bits<8, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+    else if (fast)
+        k_bits<16, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
     else
-        k_bits<16, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rb,
-                                                               rl, epi);
+        k_bits<16, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
     return launch_check(name);
 }
 
@@ -1969,14 +2190,14 @@ int zero_fill_i32(siml_ctx* ctx, int32_t* p, int64_t n) {
 // uploads the arrays as they are, skipping the host build (ordering,
 // tiling, level stream).
 constexpr char kDbMagic[8] = {'S', 'I', 'M', 'L', 'D', 'B', 0, 1};
-constexpr int32_t kDbVersion = 2;  // level keys and slot words of bitgroups.h
+constexpr int32_t kDbVersion = 3;  // level keys and slot words of bitgroups.h, tile magnitude ranges
 
 struct DbFileHeader {
     char magic[8];
     int32_t version, layout;
     int64_t n, ntiles, tiles_total, btiles_total;
     int32_t maxlen;
-    uint8_t identity, blocked, levels_ok, sorted_unique, counts_nonneg, pad[3];
+    uint8_t identity, blocked, levels_ok, sorted_unique, counts_nonneg, mags_small, pad[2];
 };
 
 struct DbArray {
@@ -1992,7 +2213,8 @@ std::vector<DbArray> db_arrays(ColSet& C) {
                               {&C.len, sizeof(int32_t) * n},
                               {&C.mag, sizeof(int32_t) * n},
                               {&C.csum, sizeof(int32_t) * n},
-                              {&C.perm, sizeof(int32_t) * n}};
+                              {&C.perm, sizeof(int32_t) * n},
+                              {&C.tmag, sizeof(int2) * nt}};
     if (C.levels_ok) {
         a.push_back({&C.btiles, sizeof(int2) * (size_t)C.btiles_total});
         a.push_back({&C.btoff, sizeof(int64_t) * (nt + 1)});
@@ -2202,6 +2424,7 @@ int siml_db_save(siml_db* db, const char* path) {
     h.levels_ok = C.levels_ok;
     h.sorted_unique = C.sorted_unique;
     h.counts_nonneg = C.counts_nonneg;
+    h.mags_small = C.mags_small;
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(SIML_EINVAL, "siml_db_save: cannot open %s for writing", path);
     std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
@@ -2257,6 +2480,7 @@ int siml_db_load(int device, const char* path, siml_db** out) {
     C.levels_ok = h.levels_ok;
     C.sorted_unique = h.sorted_unique;
     C.counts_nonneg = h.counts_nonneg;
+    C.mags_small = h.mags_small;
     size_t pos = sizeof(h);
     std::vector<char> host;
     for (const DbArray& a : db_arrays(C)) {
@@ -2311,7 +2535,7 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     if (nr == 0 || nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     BitPlan bp;
-    plan_bits(ctx, rowbase, rowlimit, bp);
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
@@ -2396,7 +2620,7 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     int32_t* hist = ctx->res.as<int32_t>();
     BitPlan bp;
-    plan_bits(ctx, rowbase, rowlimit, bp);
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
@@ -2451,7 +2675,7 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
         return SIML_OK;
     }
     BitPlan bp;
-    plan_bits(ctx, rowbase, rowlimit, bp);
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp, true));
     RowPlan plan;
     if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
     const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
@@ -2465,16 +2689,18 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     }
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t ng = plan.g1 - plan.g0;
+    int32_t* d_nbuf = nullptr;
     const dim3 hgrid((unsigned)std::max<int64_t>(ng, 1), grid.y);
     const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y);
     auto launch = [&](bool fill, const int64_t* offs, int32_t* idx) -> int {
         if (bp.use) {
             if (fill)
-                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase},
-                                "k_bits<nbr fill>"));
+                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
+                                EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase, nullptr, 0}, "k_bits<nbr fill>"));
             else
                 TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
-                                EpiNbr<false>{lo, hi, d_counts_of(ctx), nullptr, nullptr, rowbase}, "k_bits<nbr count>"));
+                                EpiNbr<false>{lo, hi, d_counts_of(ctx), nullptr, nullptr, rowbase, d_nbuf, kNbrCap},
+                                "k_bits<nbr count>"));
             if (!mrows.empty()) {
                 if (fill)
                     k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
@@ -2523,9 +2749,15 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     };
     const int64_t nslots = nr * nw;
     const size_t counts_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
-    TRY(ctx->scratch.reserve(counts_bytes + sizeof(int64_t) * (size_t)nslots));
+    const size_t offs_bytes = (sizeof(int64_t) * (size_t)nslots + 255) & ~size_t(255);
+    // single pass when every row is on the bit-sliced kernel: the count pass
+    // keeps the first kNbrCap indices of each slot
+    const bool one_pass = bp.use && mrows.empty();
+    const size_t nbuf_bytes = one_pass ? sizeof(int32_t) * (size_t)nslots * kNbrCap : 0;
+    TRY(ctx->scratch.reserve(counts_bytes + offs_bytes + nbuf_bytes));
     int32_t* d_counts = ctx->scratch.as<int32_t>();
     int64_t* d_offs = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + counts_bytes);
+    d_nbuf = one_pass ? reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + counts_bytes + offs_bytes) : nullptr;
     TimedRegion tr(ctx, nr * nq);
     TRY(launch(false, nullptr, nullptr));
     std::vector<int32_t> h_counts((size_t)nslots);
@@ -2533,18 +2765,29 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::vector<int64_t> h_offs((size_t)nslots);
     int64_t acc = 0;
+    int32_t most = 0;
     for (int64_t r = 0; r < nr; ++r) {
         ctx->csr_rows[(size_t)r] = acc;
         for (int64_t w = 0; w < nw; ++w) {
             h_offs[(size_t)(r * nw + w)] = acc;
             acc += h_counts[(size_t)(r * nw + w)];
+            most = std::max(most, h_counts[(size_t)(r * nw + w)]);
         }
     }
     ctx->csr_rows[(size_t)nr] = acc;
     ctx->nnz = acc;
     HIP_TRY(hipMemcpyAsync(d_offs, h_offs.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(acc, 1)));
-    TRY(launch(true, d_offs, ctx->res.as<int32_t>()));
+    if (one_pass && most <= kNbrCap) {
+        if (acc > 0) {
+            const int64_t nthreads = nslots * kNbrCap;
+            k_nbr_compact<<<dim3((unsigned)((nthreads + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
+                d_nbuf, d_counts, d_offs, nslots, kNbrCap, ctx->res.as<int32_t>());
+            TRY(launch_check("k_nbr_compact"));
+        }
+    } else {
+        TRY(launch(true, d_offs, ctx->res.as<int32_t>()));
+    }
     if (!ctx->qry->identity && acc > 0) {
         // indices were written in storage order: sort each row ascending
         TRY(upload(ctx->csr_dev, ctx->csr_rows.data(), (size_t)nr + 1, ctx->stream));
@@ -2578,22 +2821,28 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     ctx->cols = k;
     if (nr == 0) return SIML_OK;
     BitPlan bp;
-    plan_bits(ctx, rowbase, rowlimit, bp);
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     RowPlan plan;
     if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
     const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const int cap = (int)(((k + kWave) + kWave - 1) / kWave * kWave);
+    static_assert(((SIML_MAX_TOPK + kWave) + kWave - 1) / kWave <= kTopkRegs, "top-k buffer exceeds wave registers");
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? kBitsTargetWG : 2048);
-    grid.y = std::min<unsigned>(grid.y, 64u);
+    grid.y = std::min<unsigned>(grid.y, 256u);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
-    TRY(ctx->scratch.reserve(buf_bytes + sizeof(int32_t) * (size_t)nslots));
+    const size_t cnt_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
+    const int64_t nstage = nw > kTopkStage ? nr * ((nw + kTopkStage - 1) / kTopkStage) : 0;
+    const size_t sbuf_bytes = sizeof(uint64_t) * (size_t)nstage * (size_t)k;
+    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + sbuf_bytes + sizeof(int32_t) * (size_t)nstage));
     uint64_t* d_buf = ctx->scratch.as<uint64_t>();
     int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes);
+    uint64_t* d_sbuf = reinterpret_cast<uint64_t*>(ctx->scratch.as<char>() + buf_bytes + cnt_bytes);
+    int32_t* d_scnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes + cnt_bytes + sbuf_bytes);
     const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
     if (bp.use) {
@@ -2620,8 +2869,18 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
             TRY(launch_check("k_topk<rowlist>"));
         }
     }
-    k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_buf, d_cnt, (int)nw, k, cap,
-                                                                     ctx->res.as<int32_t>(), ctx->res2.as<float>());
+    if (nw > kTopkStage) {
+        // many buffers per row: reduce groups of kTopkStage buffers first
+        const int ns = (int)((nw + kTopkStage - 1) / kTopkStage);
+        k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(
+            d_buf, d_cnt, (int)nw, cap, k, kTopkStage, d_sbuf, d_scnt);
+        TRY(launch_check("k_topk_stage"));
+        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_sbuf, d_scnt, ns, k, k,
+                                                                         ctx->res.as<int32_t>(), ctx->res2.as<float>());
+    } else {
+        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_buf, d_cnt, (int)nw, k, cap,
+                                                                         ctx->res.as<int32_t>(), ctx->res2.as<float>());
+    }
     TRY(launch_check("k_topk_final"));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
@@ -2642,7 +2901,7 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(launch_check("k_fill"));
     if (nr == 0 || nq == 0) return SIML_OK;
     BitPlan bp;
-    plan_bits(ctx, rowbase, rowlimit, bp);
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     RowPlan plan;
     if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
     const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;

@@ -63,7 +63,7 @@ def test_db_load_rejects_bad_files(tmp_path):
         LingoDB.load(str(bad))
     old = tmp_path / "old.db"
     hdr = bytearray(b"SIMLDB\x00\x01" + b"\x00" * 64)
-    hdr[8:12] = (1).to_bytes(4, "little")
+    hdr[8:12] = (1).to_bytes(4, "little")  # an older format version
     old.write_bytes(bytes(hdr))
     with pytest.raises(ValueError, match="format version"):
         LingoDB.load(str(old))

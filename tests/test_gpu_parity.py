@@ -410,3 +410,17 @@ def test_short_division_exhaustive(method):
     bad, fn, fd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
     _lib.check(lib.siml_selftest_division(0, method, ctypes.byref(bad), ctypes.byref(fn), ctypes.byref(fd)))
     print(f"method {method}: {bad.value} mismatches, first n={fn.value} d={fd.value}", flush=True)
+
+
+@pytest.mark.parametrize("lo", [0.0, 0.05, 0.6, 0.95])
+def test_neighbors_single_and_two_pass(demo_lines, demo_set, lo):
+    """Slots with at most 16 neighbours take the single pass; any larger
+    slot (lo = 0 makes every pair a neighbour) takes the fill pass.  Both
+    give the complete, ascending lists."""
+    g = GPULingo(0)
+    ref = oracle.pack(demo_lines[3000:3060])
+    _set(g, ref, demo_set)
+    offs, idx = g.getNeighborsCSR(0, 60, lo)
+    lists, counts = oracle.neighbors(ref, demo_set, lo)
+    assert np.array_equal(np.diff(offs), counts)
+    assert np.array_equal(idx, np.concatenate(lists))
