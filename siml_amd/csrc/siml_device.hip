@@ -696,37 +696,52 @@ __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int
 // Final top-k of each row from its nw candidate buffers: a workgroup radix
 // select of the k-th best key, then a bitonic sort of the (at most 256)
 // keys at or above it.  Output rows are padded with (-1, -inf).
+// Candidate buffers one selection workgroup takes (its keys are staged in
+// LDS: kTopkFan x (SIML_MAX_TOPK + 64) keys = 40 KiB).
+constexpr int kTopkFan = 16;
+constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kWave);
+
 // Workgroup radix select over the candidate segments [s0, s1) of one row
-// (segment w holds cnt[w] keys at buf + w * cap): leaves in `sorted` the
-// min(k, total) largest keys (unordered, rest 0) and returns their number.
+// (segment w holds cnt[w] keys at buf + w * cap, s1 - s0 <= kTopkFan): the
+// keys are copied to LDS once, an MSB-first 8-bit radix select finds the
+// k-th largest, and `sorted` receives the min(k, total) largest keys
+// (unordered, rest 0).  Returns their number.
 __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32_t* __restrict__ cnt, int s0,
                                    int s1, int cap, int k, uint64_t* sorted) {
+    __shared__ uint64_t cand[kTopkLdsKeys];
     __shared__ uint32_t hist[256];
+    __shared__ int32_t s_base[kTopkFan + 1];
     __shared__ uint64_t s_prefix, s_pmask;
-    __shared__ uint32_t s_need, s_total, s_fill;
+    __shared__ uint32_t s_need, s_fill;
     if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int w = s0; w < s1; ++w) tot += (uint32_t)cnt[w];
-        s_total = tot;
+        int32_t tot = 0;
+        for (int w = s0; w < s1; ++w) {
+            s_base[w - s0] = tot;
+            tot += cnt[w];
+        }
+        s_base[s1 - s0] = tot;
         s_prefix = 0ull;
         s_pmask = 0ull;
         s_need = (uint32_t)k;
         s_fill = 0u;
     }
     __syncthreads();
-    const uint32_t total = s_total;
+    const uint32_t total = (uint32_t)s_base[s1 - s0];
+    for (int w = s0; w < s1; ++w) {
+        const int n = s_base[w - s0 + 1] - s_base[w - s0];
+        uint64_t* d = cand + s_base[w - s0];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = buf[(int64_t)w * cap + i];
+    }
+    __syncthreads();
     uint64_t K = 0ull;  // every key is > 0
     if (total > (uint32_t)k) {
         for (int shift = 56; shift >= 0; shift -= 8) {
             hist[threadIdx.x] = 0u;
             __syncthreads();
             const uint64_t prefix = s_prefix, pmask = s_pmask;
-            for (int w = s0; w < s1; ++w) {
-                const int n = cnt[w];
-                for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                    const uint64_t key = buf[(int64_t)w * cap + i];
-                    if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
-                }
+            for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+                const uint64_t key = cand[i];
+                if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
             }
             __syncthreads();
             if (threadIdx.x < kWave) {
@@ -766,37 +781,42 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
     }
     sorted[threadIdx.x] = 0ull;
     __syncthreads();
-    for (int w = s0; w < s1; ++w) {
-        const int n = cnt[w];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t key = buf[(int64_t)w * cap + i];
-            if (key >= K) sorted[atomicAdd(&s_fill, 1u)] = key;
-        }
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint64_t key = cand[i];
+        if (key >= K) sorted[atomicAdd(&s_fill, 1u)] = key;
     }
     __syncthreads();
     return s_fill;
 }
 
-// First stage of the per-row selection when a row has many candidate
-// buffers: workgroup (row, s) reduces buffers [s * per, (s + 1) * per) to
-// their k largest keys.
+// One level of the per-row selection when a row has more than kTopkFan
+// candidate buffers: workgroup (row, s) reduces buffers
+// [s * kTopkFan, (s + 1) * kTopkFan) to their k largest keys.
 __global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restrict__ buf,
                                                        const int32_t* __restrict__ bufcnt, int nw, int cap, int k,
-                                                       int per, uint64_t* __restrict__ out, int32_t* __restrict__ outcnt) {
+                                                       uint64_t* __restrict__ out, int32_t* __restrict__ outcnt) {
     __shared__ uint64_t sorted[kBlock];
     const int64_t r = blockIdx.x;
     const int s = blockIdx.y, ns = gridDim.y;
-    const int s0 = s * per, s1 = min(nw, s0 + per);
+    const int s0 = s * kTopkFan, s1 = min(nw, s0 + kTopkFan);
     const uint32_t have = wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, s0, s1, cap, k, sorted);
     uint64_t* o = out + (r * ns + s) * (int64_t)k;
     for (int i = threadIdx.x; i < (int)have; i += blockDim.x) o[i] = sorted[i];
     if (threadIdx.x == 0) outcnt[r * ns + s] = (int32_t)have;
 }
 
+// Final per-row selection: the k largest keys of the row's candidate
+// segments, sorted (T descending, then smaller column).  With zero_fill
+// the candidates hold only the pairs with T > 0 and every other pair of the
+// row has T == 0: the slots after the positive candidates take the columns
+// with T == 0 in ascending order -- the smallest column indices of [0, ncols)
+// that are not among the positive candidates.
 __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restrict__ buf,
                                                        const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
-                                                       int32_t* __restrict__ out_idx, float* __restrict__ out_t) {
+                                                       int zero_fill, int64_t ncols, int32_t* __restrict__ out_idx,
+                                                       float* __restrict__ out_t) {
     __shared__ uint64_t sorted[kBlock];
+    __shared__ uint32_t poscol[kBlock];
     const int64_t rloc = blockIdx.x;
     const uint32_t have =
         wg_select_topk(buf + rloc * nw * (int64_t)cap, bufcnt + rloc * nw, 0, nw, cap, k, sorted);
@@ -826,6 +846,35 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
         }
         out_idx[rloc * k + i] = ix;
         out_t[rloc * k + i] = tv;
+    }
+    if (!zero_fill || have >= (uint32_t)k) return;
+    // columns of the positive candidates, ascending
+    poscol[threadIdx.x] = (uint32_t)threadIdx.x < have ? 0xFFFFFFFFu - (uint32_t)sorted[threadIdx.x] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int size = 2; size <= kBlock; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int i = threadIdx.x, j = i ^ stride;
+            if (j > i) {
+                const bool asc = (i & size) == 0;
+                const uint32_t a = poscol[i], b = poscol[j];
+                if (asc ? (a > b) : (a < b)) {
+                    poscol[i] = b;
+                    poscol[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint32_t p = 0;
+        int64_t c = 0;
+        for (int i = (int)have; i < k && c < ncols; ++c) {
+            while (p < have && poscol[p] < (uint32_t)c) ++p;
+            if (p < have && poscol[p] == (uint32_t)c) continue;
+            out_idx[rloc * k + i] = (int32_t)c;
+            out_t[rloc * k + i] = 0.0f;
+            ++i;
+        }
     }
 }
 
@@ -1148,9 +1197,9 @@ struct EpiNbr {
 };
 
 // Top-k candidates per (row, wave), as topk_push with the per-row state in
-// LDS.  A pair whose ordered T is below the high word of the row's
-// threshold key cannot enter, so most tiles are rejected with one compare;
-// lane r of Reg::thr holds that high word for row r.
+// LDS.  Lane r of Reg::thr holds row r's threshold key (0 until armed), so
+// a tile in which no key of a row can enter is rejected with one 64-bit
+// compare and a ballot.
 struct EpiTopk {
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
@@ -1158,6 +1207,7 @@ struct EpiTopk {
     uint64_t* buf;
     int32_t* bufcnt;
     int64_t rb;
+    bool pos_only;  // only pairs with T > 0 are candidates
     struct Lds {
         int32_t cnt[kBlock / kWave][32];
         int32_t armed[kBlock / kWave][32];
@@ -1165,7 +1215,7 @@ struct EpiTopk {
         uint32_t hist[kBlock / kWave][256];
     };
     struct Reg {
-        uint32_t thr;
+        uint64_t thr;
     };
     __device__ void init(Lds&) const {}
     __device__ void begin(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
@@ -1175,15 +1225,17 @@ struct EpiTopk {
             L.armed[w][lane] = 0;
             L.theta[w][lane] = 0ull;
         }
-        R.thr = 0u;
+        R.thr = 0ull;
         wave_lds_sync();
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
                           int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave;
         const uint64_t key = topk_key(t, col);
-        const uint32_t thr = (uint32_t)__builtin_amdgcn_readlane((int)R.thr, r);
-        if (__ballot(valid && (uint32_t)(key >> 32) >= thr) == 0ull) return;
+        const uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr >> 32), r) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)R.thr, r);
+        valid = valid && (!pos_only || t > 0.0f);
+        if (__ballot(valid && key > thr) == 0ull) return;
         TopkBuf B{buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap, L.cnt[w][r], L.armed[w][r] != 0,
                   L.theta[w][r]};
         const int before = B.cnt;
@@ -1195,7 +1247,7 @@ struct EpiTopk {
                 L.armed[w][r] = B.armed ? 1 : 0;
                 L.theta[w][r] = B.theta;
             }
-            if (lane_id() == r) R.thr = B.armed ? (uint32_t)(B.theta >> 32) : 0u;
+            if (lane_id() == r) R.thr = B.armed ? B.theta : 0ull;
             wave_lds_sync();
         }
     }
@@ -2118,9 +2170,6 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
 
 int launch_check(const char* name);
 
-// Candidate buffers reduced by one workgroup of the first top-k stage.
-constexpr int64_t kTopkStage = 16;
-
 // Neighbour indices kept per (row, wave) slot by the count pass.
 constexpr int kNbrCap = 16;
 
@@ -2828,25 +2877,38 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const int cap = (int)(((k + kWave) + kWave - 1) / kWave * kWave);
+    // every row on the bit-sliced kernel with consistent magnitudes: T is in
+    // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
+    // final selection fills up with T == 0 columns in index order)
+    const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
     static_assert(((SIML_MAX_TOPK + kWave) + kWave - 1) / kWave <= kTopkRegs, "top-k buffer exceeds wave registers");
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? kBitsTargetWG : 2048);
-    grid.y = std::min<unsigned>(grid.y, 256u);
+    grid.y = std::min<unsigned>(grid.y, 1024u);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
     const size_t cnt_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
-    const int64_t nstage = nw > kTopkStage ? nr * ((nw + kTopkStage - 1) / kTopkStage) : 0;
-    const size_t sbuf_bytes = sizeof(uint64_t) * (size_t)nstage * (size_t)k;
-    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + sbuf_bytes + sizeof(int32_t) * (size_t)nstage));
-    uint64_t* d_buf = ctx->scratch.as<uint64_t>();
-    int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes);
-    uint64_t* d_sbuf = reinterpret_cast<uint64_t*>(ctx->scratch.as<char>() + buf_bytes + cnt_bytes);
-    int32_t* d_scnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + buf_bytes + cnt_bytes + sbuf_bytes);
+    // selection levels: segments of the first level and (ping-pong) of the
+    // later, smaller ones
+    const int64_t ns1 = nw > kTopkFan ? nr * ((nw + kTopkFan - 1) / kTopkFan) : 0;
+    const int64_t ns2 = nw > kTopkFan * kTopkFan ? nr * ((nw + kTopkFan * kTopkFan - 1) / (kTopkFan * kTopkFan)) : 0;
+    const size_t s1_bytes = (sizeof(uint64_t) * (size_t)ns1 * (size_t)k + 255) & ~size_t(255);
+    const size_t s2_bytes = (sizeof(uint64_t) * (size_t)ns2 * (size_t)k + 255) & ~size_t(255);
+    const size_t c1_bytes = (sizeof(int32_t) * (size_t)ns1 + 255) & ~size_t(255);
+    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes + sizeof(int32_t) * (size_t)ns2));
+    char* base = ctx->scratch.as<char>();
+    uint64_t* d_buf = reinterpret_cast<uint64_t*>(base);
+    int32_t* d_cnt = reinterpret_cast<int32_t*>(base + buf_bytes);
+    uint64_t* d_sbuf = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes);
+    uint64_t* d_sbuf2 = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes);
+    int32_t* d_scnt = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes);
+    int32_t* d_scnt2 = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes);
     const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
     if (bp.use) {
-        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase}, "k_bits<topk>"));
+        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only},
+                        "k_bits<topk>"));
         if (!mrows.empty()) {
             k_topk<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
                 ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
@@ -2869,18 +2931,27 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
             TRY(launch_check("k_topk<rowlist>"));
         }
     }
-    if (nw > kTopkStage) {
-        // many buffers per row: reduce groups of kTopkStage buffers first
-        const int ns = (int)((nw + kTopkStage - 1) / kTopkStage);
-        k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(
-            d_buf, d_cnt, (int)nw, cap, k, kTopkStage, d_sbuf, d_scnt);
+    // reduce kTopkFan buffers per workgroup until one workgroup per row can
+    // take the rest
+    const uint64_t* src = d_buf;
+    const int32_t* srccnt = d_cnt;
+    int64_t nseg = nw;
+    int segcap = cap;
+    for (int level = 0; nseg > kTopkFan; ++level) {
+        const int64_t ns = (nseg + kTopkFan - 1) / kTopkFan;
+        uint64_t* dst = level % 2 ? d_sbuf2 : d_sbuf;
+        int32_t* dstcnt = level % 2 ? d_scnt2 : d_scnt;
+        k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg,
+                                                                                        segcap, k, dst, dstcnt);
         TRY(launch_check("k_topk_stage"));
-        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_sbuf, d_scnt, ns, k, k,
-                                                                         ctx->res.as<int32_t>(), ctx->res2.as<float>());
-    } else {
-        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(d_buf, d_cnt, (int)nw, k, cap,
-                                                                         ctx->res.as<int32_t>(), ctx->res2.as<float>());
+        src = dst;
+        srccnt = dstcnt;
+        nseg = ns;
+        segcap = k;
     }
+    k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
+                                                                     pos_only ? 1 : 0, nq, ctx->res.as<int32_t>(),
+                                                                     ctx->res2.as<float>());
     TRY(launch_check("k_topk_final"));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
