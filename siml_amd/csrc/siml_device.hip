@@ -1272,12 +1272,12 @@ __device__ __forceinline__ float pair_tanimoto(float fi, float maf, float mbf, b
     return zero ? 0.0f : d;
 }
 
-template <int NB, bool FAST, class Epi>
-__global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
-                                                 int64_t rb, int64_t rl, Epi epi) {
-    __shared__ uint2 tab[siml::kBitSlots];
-    __shared__ BitRowInfo ri;
-    __shared__ typename Epi::Lds el;
+// Workgroup prologue of the bit-sliced kernels: the group's table and row
+// information into LDS.
+template <class Epi>
+__device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
+                                              int64_t rl, const Epi& epi, uint2* tab, BitRowInfo& ri,
+                                              typename Epi::Lds& el) {
     const int g = G.glist[blockIdx.x];
     const siml::BitGroupHdr hd = G.hdr[g];
     for (int s = threadIdx.x; s < siml::kBitSlots; s += blockDim.x) tab[s] = make_uint2(0u, 0u);
@@ -1307,93 +1307,113 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         tab[e.z] = make_uint2(e.x, e.y);
     }
     __syncthreads();
+}
+
+// Float32 matrix, a 256-column block per wave step (EpiF32Block).  At most
+// 128 VGPRs, so 4 waves share each SIMD.
+template <int NB, bool FAST>
+__global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
+                                                          int64_t rb, int64_t rl, EpiF32Block epi) {
+    __shared__ uint2 tab[siml::kBitSlots];
+    __shared__ BitRowInfo ri;
+    __shared__ EpiF32Block::Lds el;
+    bits_prologue(G, rmag, rb, rl, epi, tab, ri, el);
     const int lane = lane_id();
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
-    if constexpr (Epi::kBlockMode) {
-        const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
-        const int64_t gw = (int64_t)blockIdx.y * wpb + w, nw = (int64_t)gridDim.y * wpb;
-        const int64_t nblocks = (C.ntiles + 3) / 4;
-        const int64_t b0 = (nblocks * gw) / nw, b1 = (nblocks * (gw + 1)) / nw;
-        float* stage = el.stage[w];  // 256 column slots + 64 for lanes past the end
-        for (int64_t blk = b0; blk < b1; ++blk) {
-            uint32_t Q[4][NB];
-            float mbf[4];
-            bool zb[4];
-            int loc[4];
+    const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
+    const int64_t gw = (int64_t)blockIdx.y * wpb + w, nw = (int64_t)gridDim.y * wpb;
+    const int64_t nblocks = (C.ntiles + 3) / 4;
+    const int64_t b0 = (nblocks * gw) / nw, b1 = (nblocks * (gw + 1)) / nw;
+    float* stage = el.stage[w];  // 256 column slots + 64 for lanes past the end
+    for (int64_t blk = b0; blk < b1; ++blk) {
+        uint32_t Q[4][NB];
+        float mbf[4];
+        bool zb[4];
+        int loc[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int64_t t = blk * 4 + q;
-                const int64_t c = t * kWave + lane;
-                if (t < C.ntiles) {
-                    bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], Q[q]);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) Q[q][k] = 0u;
-                }
-                transpose8(Q[q]);
-                if constexpr (NB == 16) transpose8(Q[q] + 8);
-                const bool valid = c < C.n;
-                const int mb = valid ? C.mag[c] : 0;
-                mbf[q] = (float)mb;
-                zb[q] = mb == 0;
-                loc[q] = valid ? (int)(col_of(C, c) - blk * 4 * kWave) : 4 * kWave + lane;
-            }
-            const int64_t cbase = blk * 4 * kWave;
-            const bool full = cbase + 4 * kWave <= C.n;
-#pragma unroll
-            for (int r = 0; r < siml::kBitRows; ++r) {
-                const int32_t row = ri.row[r];
-                if (row < 0) continue;
-                const bool za = ri.ma[r] == 0;
-                const float maf = ri.maf[r];
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    stage[loc[q]] = pair_tanimoto<FAST>(count_of<NB>(Q[q], r), maf, mbf[q], za || zb[q]);
-                wave_lds_sync();
-                float* orow = epi.out + (int64_t)(row - epi.rb) * epi.opitch + cbase;
-                if (full) {
-                    reinterpret_cast<float4*>(orow)[lane] = reinterpret_cast<const float4*>(stage)[lane];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int o = 4 * lane + i;
-                        if (cbase + o < C.n) orow[o] = stage[o];
-                    }
-                }
-                wave_lds_sync();
-            }
-        }
-    } else {
-        typename Epi::Reg R;
-        int64_t t0, t1, gw, nw;
-        wave_tiles(C.ntiles, t0, t1, gw, nw);
-        epi.begin(el, R, ri, gw, nw);
-        for (int64_t t = t0; t < t1; ++t) {
-            if constexpr (Epi::kPrune) {
-                if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
-            }
+        for (int q = 0; q < 4; ++q) {
+            const int64_t t = blk * 4 + q;
             const int64_t c = t * kWave + lane;
-            uint32_t P[NB];
-            bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
-            transpose8(P);
-            if constexpr (NB == 16) transpose8(P + 8);
+            if (t < C.ntiles) {
+                bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], Q[q]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NB; ++k) Q[q][k] = 0u;
+            }
+            transpose8(Q[q]);
+            if constexpr (NB == 16) transpose8(Q[q] + 8);
             const bool valid = c < C.n;
             const int mb = valid ? C.mag[c] : 0;
-            const float mbf = (float)mb;
-            const int64_t col = valid ? col_of(C, c) : 0;
-#pragma unroll
-            for (int r = 0; r < siml::kBitRows; ++r) {
-                const int32_t row = ri.row[r];
-                if (row < 0) continue;
-                const int ma = ri.ma[r];
-                const float tv = pair_tanimoto<FAST>(count_of<NB>(P, r), ri.maf[r], mbf, ma == 0 || mb == 0);
-                epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw);
-            }
+            mbf[q] = (float)mb;
+            zb[q] = mb == 0;
+            loc[q] = valid ? (int)(col_of(C, c) - blk * 4 * kWave) : 4 * kWave + lane;
         }
-        epi.end(el, R, ri, gw, nw);
-        __syncthreads();
-        epi.finish(el, ri);
+        const int64_t cbase = blk * 4 * kWave;
+        const bool full = cbase + 4 * kWave <= C.n;
+#pragma unroll
+        for (int r = 0; r < siml::kBitRows; ++r) {
+            const int32_t row = ri.row[r];
+            if (row < 0) continue;
+            const bool za = ri.ma[r] == 0;
+            const float maf = ri.maf[r];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                stage[loc[q]] = pair_tanimoto<FAST>(count_of<NB>(Q[q], r), maf, mbf[q], za || zb[q]);
+            wave_lds_sync();
+            float* orow = epi.out + (int64_t)(row - epi.rb) * epi.opitch + cbase;
+            if (full) {
+                reinterpret_cast<float4*>(orow)[lane] = reinterpret_cast<const float4*>(stage)[lane];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int o = 4 * lane + i;
+                    if (cbase + o < C.n) orow[o] = stage[o];
+                }
+            }
+            wave_lds_sync();
+        }
     }
+}
+
+// Every other epilogue: each wave walks a contiguous range of tiles.
+template <int NB, bool FAST, class Epi>
+__global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
+                                                 int64_t rb, int64_t rl, Epi epi) {
+    __shared__ uint2 tab[siml::kBitSlots];
+    __shared__ BitRowInfo ri;
+    __shared__ typename Epi::Lds el;
+    bits_prologue(G, rmag, rb, rl, epi, tab, ri, el);
+    const int lane = lane_id();
+    const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
+    typename Epi::Reg R;
+    int64_t t0, t1, gw, nw;
+    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    epi.begin(el, R, ri, gw, nw);
+    for (int64_t t = t0; t < t1; ++t) {
+        if constexpr (Epi::kPrune) {
+            if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
+        }
+        const int64_t c = t * kWave + lane;
+        uint32_t P[NB];
+        bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
+        transpose8(P);
+        if constexpr (NB == 16) transpose8(P + 8);
+        const bool valid = c < C.n;
+        const int mb = valid ? C.mag[c] : 0;
+        const float mbf = (float)mb;
+        const int64_t col = valid ? col_of(C, c) : 0;
+#pragma unroll
+        for (int r = 0; r < siml::kBitRows; ++r) {
+            const int32_t row = ri.row[r];
+            if (row < 0) continue;
+            const int ma = ri.ma[r];
+            const float tv = pair_tanimoto<FAST>(count_of<NB>(P, r), ri.maf[r], mbf, ma == 0 || mb == 0);
+            epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw);
+        }
+    }
+    epi.end(el, R, ri, gw, nw);
+    __syncthreads();
+    epi.finish(el, ri);
 }
 
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
@@ -2191,14 +2211,25 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
     const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
     const int32_t* rmag = ctx->ref.mag.as<int32_t>();
     const ColArgs C = ctx->qry->args();
-    if (p.nb == 8 && fast)
-        k_bits<8, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
-    else if (p.nb == 8)
-        k_bits<8, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
-    else if (fast)
-        k_bits<16, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
-    else
-        k_bits<16, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+    if constexpr (Epi::kBlockMode) {
+        if (p.nb == 8 && fast)
+            k_bits_block<8, true><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else if (p.nb == 8)
+            k_bits_block<8, false><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else if (fast)
+            k_bits_block<16, true><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else
+            k_bits_block<16, false><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+    } else {
+        if (p.nb == 8 && fast)
+            k_bits<8, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else if (p.nb == 8)
+            k_bits<8, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else if (fast)
+            k_bits<16, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else
+            k_bits<16, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+    }
     return launch_check(name);
 }
 
@@ -2884,7 +2915,7 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     static_assert(((SIML_MAX_TOPK + kWave) + kWave - 1) / kWave <= kTopkRegs, "top-k buffer exceeds wave registers");
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? kBitsTargetWG : 2048);
-    grid.y = std::min<unsigned>(grid.y, 1024u);
+    grid.y = std::min<unsigned>(grid.y, 256u);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
