@@ -271,9 +271,16 @@ def workload_batched(args):
     configure(g, args)
     g.attach_db(db)
     h = g._h
-    # warm-up: one small set through the whole path
-    g.set_refsmiles_ragged(*sets[0])
-    g.getTopK(0, int(sizes[0]), k)
+    if args.autotune:
+        g.set_autotune(True)
+    # warm-up (and, with the autotuner, shape tuning): every set once and
+    # the batched call once
+    log("warm-up pass over the sets and the batched call")
+    for s in sets:
+        g.set_refsmiles_ragged(*s)
+        g.getTopK(0, len(s[2]) - 1, k)
+    g.set_refsmiles_ragged(*q)
+    g.getTopK(0, nq, k)
     pairs = nq * ndb
 
     h.call("siml_set_timing", 1)
@@ -295,6 +302,7 @@ def workload_batched(args):
     t_batched = time.perf_counter() - t0
     kms_b, n_b, _ = h.kernel_stats(reset=True)
     kernel = h.last_kernel()
+    shape = g.launch_shape()
     log(f"batched: {t_batched:.2f} s wall, kernels {kms_b:.0f} ms over {n_b} launches ({kernel})")
 
     pi = np.concatenate([p[0] for p in plain])
@@ -325,6 +333,8 @@ def workload_batched(args):
         "plain_equals_batched": same,
         "oracle_rows_match": parity,
         "kernel": kernel,
+        "autotune": bool(args.autotune),
+        "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1]},
         "roofline": {"bound": "valu", "achieved": ops / kernel_s / 1e12, "peak": P_VALU / 1e12,
                      "unit": "Tops/s (int32 lane-ops)", "frac": ops / kernel_s / P_VALU, "traffic": None},
     })
@@ -430,11 +440,21 @@ def workload_serve(args):
     np.cumsum(sizes, out=bounds[1:])
     rates = [float(r) for r in args.rates.split(",")]
     results = []
-    with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30) as srv:
-        # warm-up: one request of each kind
-        for op, prm in (("topk", 50), ("threshold", 0.8)):
-            s = synth.ragged_slice(*pool, 0, 4)
-            srv.submit(Request(-1, *s, op, prm)).event.wait()
+    with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30,
+                autotune=args.autotune) as srv:
+        # warm-up: one request of each kind per power-of-two size up to the
+        # batch limit (with the autotuner this also fixes every shape bucket
+        # before the timed arrivals)
+        log("warm-up requests" + (" (tuning launch shapes)" if args.autotune else ""))
+        n = 1
+        while n <= args.max_rows:
+            for op, prm in (("topk", 50), ("threshold", 0.8)):
+                s = synth.ragged_slice(*pool, 0, min(n, len(pool[2]) - 1))
+                r = srv.submit(Request(-1, *s, op, prm))
+                r.event.wait()
+                if r.error is not None:
+                    raise r.error
+            n *= 2
         for rate in rates:
             reqs = []
             for i in range(nreq):
@@ -484,6 +504,8 @@ def main():
     ap.add_argument("--max-rows", type=int, default=1024, help="serve: query molecules per launch")
     ap.add_argument("--mem-budget", type=int, default=16, help="serve: admission budget, GiB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="batched/serve: launch shapes chosen by the autotuner (1) or the defaults (0)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--traffic", type=float, default=None,
                     help="all-pairs: HBM bytes per launch from counter runs, reported as roofline.traffic")

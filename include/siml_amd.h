@@ -206,6 +206,16 @@ int siml_set_policy(siml_ctx* ctx, int policy);
 int siml_set_layout(siml_ctx* ctx, int layout);
 /* Name of the kernel variant the last similarity launch used. */
 const char* siml_last_kernel(siml_ctx* ctx);
+/* Launch-shape autotuner (no counterpart in the reference).  When on, the
+ * first multirow / hist_rows / neighbors / topk / hist1000 call of a shape
+ * bucket (operation, log2 of the rows, log2 of the query tiles, k) runs once
+ * per candidate launch shape -- workgroups aimed at per launch and, for
+ * top-k, the most waves per row -- times each with events and keeps the
+ * fastest for the bucket, shared by every handle on the device.  Results do
+ * not depend on the shape.  Off by default.  siml_launch_shape reports the
+ * shape the handle used last. */
+int siml_set_autotune(siml_ctx* ctx, int on);
+int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap);
 /* Self-test of a short float32 division method of the kernels (no
  * counterpart in the reference): compares it on the device with IEEE
  * division for every integer numerator in [0, 65536) and denominator in

@@ -129,6 +129,19 @@ class GPULingo:
         self.qshape = (self.nquery, 0)
 
 
+    def set_autotune(self, on=True):
+        """Launch shapes chosen by measurement on the first call of each
+        shape bucket (results are unchanged; see siml_set_autotune)."""
+        self._h.call("siml_set_autotune", 1 if on else 0)
+
+    def launch_shape(self):
+        """(workgroups aimed at per launch, most waves per top-k row) of
+        the last call."""
+        import ctypes
+        t, c = ctypes.c_int64(), ctypes.c_int32()
+        self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c))
+        return t.value, c.value
+
     # -- helpers -----------------------------------------------------------
     def _rows(self, rowbase, rowlimit):
         rowbase, rowlimit = int(rowbase), int(rowlimit)

@@ -22,10 +22,12 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "bitgroups.h"
@@ -1801,6 +1803,11 @@ struct siml_ctx {
     std::string last_kernel = "none";
     int policy = SIML_POLICY_AUTO;
     int layout = SIML_LAYOUT_BLOCK;
+    // launch shape of the bit-sliced kernels: workgroups aimed at per launch
+    // and, for top-k, the most waves per row (chosen by the autotuner when on)
+    int64_t target_wg = 8192;
+    unsigned topk_wave_cap = 256;
+    bool autotune = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
@@ -2192,10 +2199,6 @@ int launch_check(const char* name);
 
 // Neighbour indices kept per (row, wave) slot by the count pass.
 constexpr int kNbrCap = 16;
-
-// Workgroups per bit-sliced launch: ~8 rounds of the 1024 that fit at once
-// (256 CUs x 4), so the last, partial round costs little.
-constexpr int64_t kBitsTargetWG = 8192;
 
 // Launches k_bits<NB, Epi> over the plan's groups with grid.y = gy.
 template <class Epi>
@@ -2599,7 +2602,7 @@ int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery) {
     return SIML_OK;
 }
 
-int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleRows"));
     TRY(set_device(ctx));
     clear_result(ctx);
@@ -2619,7 +2622,7 @@ int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
-        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
         if (ctx->qry->blocked)
             TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32Block{ctx->res.as<float>(), dp, rowbase},
                             "k_bits<f32 block>"));
@@ -2685,7 +2688,7 @@ int siml_single_row_async(siml_ctx* ctx, int64_t row) {
     return SIML_OK;
 }
 
-int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+static int hist_rows_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleHistogrammedRows"));
     TRY(set_device(ctx));
     clear_result(ctx);
@@ -2704,7 +2707,7 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
-        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
         TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiHist101{hist, rowbase}, "k_bits<hist101>"));
         if (!bp.merge.empty()) {
             k_hist_rows<<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
@@ -2741,7 +2744,7 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     return SIML_OK;
 }
 
-int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float lo, float hi) {
+static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float lo, float hi) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getNeighbors"));
     TRY(set_device(ctx));
     clear_result(ctx);
@@ -2764,7 +2767,7 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
     if (bp.use) {
         // bit-sliced groups hold up to 28 rows: size grid.y by groups
-        const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, kBitsTargetWG).y;
+        const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
         grid.y = (unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, (int64_t(1) << 24) / (nr * (kBlock / kWave))));
     }
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
@@ -2887,7 +2890,7 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
     return SIML_OK;
 }
 
-int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
+static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getTopK"));
     if (k < 1 || k > SIML_MAX_TOPK) return fail(SIML_EINVAL, "getTopK: k=%d outside [1, %d]", k, SIML_MAX_TOPK);
     if (ctx->qry->n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "getTopK: more than 2^31-1 query molecules");
@@ -2914,8 +2917,8 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
     static_assert(((SIML_MAX_TOPK + kWave) + kWave - 1) / kWave <= kTopkRegs, "top-k buffer exceeds wave registers");
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
-                         bp.use ? kBitsTargetWG : 2048);
-    grid.y = std::min<unsigned>(grid.y, 256u);
+                         bp.use ? ctx->target_wg : 2048);
+    grid.y = std::min<unsigned>(grid.y, ctx->topk_wave_cap);
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
@@ -2989,7 +2992,7 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
     return SIML_OK;
 }
 
-int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getGlobalHistogram"));
     TRY(set_device(ctx));
     clear_result(ctx);
@@ -3010,7 +3013,7 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
-                         bp.use ? kBitsTargetWG : 2048);
+                         bp.use ? ctx->target_wg : 2048);
     // one workgroup counts at most 28 rows x its columns in uint32 LDS bins
     const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
@@ -3040,6 +3043,139 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     }
     tr.done();
     ctx->last_kernel = bp.use ? "hist1000_bits" : (plan.all_merge ? "hist1000_merge" : "hist1000_hash");
+    return SIML_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner.  With siml_set_autotune(ctx, 1), the first call of
+// an operation for a shape bucket (operation, log2 of the rows, log2 of the
+// query tiles, k) runs the operation once per candidate shape, timed with
+// events on the handle's stream, and keeps the fastest for the bucket (shared
+// by every handle of the process on that device).  Results do not depend on
+// the shape, so the last run's result is the call's result.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+enum TuneOp { kTuneMultirow = 1, kTuneHistRows, kTuneNeighbors, kTuneTopk, kTuneHist1000 };
+
+struct TuneShape {
+    int64_t target_wg;
+    unsigned topk_wave_cap;
+};
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, TuneShape> g_tune;
+
+int log2_bucket(int64_t x) {
+    int b = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+template <class F>
+int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
+    if (!ctx->autotune || ctx->qry->n < 0) return impl();
+    const auto key = std::make_tuple(ctx->device, op, log2_bucket(rows), log2_bucket(ctx->qry->ntiles), k);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            ctx->target_wg = it->second.target_wg;
+            ctx->topk_wave_cap = it->second.topk_wave_cap;
+            return impl();
+        }
+    }
+    static const TuneShape kCands[] = {{8192, 256}, {2048, 256}, {4096, 256}, {16384, 256}, {32768, 1024},
+                                       {8192, 64}};
+    const int ncand = op == kTuneTopk ? 6 : 5;
+    TRY(set_device(ctx));
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(SIML_EDEVICE, "hipEventCreate failed");
+    }
+    TuneShape best = kCands[0];
+    float best_ms = 0.f;
+    int rc = SIML_OK;
+    for (int i = 0; i < ncand && rc == SIML_OK; ++i) {
+        ctx->target_wg = kCands[i].target_wg;
+        ctx->topk_wave_cap = kCands[i].topk_wave_cap;
+        (void)hipEventRecord(a, ctx->stream);
+        rc = impl();
+        if (rc != SIML_OK) break;
+        (void)hipEventRecord(b, ctx->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            rc = fail(SIML_EDEVICE, "timing a candidate launch shape failed");
+            break;
+        }
+        if (i == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = kCands[i];
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != SIML_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    // later calls of the bucket use `best`; this call's result (the last
+    // candidate's) is the same
+    ctx->target_wg = best.target_wg;
+    ctx->topk_wave_cap = best.topk_wave_cap;
+    return SIML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int siml_multirow_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    return tuned_call(ctx, kTuneMultirow, rowlimit - rowbase, 0, [&] { return multirow_async_impl(ctx, rowbase, rowlimit); });
+}
+
+int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    return tuned_call(ctx, kTuneHistRows, rowlimit - rowbase, 0, [&] { return hist_rows_async_impl(ctx, rowbase, rowlimit); });
+}
+
+int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float lo, float hi) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    return tuned_call(ctx, kTuneNeighbors, rowlimit - rowbase, 0,
+                      [&] { return neighbors_async_impl(ctx, rowbase, rowlimit, lo, hi); });
+}
+
+int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    return tuned_call(ctx, kTuneTopk, rowlimit - rowbase, k, [&] { return topk_async_impl(ctx, rowbase, rowlimit, k); });
+}
+
+int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    return tuned_call(ctx, kTuneHist1000, rowlimit - rowbase, 0, [&] { return hist1000_async_impl(ctx, rowbase, rowlimit); });
+}
+
+int siml_set_autotune(siml_ctx* ctx, int on) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    ctx->autotune = on != 0;
+    if (!ctx->autotune) {
+        ctx->target_wg = 8192;
+        ctx->topk_wave_cap = 256;
+    }
+    return SIML_OK;
+}
+
+int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap) {
+    if (!ctx || !target_wg || !topk_wave_cap) return fail(SIML_EINVAL, "null pointer");
+    *target_wg = ctx->target_wg;
+    *topk_wave_cap = (int32_t)ctx->topk_wave_cap;
     return SIML_OK;
 }
 

@@ -72,7 +72,7 @@ def concat_requests(reqs):
 class Server:
     """Continuous batching over ``slots`` comparators sharing ``db``."""
 
-    def __init__(self, db, slots=4, max_rows=2048, mem_budget=8 << 30, device=0):
+    def __init__(self, db, slots=4, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False):
         self.db = db
         self.max_rows = int(max_rows)
         self.mem_budget = int(mem_budget)
@@ -86,6 +86,8 @@ class Server:
         for _ in range(slots):
             g = GPULingo(device)
             g.attach_db(db)
+            if autotune:
+                g.set_autotune(True)
             self.handles.append(g)
         for i in range(slots):
             t = threading.Thread(target=self._worker, args=(i,), daemon=True)

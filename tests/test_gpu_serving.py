@@ -151,3 +151,31 @@ def test_db_save_load_roundtrip(db_and_queries, tmp_path):
     assert np.array_equal(a[0], b[0]) and np.array_equal(bits(a[1]), bits(b[1]))
     db.close()
     db2.close()
+
+
+def test_autotune_changes_no_result(db_and_queries):
+    """Launch shapes chosen by the autotuner give the same bits as the
+    default shapes for every operation."""
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set)
+    sub = synth.ragged_slice(*q, 0, 90)
+    out = []
+    for tune in (False, True, True):      # the second tuned pass reuses the cached shapes
+        g = GPULingo(0)
+        g.attach_db(db)
+        g.set_autotune(tune)
+        g.set_refsmiles_ragged(*sub)
+        r = [g.getMultipleRows(0, 90).view(np.int32)]
+        r += list(g.getTopK(0, 90, 30))
+        r += list(g.getNeighborsCSR(0, 90, 0.5))
+        r.append(g.getGlobalHistogram(0, 90))
+        r.append(g.getMultipleHistogrammedRows(0, 90))
+        out.append(r)
+        if tune:
+            t, c = g.launch_shape()
+            assert t > 0 and c > 0
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(np.asarray(a).view(np.int32) if np.asarray(a).dtype == np.float32 else a,
+                                  np.asarray(b).view(np.int32) if np.asarray(b).dtype == np.float32 else b)
+    db.close()
