@@ -14,6 +14,9 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 rows) timed on a bounded sample of the same workload
 
 Other workloads (--workload), each one JSON line:
+  zinc       real SMILES: the reference's examples/demo4096.smi (4096 ZINC
+             molecules) cycled to --n molecules, all-pairs float32 matrix and
+             top-100, checked bit-exact on sampled rows.
   cpu4096    the reference's CPU case through the CPULingo interface: 4096
              strings U[20,80] (seed 1), the 4096 x 4096 matrix in 256-row
              blocks with the reference's default magnitudes, checked
@@ -52,6 +55,9 @@ N_ALLPAIRS = 32768
 # 4,264,020 KiB x 1.0, the factors being what the same counters read for a
 # device copy of known size (profiles/k_bits_f32block_allpairs32768.md).
 TRAFFIC_ALLPAIRS = (2.0 * 1051660 + 1.0 * 4264020) * 1024
+# VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 8.763e8
+# wave-instructions x 4 cycles / (1024 SIMDs x 2.4 GHz x its duration).
+VALU_INSTS_ALLPAIRS = 8.763e8
 
 
 def log(msg):
@@ -164,7 +170,8 @@ def workload_allpairs(args):
                      "unit": "Tops/s (int32 lane-ops)", "frac": achieved / P_VALU, "traffic": traffic,
                      "alg_ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
                      "hbm_frac": bytes_alg / kernel_s / B_HBM,
-                     "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None},
+                     "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
+                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 4 / (1024 * 2.4e9 * kernel_s)) if traffic else None},
     })
     if not args.no_cpu:
         import oracle
@@ -177,6 +184,64 @@ def workload_allpairs(args):
                                                                                  want.view(np.int32)))}
         log("timing the CPU oracle on a bounded sample")
         rec["cpu_baseline"] = cpu_baseline(s, args.cpu_budget)
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# real SMILES: the reference's 4096 ZINC molecules, cycled to 32768
+# ---------------------------------------------------------------------------
+def workload_zinc(args):
+    import oracle
+    from siml_amd import _lib, compiler
+    from siml_amd.GPULingo import GPULingo
+
+    path = os.path.join(ROOT, "tests", "golden", "demo4096.smi")
+    with open(path, "rb") as f:
+        lines = f.readlines()
+    n = args.n
+    lines = [lines[i % len(lines)] for i in range(n)]
+    s = compiler.cSMILEStoMatrices(lines)[:4]
+    lens = s[2].ravel().astype(np.int64)
+    g = GPULingo(0)
+    configure(g, args)
+    g.set_refsmiles(*s)
+    g.set_qsmiles(*s)
+    h = g._h
+    if args.autotune:
+        g.set_autotune(True)
+        g.getTopK(0, n, 100)            # tunes the top-k shape before it is timed
+    for _ in range(args.warmup):
+        h.call("siml_multirow_async", 0, n)
+        h.call("siml_synchronize")
+    h.call("siml_set_timing", 1)
+    h.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.call("siml_multirow_async", 0, n)
+    h.call("siml_synchronize")
+    wall = time.perf_counter() - t0
+    kms, launches, _ = h.kernel_stats(reset=True)
+    rows = [0, 1, n // 2, n - 1]
+    peek = np.empty((len(rows), n), np.float32)
+    for i, r in enumerate(rows):
+        h.call("siml_peek_rows_f32", r, r + 1, _lib.f32p(peek[i:i + 1]), n)
+    want = oracle.tanimoto_block(tuple(x[rows] for x in s), s, nthreads=min(os.cpu_count() or 1, 16))
+    t0 = time.perf_counter()
+    ti, tt = g.getTopK(0, n, 100)
+    t_topk = time.perf_counter() - t0
+    oi, ot = oracle.topk(tuple(x[rows] for x in s), s, 100, nthreads=min(os.cpu_count() or 1, 16))
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), real SMILES", n * n * args.steps / wall,
+                      "pairs/s", wall, {"workload": f"all-pairs {n} x {n} of the reference's demo4096.smi ZINC "
+                                                    f"molecules cycled, full float32 matrix in HBM", "rows": n,
+                                        "columns": n})
+    rec["data"] = "examples/demo4096.smi of the reference (4096 ZINC SMILES), cycled"
+    rec.update({"kernel": h.last_kernel(), "kernel_ms_mean": kms / max(launches, 1),
+                "mean_unique_lingos": float(lens.mean()),
+                "topk100_s": t_topk, "topk100_pairs_per_s": n * n / t_topk,
+                "parity": {"rows_checked": rows,
+                           "bit_exact": bool(np.array_equal(peek.view(np.int32), want.view(np.int32))),
+                           "topk_equal": bool(np.array_equal(ti[rows], oi) and
+                                              np.array_equal(tt[rows].view(np.int32), ot.view(np.int32)))}})
     return rec
 
 
@@ -491,7 +556,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
-    ap.add_argument("--workload", choices=["allpairs", "cpu4096", "batched", "threshold", "serve"], default="allpairs")
+    ap.add_argument("--workload", choices=["allpairs", "zinc", "cpu4096", "batched", "threshold", "serve"],
+                    default="allpairs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -521,7 +587,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
-    rec = {"allpairs": workload_allpairs, "cpu4096": workload_cpu4096, "batched": workload_batched,
+    rec = {"allpairs": workload_allpairs, "zinc": workload_zinc, "cpu4096": workload_cpu4096,
+           "batched": workload_batched,
            "threshold": workload_threshold, "serve": workload_serve}[args.workload](args)
     print(json.dumps(rec), flush=True)
     return 0

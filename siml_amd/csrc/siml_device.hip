@@ -482,9 +482,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Keys one lane holds while a wave re-selects its candidate buffer: the
-// buffer capacity is at most kTopkRegs * 64.
-constexpr int kTopkRegs = 6;
+// Candidate buffer of one (row, wave): k + kTopkSlack keys rounded up to a
+// multiple of 64, so a re-selection happens at most every kTopkSlack / 64
+// pushes; a lane holds kTopkRegs keys while the wave re-selects.
+constexpr int kTopkSlack = 128;
+constexpr int kTopkRegs = (SIML_MAX_TOPK + kTopkSlack + 63) / 64;
 
 // Replaces the n > k distinct keys of buf[0..n) by the k largest (in any
 // order) and returns the k-th largest.  One wave: the keys are read into
@@ -521,9 +523,11 @@ __device__ uint64_t wave_keep_topk(uint64_t* buf, int n, int k, uint32_t* hist) 
         const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
                                 (uint32_t)__shfl(h3, L)};
         int d = 4 * L;
+        uint32_t cnt_d = hb[0];
         for (int q = 3; q >= 0; --q) {
             if (above + hb[q] >= need) {
                 d = 4 * L + q;
+                cnt_d = hb[q];
                 break;
             }
             above += hb[q];
@@ -532,6 +536,21 @@ __device__ uint64_t wave_keep_topk(uint64_t* buf, int n, int k, uint32_t* hist) 
         prefix |= (uint64_t)d << shift;
         pmask |= (uint64_t)255 << shift;
         wave_lds_sync();
+        if (cnt_d == need) {
+            // every key of the chosen bucket is needed: the k-th largest is
+            // the bucket's smallest key
+            uint64_t mn = ~0ull;
+#pragma unroll
+            for (int j = 0; j < kTopkRegs; ++j)
+                if (v[j] != 0ull && (v[j] & pmask) == prefix && v[j] < mn) mn = v[j];
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(mn >> 32), off) << 32) |
+                                   (uint32_t)__shfl_xor((int)(uint32_t)mn, off);
+                mn = o < mn ? o : mn;
+            }
+            prefix = mn;
+            break;
+        }
     }
     // the keys >= prefix are exactly the k largest (keys are distinct)
     int w = 0;
@@ -701,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int
 // Candidate buffers one selection workgroup takes (its keys are staged in
 // LDS: kTopkFan x (SIML_MAX_TOPK + 64) keys = 40 KiB).
 constexpr int kTopkFan = 16;
-constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kWave);
+constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kTopkSlack);
 
 // Workgroup radix select over the candidate segments [s0, s1) of one row
 // (segment w holds cnt[w] keys at buf + w * cap, s1 - s0 <= kTopkFan): the
@@ -1117,7 +1136,7 @@ struct EpiHist1000 {
     __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const { R.zero = 0u; }
     __device__ void apply(Lds& L, Reg& R, int, int32_t, bool valid, int64_t, int, int, float t, int64_t,
                           int64_t) const {
-        const bool z = !(t >= L.edges[1]);
+        const bool z = !(t >= 1.0f / 1000.0f);  // edge 1 of stage_edges, fl32(1/1000)
         R.zero += (valid && z) ? 1u : 0u;
         const bool nz = valid && !z;
         if (__ballot(nz) != 0ull && nz) atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
@@ -2910,12 +2929,13 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    const int cap = (int)(((k + kWave) + kWave - 1) / kWave * kWave);
+    const int cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
     // every row on the bit-sliced kernel with consistent magnitudes: T is in
     // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
     // final selection fills up with T == 0 columns in index order)
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
-    static_assert(((SIML_MAX_TOPK + kWave) + kWave - 1) / kWave <= kTopkRegs, "top-k buffer exceeds wave registers");
+    static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
+                  "top-k buffer exceeds wave registers");
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? ctx->target_wg : 2048);
     grid.y = std::min<unsigned>(grid.y, ctx->topk_wave_cap);
