@@ -399,7 +399,8 @@ def workload_batched(args):
         "oracle_rows_match": parity,
         "kernel": kernel,
         "autotune": bool(args.autotune),
-        "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1]},
+        "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1],
+                                 "topk_seed": shape[2]},
         "roofline": {"bound": "valu", "achieved": ops / kernel_s / 1e12, "peak": P_VALU / 1e12,
                      "unit": "Tops/s (int32 lane-ops)", "frac": ops / kernel_s / P_VALU, "traffic": None},
     })

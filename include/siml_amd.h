@@ -213,9 +213,13 @@ const char* siml_last_kernel(siml_ctx* ctx);
  * top-k, the most waves per row -- times each with events and keeps the
  * fastest for the bucket, shared by every handle on the device.  Results do
  * not depend on the shape.  Off by default.  siml_launch_shape reports the
- * shape the handle used last. */
+ * shape the handle used last; siml_set_launch_shape fixes one (and turns
+ * the autotuner off): workgroups aimed at per launch, most waves per top-k
+ * row, and whether top-k first seeds its row thresholds from every 16th
+ * query tile. */
 int siml_set_autotune(siml_ctx* ctx, int on);
-int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap);
+int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap, int32_t* topk_seed);
+int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed);
 /* Self-test of a short float32 division method of the kernels (no
  * counterpart in the reference): compares it on the device with IEEE
  * division for every integer numerator in [0, 65536) and denominator in

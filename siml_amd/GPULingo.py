@@ -135,12 +135,16 @@ class GPULingo:
         self._h.call("siml_set_autotune", 1 if on else 0)
 
     def launch_shape(self):
-        """(workgroups aimed at per launch, most waves per top-k row) of
-        the last call."""
+        """(workgroups aimed at per launch, most waves per top-k row,
+        top-k seeding) of the last call."""
         import ctypes
-        t, c = ctypes.c_int64(), ctypes.c_int32()
-        self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c))
-        return t.value, c.value
+        t, c, sd = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c), ctypes.byref(sd))
+        return t.value, c.value, bool(sd.value)
+
+    def set_launch_shape(self, target_wg=8192, topk_wave_cap=256, topk_seed=False):
+        """Fixes the launch shape (turns the autotuner off)."""
+        self._h.call("siml_set_launch_shape", int(target_wg), int(topk_wave_cap), 1 if topk_seed else 0)
 
     # -- helpers -----------------------------------------------------------
     def _rows(self, rowbase, rowlimit):

@@ -103,7 +103,8 @@ struct ColArgs {
     const int32_t* btlen;
     const int2* tmag;      // (min, max) magnitude of the molecules of each tile
     int64_t n;
-    int64_t ntiles;
+    int64_t ntiles;        // tiles visited by the tile-mode bit-sliced kernel ...
+    int64_t tstride;       // ... namely tiles 0, tstride, 2 tstride, ... (a sample when > 1)
 };
 
 __device__ __forceinline__ int64_t col_of(const ColArgs& C, int64_t p) { return C.perm ? (int64_t)C.perm[p] : p; }
@@ -899,6 +900,18 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
     }
 }
 
+// Per-row seed threshold from a top-k over a sample of the columns: the key
+// of the row's k-th entry.  The k listed columns are distinct and each one's
+// key over the whole set is at least its listed key (a zero-filled entry
+// lists T = 0), so at least k keys of the full row are >= the seed.
+__global__ void k_topk_seed(const int32_t* __restrict__ idx, const float* __restrict__ t, int64_t nr, int k,
+                            uint64_t* __restrict__ seed) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int32_t c = idx[r * k + k - 1];
+    seed[r] = c >= 0 ? topk_key(t[r * k + k - 1], c) : 0ull;
+}
+
 // ---------------------------------------------------------------------------
 // Bit-sliced multi-row kernels (row groups of bitgroups.h)
 //
@@ -1228,7 +1241,8 @@ struct EpiTopk {
     uint64_t* buf;
     int32_t* bufcnt;
     int64_t rb;
-    bool pos_only;  // only pairs with T > 0 are candidates
+    bool pos_only;           // only pairs with T > 0 are candidates
+    const uint64_t* theta0;  // per row: a key at most the row's k-th largest (0: none), or null
     struct Lds {
         int32_t cnt[kBlock / kWave][32];
         int32_t armed[kBlock / kWave][32];
@@ -1239,14 +1253,20 @@ struct EpiTopk {
         uint64_t thr;
     };
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
+    __device__ void begin(Lds& L, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
+        // a seeded row admits only keys >= its seed: theta = seed - 1
+        uint64_t th = 0ull;
+        if (theta0 && lane < 32 && ri.row[lane] >= 0) {
+            const uint64_t s0 = theta0[(int64_t)ri.row[lane] - rb];
+            th = s0 ? s0 - 1 : 0ull;
+        }
         if (lane < 32) {
             L.cnt[w][lane] = 0;
-            L.armed[w][lane] = 0;
-            L.theta[w][lane] = 0ull;
+            L.armed[w][lane] = th ? 1 : 0;
+            L.theta[w][lane] = th;
         }
-        R.thr = 0ull;
+        R.thr = th;
         wave_lds_sync();
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
@@ -1410,7 +1430,8 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
     int64_t t0, t1, gw, nw;
     wave_tiles(C.ntiles, t0, t1, gw, nw);
     epi.begin(el, R, ri, gw, nw);
-    for (int64_t t = t0; t < t1; ++t) {
+    for (int64_t ti = t0; ti < t1; ++ti) {
+        const int64_t t = ti * C.tstride;
         if constexpr (Epi::kPrune) {
             if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
         }
@@ -1782,7 +1803,7 @@ struct ColSet {
     ColArgs args() const {
         return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
                 csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), btiles.as<int2>(), btoff.as<int64_t>(),
-                btlen.as<int32_t>(), tmag.as<int2>(), n, ntiles};
+                btlen.as<int32_t>(), tmag.as<int2>(), n, ntiles, 1};
     }
     void release() {
         tiles.release();
@@ -1826,6 +1847,7 @@ struct siml_ctx {
     // and, for top-k, the most waves per row (chosen by the autotuner when on)
     int64_t target_wg = 8192;
     unsigned topk_wave_cap = 256;
+    bool topk_seed = false;  // top-k: seed row thresholds from a column sample first
     bool autotune = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
@@ -2216,13 +2238,16 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
 
 int launch_check(const char* name);
 
+// Top-k seeding samples every kSeedStride-th tile.
+constexpr int64_t kSeedStride = 16;
+
 // Neighbour indices kept per (row, wave) slot by the count pass.
 constexpr int kNbrCap = 16;
 
 // Launches k_bits<NB, Epi> over the plan's groups with grid.y = gy.
 template <class Epi>
 int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_t rl, const Epi& epi,
-                const char* name) {
+                const char* name, const ColArgs* cols = nullptr) {
     if (p.groups.empty()) return SIML_OK;
     ctx->h_glist = p.groups;
     TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
@@ -2232,7 +2257,7 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
                          ctx->glist.as<int32_t>()};
     const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
     const int32_t* rmag = ctx->ref.mag.as<int32_t>();
-    const ColArgs C = ctx->qry->args();
+    const ColArgs C = cols ? *cols : ctx->qry->args();
     if constexpr (Epi::kBlockMode) {
         if (p.nb == 8 && fast)
             k_bits_block<8, true><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
@@ -2939,7 +2964,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? ctx->target_wg : 2048);
     grid.y = std::min<unsigned>(grid.y, ctx->topk_wave_cap);
-    const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
+    int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
     const size_t cnt_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
@@ -2950,8 +2975,12 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const size_t s1_bytes = (sizeof(uint64_t) * (size_t)ns1 * (size_t)k + 255) & ~size_t(255);
     const size_t s2_bytes = (sizeof(uint64_t) * (size_t)ns2 * (size_t)k + 255) & ~size_t(255);
     const size_t c1_bytes = (sizeof(int32_t) * (size_t)ns1 + 255) & ~size_t(255);
-    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes + sizeof(int32_t) * (size_t)ns2));
+    const size_t c2_bytes = (sizeof(int32_t) * (size_t)ns2 + 255) & ~size_t(255);
+    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes + c2_bytes +
+                             sizeof(uint64_t) * (size_t)nr));
     char* base = ctx->scratch.as<char>();
+    uint64_t* d_seed = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes +
+                                                   c2_bytes);
     uint64_t* d_buf = reinterpret_cast<uint64_t*>(base);
     int32_t* d_cnt = reinterpret_cast<int32_t*>(base + buf_bytes);
     uint64_t* d_sbuf = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes);
@@ -2959,9 +2988,54 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     int32_t* d_scnt = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes);
     int32_t* d_scnt2 = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes);
     const int64_t ng = plan.g1 - plan.g0;
+    // reduce kTopkFan buffers per workgroup until one workgroup per row can
+    // take the rest, then select and sort
+    auto select = [&](int32_t* out_idx, float* out_t) -> int {
+        const uint64_t* src = d_buf;
+        const int32_t* srccnt = d_cnt;
+        int64_t nseg = nw;
+        int segcap = cap;
+        for (int level = 0; nseg > kTopkFan; ++level) {
+            const int64_t ns = (nseg + kTopkFan - 1) / kTopkFan;
+            uint64_t* dst = level % 2 ? d_sbuf2 : d_sbuf;
+            int32_t* dstcnt = level % 2 ? d_scnt2 : d_scnt;
+            k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(
+                src, srccnt, (int)nseg, segcap, k, dst, dstcnt);
+            TRY(launch_check("k_topk_stage"));
+            src = dst;
+            srccnt = dstcnt;
+            nseg = ns;
+            segcap = k;
+        }
+        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
+                                                                         pos_only ? 1 : 0, nq, out_idx, out_t);
+        return launch_check("k_topk_final");
+    };
     TimedRegion tr(ctx, nr * nq);
+    const uint64_t* seed = nullptr;
+    if (bp.use && pos_only && ctx->topk_seed && ctx->qry->ntiles >= 4 * kSeedStride) {
+        // seeding: a top-k over every kSeedStride-th tile gives each row a
+        // key that at least k of its keys reach; the full pass then buffers
+        // only keys >= that seed
+        ColArgs S = ctx->qry->args();
+        S.tstride = kSeedStride;
+        S.ntiles = (ctx->qry->ntiles + kSeedStride - 1) / kSeedStride;
+        const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid.y, (S.ntiles + 3) / 4));
+        const int64_t nws = (int64_t)gys * (kBlock / kWave);
+        TRY(launch_bits(ctx, bp, gys, rowbase, rowlimit,
+                        EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only, nullptr}, "k_bits<topk sample>", &S));
+        const int64_t nw_full = nw;
+        nw = nws;
+        const int rc = select(ctx->res.as<int32_t>(), ctx->res2.as<float>());
+        nw = nw_full;
+        TRY(rc);
+        k_topk_seed<<<dim3((unsigned)((nr + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
+            ctx->res.as<int32_t>(), ctx->res2.as<float>(), nr, k, d_seed);
+        TRY(launch_check("k_topk_seed"));
+        seed = d_seed;
+    }
     if (bp.use) {
-        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only},
+        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only, seed},
                         "k_bits<topk>"));
         if (!mrows.empty()) {
             k_topk<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
@@ -2985,28 +3059,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
             TRY(launch_check("k_topk<rowlist>"));
         }
     }
-    // reduce kTopkFan buffers per workgroup until one workgroup per row can
-    // take the rest
-    const uint64_t* src = d_buf;
-    const int32_t* srccnt = d_cnt;
-    int64_t nseg = nw;
-    int segcap = cap;
-    for (int level = 0; nseg > kTopkFan; ++level) {
-        const int64_t ns = (nseg + kTopkFan - 1) / kTopkFan;
-        uint64_t* dst = level % 2 ? d_sbuf2 : d_sbuf;
-        int32_t* dstcnt = level % 2 ? d_scnt2 : d_scnt;
-        k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg,
-                                                                                        segcap, k, dst, dstcnt);
-        TRY(launch_check("k_topk_stage"));
-        src = dst;
-        srccnt = dstcnt;
-        nseg = ns;
-        segcap = k;
-    }
-    k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
-                                                                     pos_only ? 1 : 0, nq, ctx->res.as<int32_t>(),
-                                                                     ctx->res2.as<float>());
-    TRY(launch_check("k_topk_final"));
+    TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>()));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
     return SIML_OK;
@@ -3083,6 +3136,7 @@ enum TuneOp { kTuneMultirow = 1, kTuneHistRows, kTuneNeighbors, kTuneTopk, kTune
 struct TuneShape {
     int64_t target_wg;
     unsigned topk_wave_cap;
+    bool topk_seed;
 };
 
 std::mutex g_tune_mu;
@@ -3107,12 +3161,14 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
         if (it != g_tune.end()) {
             ctx->target_wg = it->second.target_wg;
             ctx->topk_wave_cap = it->second.topk_wave_cap;
+            ctx->topk_seed = it->second.topk_seed;
             return impl();
         }
     }
-    static const TuneShape kCands[] = {{8192, 256}, {2048, 256}, {4096, 256}, {16384, 256}, {32768, 1024},
-                                       {8192, 64}};
-    const int ncand = op == kTuneTopk ? 6 : 5;
+    static const TuneShape kCands[] = {{8192, 256, false}, {2048, 256, false}, {4096, 256, false},
+                                       {16384, 256, false}, {32768, 1024, false}, {8192, 64, false},
+                                       {8192, 256, true}, {2048, 256, true}, {32768, 1024, true}};
+    const int ncand = op == kTuneTopk ? 9 : 5;
     TRY(set_device(ctx));
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -3126,6 +3182,7 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
     for (int i = 0; i < ncand && rc == SIML_OK; ++i) {
         ctx->target_wg = kCands[i].target_wg;
         ctx->topk_wave_cap = kCands[i].topk_wave_cap;
+        ctx->topk_seed = kCands[i].topk_seed;
         (void)hipEventRecord(a, ctx->stream);
         rc = impl();
         if (rc != SIML_OK) break;
@@ -3149,6 +3206,7 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
     // candidate's) is the same
     ctx->target_wg = best.target_wg;
     ctx->topk_wave_cap = best.topk_wave_cap;
+    ctx->topk_seed = best.topk_seed;
     return SIML_OK;
 }
 
@@ -3188,14 +3246,26 @@ int siml_set_autotune(siml_ctx* ctx, int on) {
     if (!ctx->autotune) {
         ctx->target_wg = 8192;
         ctx->topk_wave_cap = 256;
+        ctx->topk_seed = false;
     }
     return SIML_OK;
 }
 
-int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap) {
-    if (!ctx || !target_wg || !topk_wave_cap) return fail(SIML_EINVAL, "null pointer");
+int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap, int32_t* topk_seed) {
+    if (!ctx || !target_wg || !topk_wave_cap || !topk_seed) return fail(SIML_EINVAL, "null pointer");
     *target_wg = ctx->target_wg;
     *topk_wave_cap = (int32_t)ctx->topk_wave_cap;
+    *topk_seed = ctx->topk_seed ? 1 : 0;
+    return SIML_OK;
+}
+
+int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (target_wg < 1 || topk_wave_cap < 1) return fail(SIML_EINVAL, "launch shape values must be positive");
+    ctx->autotune = false;
+    ctx->target_wg = target_wg;
+    ctx->topk_wave_cap = (unsigned)topk_wave_cap;
+    ctx->topk_seed = topk_seed != 0;
     return SIML_OK;
 }
 

@@ -424,3 +424,19 @@ def test_neighbors_single_and_two_pass(demo_lines, demo_set, lo):
     lists, counts = oracle.neighbors(ref, demo_set, lo)
     assert np.array_equal(np.diff(offs), counts)
     assert np.array_equal(idx, np.concatenate(lists))
+
+
+@pytest.mark.parametrize("k", [1, 50, 256])
+def test_topk_seeded_real_smiles(demo_lines, k):
+    """Real SMILES (dense similarities): top-k with the row thresholds seeded
+    from every 16th query tile equals the oracle."""
+    g = GPULingo(0)
+    g.set_launch_shape(2048, 256, True)
+    ref = oracle.pack(demo_lines[100:220])
+    qry = oracle.pack(demo_lines)
+    _set(g, ref, qry)
+    idx, t = g.getTopK(0, 120, k)
+    oi, ot = oracle.topk(ref, qry, k)
+    assert np.array_equal(idx, oi)
+    assert_same_f32(t, ot)
+    assert g.launch_shape()[2] is True

@@ -172,10 +172,32 @@ def test_autotune_changes_no_result(db_and_queries):
         r.append(g.getMultipleHistogrammedRows(0, 90))
         out.append(r)
         if tune:
-            t, c = g.launch_shape()
+            t, c, _ = g.launch_shape()
             assert t > 0 and c > 0
     for other in out[1:]:
         for a, b in zip(out[0], other):
             assert np.array_equal(np.asarray(a).view(np.int32) if np.asarray(a).dtype == np.float32 else a,
                                   np.asarray(b).view(np.int32) if np.asarray(b).dtype == np.float32 else b)
+    db.close()
+
+
+@pytest.mark.parametrize("shape", [(8192, 256, True), (2048, 64, True), (32768, 1024, False), (1024, 16, True)])
+def test_topk_launch_shapes_and_seeding(db_and_queries, shape):
+    """Top-k is the same for every launch shape, with and without seeding
+    the row thresholds from a column sample, on sparse (random) and dense
+    (near-duplicate) rows."""
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set)
+    D = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+    g = GPULingo(0)
+    g.attach_db(db)
+    g.set_launch_shape(*shape)
+    sub = synth.ragged_slice(*q, 200, 260)
+    g.set_refsmiles_ragged(*sub)
+    Q = synth.ragged_to_padded(*sub)
+    for k in (1, 17, 100):
+        ti, tt = g.getTopK(0, 60, k)
+        oi, ot = oracle.topk(Q, D, k)
+        assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
+    assert g.launch_shape() == (shape[0], shape[1], shape[2])
     db.close()
