@@ -9,7 +9,7 @@ copies are timed.
 Prints ONE JSON line.  Besides the contract fields it carries
   roofline      the dominant kernel's algorithmic integer ops per launch
                 (4 * (La + Lb) per pair) over its mean event-timed duration,
-                against the VALU peak 256 CU * 64 lanes * 2.4 GHz
+                against the VALU peak 256 CU * 128 lanes * 2.4 GHz
   cpu_baseline  the CPU oracle (C restatement of _CPULingo.c, OpenMP over
                 rows) timed on a bounded sample of the same workload
 
@@ -47,7 +47,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-P_VALU = 256 * 64 * 2.4e9          # int32 lane-ops/s
+# MI355X vector peak: 256 CUs x 128 lanes per clock (4 SIMDs, a wave64
+# instruction every 2 cycles; the 157.3 TFLOP/s FP32 vector rate is these
+# lanes doing FMA) x 2.4 GHz = 7.86e13 int32 lane-ops/s.  siml_selftest_valu
+# measures 6.7e13 with independent v_add_u32 chains.
+P_VALU = 256 * 128 * 2.4e9
 B_HBM = 8.0e12                     # bytes/s
 N_ALLPAIRS = 32768
 # HBM bytes per launch of the default all-pairs kernel, from rocprofv3 counter
@@ -56,7 +60,7 @@ N_ALLPAIRS = 32768
 # device copy of known size (profiles/k_bits_f32block_allpairs32768.md).
 TRAFFIC_ALLPAIRS = (2.0 * 1051660 + 1.0 * 4264020) * 1024
 # VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 8.763e8
-# wave-instructions x 4 cycles / (1024 SIMDs x 2.4 GHz x its duration).
+# wave-instructions x 2 cycles / (1024 SIMDs x 2.4 GHz x its duration).
 VALU_INSTS_ALLPAIRS = 8.763e8
 
 
@@ -171,7 +175,7 @@ def workload_allpairs(args):
                      "alg_ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
                      "hbm_frac": bytes_alg / kernel_s / B_HBM,
                      "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
-                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 4 / (1024 * 2.4e9 * kernel_s)) if traffic else None},
+                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 2 / (1024 * 2.4e9 * kernel_s)) if traffic else None},
     })
     if not args.no_cpu:
         import oracle

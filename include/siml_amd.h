@@ -226,6 +226,11 @@ int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_ca
  * [1, 131072); reports the number of differing quotients and the first
  * (smallest denominator) pair, -1 when none. */
 int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t* first_n, int64_t* first_d);
+/* Achievable int32 VALU rate of the device (no counterpart in the
+ * reference): lane-operations per second of independent v_add_u32 (op 0) or
+ * v_min_i32 (op 1) chains on every SIMD, the measured counterpart of the
+ * 256 CU x 128 lanes x 2.4 GHz vector peak. */
+int siml_selftest_valu(int device, int op, double* lane_ops_per_s);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,7 @@ SIGNATURES = {
     "siml_last_kernel": (ctypes.c_char_p, [_vp]),
     "siml_selftest_division": (_int, [_int, _int, _i64p, _i64p, _i64p]),
     "siml_set_autotune": (_int, [_vp, _int]),
+    "siml_selftest_valu": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
     "siml_launch_shape": (_int, [_vp, _i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "siml_set_launch_shape": (_int, [_vp, _i64, ctypes.c_int32, ctypes.c_int32]),
 }

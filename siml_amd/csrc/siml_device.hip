@@ -729,7 +729,7 @@ constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kTopkSlack);
 // k-th largest, and `sorted` receives the min(k, total) largest keys
 // (unordered, rest 0).  Returns their number.
 __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32_t* __restrict__ cnt, int s0,
-                                   int s1, int cap, int k, uint64_t* sorted) {
+                                   int s1, int cap, int k, uint64_t* sorted, uint64_t* kth = nullptr) {
     __shared__ uint64_t cand[kTopkLdsKeys];
     __shared__ uint32_t hist[256];
     __shared__ int32_t s_base[kTopkFan + 1];
@@ -801,6 +801,23 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
         }
         K = s_prefix;
     }
+    if (kth) {
+        // only the k-th largest key is wanted (0 when there are fewer than k)
+        if (total < (uint32_t)k) K = 0ull;
+        if (total == (uint32_t)k) {
+            // the smallest of exactly k keys
+            __shared__ uint64_t s_min;
+            if (threadIdx.x == 0) s_min = ~0ull;
+            __syncthreads();
+            uint64_t mn = ~0ull;
+            for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) mn = cand[i] < mn ? cand[i] : mn;
+            atomicMin((unsigned long long*)&s_min, (unsigned long long)mn);
+            __syncthreads();
+            K = s_min;
+        }
+        *kth = K;
+        return total;
+    }
     sorted[threadIdx.x] = 0ull;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
@@ -833,6 +850,18 @@ __global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restric
 // row has T == 0: the slots after the positive candidates take the columns
 // with T == 0 in ascending order -- the smallest column indices of [0, ncols)
 // that are not among the positive candidates.
+// Seed of each row: its k-th largest candidate key, 0 when it has fewer
+// than k candidates (then every candidate must be kept).
+__global__ __launch_bounds__(kBlock) void k_topk_kth(const uint64_t* __restrict__ buf,
+                                                     const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
+                                                     uint64_t* __restrict__ seed) {
+    __shared__ uint64_t sorted[kBlock];
+    const int64_t r = blockIdx.x;
+    uint64_t K = 0ull;
+    wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, 0, nw, cap, k, sorted, &K);
+    if (threadIdx.x == 0) seed[r] = K;
+}
+
 __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restrict__ buf,
                                                        const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
                                                        int zero_fill, int64_t ncols, int32_t* __restrict__ out_idx,
@@ -898,18 +927,6 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
             ++i;
         }
     }
-}
-
-// Per-row seed threshold from a top-k over a sample of the columns: the key
-// of the row's k-th entry.  The k listed columns are distinct and each one's
-// key over the whole set is at least its listed key (a zero-filled entry
-// lists T = 0), so at least k keys of the full row are >= the seed.
-__global__ void k_topk_seed(const int32_t* __restrict__ idx, const float* __restrict__ t, int64_t nr, int k,
-                            uint64_t* __restrict__ seed) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nr) return;
-    const int32_t c = idx[r * k + k - 1];
-    seed[r] = c >= 0 ? topk_key(t[r * k + k - 1], c) : 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -1230,10 +1247,12 @@ struct EpiNbr {
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
 
-// Top-k candidates per (row, wave), as topk_push with the per-row state in
-// LDS.  Lane r of Reg::thr holds row r's threshold key (0 until armed), so
-// a tile in which no key of a row can enter is rejected with one 64-bit
-// compare and a ballot.
+// Top-k candidates per (row, wave).  The per-row state lives in registers:
+// lane r of Reg::thr holds row r's threshold key (0 until armed) and lane r
+// of Reg::cnt the number of keys in its buffer.  A tile in which no key of a
+// row passes costs one 64-bit compare and a ballot; otherwise the passing
+// keys are appended, after the buffer has been cut to its k largest if the
+// append could overflow it.
 struct EpiTopk {
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
@@ -1244,57 +1263,51 @@ struct EpiTopk {
     bool pos_only;           // only pairs with T > 0 are candidates
     const uint64_t* theta0;  // per row: a key at most the row's k-th largest (0: none), or null
     struct Lds {
-        int32_t cnt[kBlock / kWave][32];
-        int32_t armed[kBlock / kWave][32];
-        uint64_t theta[kBlock / kWave][32];
         uint32_t hist[kBlock / kWave][256];
     };
     struct Reg {
         uint64_t thr;
+        int32_t cnt;
     };
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
-        const int w = threadIdx.x / kWave, lane = lane_id();
-        // a seeded row admits only keys >= its seed: theta = seed - 1
+    __device__ void begin(Lds&, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
+        const int lane = lane_id();
+        // a seeded row admits only keys >= its seed: threshold = seed - 1
         uint64_t th = 0ull;
         if (theta0 && lane < 32 && ri.row[lane] >= 0) {
             const uint64_t s0 = theta0[(int64_t)ri.row[lane] - rb];
             th = s0 ? s0 - 1 : 0ull;
         }
-        if (lane < 32) {
-            L.cnt[w][lane] = 0;
-            L.armed[w][lane] = th ? 1 : 0;
-            L.theta[w][lane] = th;
-        }
         R.thr = th;
-        wave_lds_sync();
+        R.cnt = 0;
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
                           int64_t gw, int64_t nw) const {
-        const int w = threadIdx.x / kWave;
+        const int w = threadIdx.x / kWave, lane = lane_id();
         const uint64_t key = topk_key(t, col);
-        const uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr >> 32), r) << 32) |
-                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)R.thr, r);
+        uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr >> 32), r) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)R.thr, r);
         valid = valid && (!pos_only || t > 0.0f);
-        if (__ballot(valid && key > thr) == 0ull) return;
-        TopkBuf B{buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap, L.cnt[w][r], L.armed[w][r] != 0,
-                  L.theta[w][r]};
-        const int before = B.cnt;
-        topk_push(B, valid, key, k, cap, L.hist[w]);
-        if (B.cnt != before || B.armed != (L.armed[w][r] != 0)) {
-            wave_lds_sync();
-            if (lane_id() == 0) {
-                L.cnt[w][r] = B.cnt;
-                L.armed[w][r] = B.armed ? 1 : 0;
-                L.theta[w][r] = B.theta;
-            }
-            if (lane_id() == r) R.thr = B.armed ? B.theta : 0ull;
-            wave_lds_sync();
+        uint64_t m = __ballot(valid && key > thr);
+        if (m == 0ull) return;
+        int cnt = __builtin_amdgcn_readlane(R.cnt, r);
+        uint64_t* b = buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap;
+        if (cnt + kWave > cap) {
+            // cnt > cap - 64 >= k distinct keys: keep the k largest
+            thr = wave_keep_topk(b, cnt, k, L.hist[w]);
+            cnt = k;
+            m = __ballot(valid && key > thr);
+        }
+        if (valid && key > thr) b[cnt + rank_below(m)] = key;
+        cnt += __popcll(m);
+        if (lane == r) {
+            R.thr = thr;
+            R.cnt = cnt;
         }
     }
-    __device__ void end(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
-        const int w = threadIdx.x / kWave, lane = lane_id();
-        if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = L.cnt[w][lane];
+    __device__ void end(Lds&, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+        const int lane = lane_id();
+        if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = R.cnt;
     }
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
@@ -1567,6 +1580,45 @@ __global__ __launch_bounds__(kBlock) void k_nbr_compact(const int32_t* __restric
     const int64_t slot = i / bcap;
     const int j = (int)(i - slot * bcap);
     if (j < counts[slot]) out[offs[slot] + j] = buf[i];
+}
+
+// Achievable int32 VALU rate: 8 independent v_add_u32 (op 0) or v_min_i32
+// (op 1) chains per lane, `iters` x 16 steps each; the result is written so
+// nothing is dead.  Lane-operations = threads * iters * 16 * 8.
+__global__ __launch_bounds__(kBlock) void k_valu_peak(uint32_t* __restrict__ out, uint32_t seed, int iters, int op) {
+    uint32_t x0 = seed + threadIdx.x, x1 = x0 * 3u, x2 = x0 * 5u, x3 = x0 * 7u, x4 = x0 * 11u, x5 = x0 * 13u,
+             x6 = x0 * 17u, x7 = x0 * 19u;
+    const uint32_t c = seed | 1u;
+    if (op == 0) {
+        for (int i = 0; i < iters; ++i) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x0) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x1) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x2) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x3) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x4) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x5) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x6) : "v"(c));
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x7) : "v"(c));
+            }
+        }
+    } else {
+        for (int i = 0; i < iters; ++i) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x0) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x1) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x2) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x3) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x4) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x5) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x6) : "v"(c));
+                asm volatile("v_min_i32 %0, %0, %1" : "+v"(x7) : "v"(c));
+            }
+        }
+    }
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
 }
 
 // Exhaustive comparison of short_div(method) with IEEE division over
@@ -2990,7 +3042,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const int64_t ng = plan.g1 - plan.g0;
     // reduce kTopkFan buffers per workgroup until one workgroup per row can
     // take the rest, then select and sort
-    auto select = [&](int32_t* out_idx, float* out_t) -> int {
+    auto select = [&](int32_t* out_idx, float* out_t, uint64_t* kth) -> int {
         const uint64_t* src = d_buf;
         const int32_t* srccnt = d_cnt;
         int64_t nseg = nw;
@@ -3007,6 +3059,10 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
             nseg = ns;
             segcap = k;
         }
+        if (kth) {
+            k_topk_kth<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap, kth);
+            return launch_check("k_topk_kth");
+        }
         k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
                                                                          pos_only ? 1 : 0, nq, out_idx, out_t);
         return launch_check("k_topk_final");
@@ -3014,9 +3070,10 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     TimedRegion tr(ctx, nr * nq);
     const uint64_t* seed = nullptr;
     if (bp.use && pos_only && ctx->topk_seed && ctx->qry->ntiles >= 4 * kSeedStride) {
-        // seeding: a top-k over every kSeedStride-th tile gives each row a
-        // key that at least k of its keys reach; the full pass then buffers
-        // only keys >= that seed
+        // seeding: the k-th largest candidate key over every kSeedStride-th
+        // tile is reached by at least k keys of the full row (with pos_only,
+        // a row with fewer than k sampled candidates gets seed 0: no limit);
+        // the full pass then buffers only keys >= that seed
         ColArgs S = ctx->qry->args();
         S.tstride = kSeedStride;
         S.ntiles = (ctx->qry->ntiles + kSeedStride - 1) / kSeedStride;
@@ -3026,12 +3083,9 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
                         EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only, nullptr}, "k_bits<topk sample>", &S));
         const int64_t nw_full = nw;
         nw = nws;
-        const int rc = select(ctx->res.as<int32_t>(), ctx->res2.as<float>());
+        const int rc = select(nullptr, nullptr, d_seed);
         nw = nw_full;
         TRY(rc);
-        k_topk_seed<<<dim3((unsigned)((nr + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
-            ctx->res.as<int32_t>(), ctx->res2.as<float>(), nr, k, d_seed);
-        TRY(launch_check("k_topk_seed"));
         seed = d_seed;
     }
     if (bp.use) {
@@ -3059,7 +3113,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
             TRY(launch_check("k_topk<rowlist>"));
         }
     }
-    TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>()));
+    TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>(), nullptr));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
     return SIML_OK;
@@ -3477,6 +3531,38 @@ int siml_row_hist101_f32(int device, const float* data, int64_t rows, int64_t co
 int siml_row_hist101_f64(int device, const double* data, int64_t rows, int64_t cols, int64_t pitch,
                          uint32_t* hist) {
     return row_hist_common(device, data, sizeof(double), rows, cols, pitch, hist);
+}
+
+int siml_selftest_valu(int device, int op, double* lane_ops_per_s) {
+    if (!lane_ops_per_s) return fail(SIML_EINVAL, "null pointer");
+    if (op != 0 && op != 1) return fail(SIML_EINVAL, "unknown operation %d", op);
+    siml_ctx* ctx = nullptr;
+    TRY(shared_ctx(device, &ctx));
+    TRY(set_device(ctx));
+    const int blocks = 256 * 4 * 8;  // 8 workgroups of 4 waves per CU
+    const int iters = 2048;
+    TRY(ctx->scratch.reserve(sizeof(uint32_t) * (size_t)blocks * kBlock));
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(SIML_EDEVICE, "hipEventCreate failed");
+    }
+    float ms = 0.f;
+    int rc = SIML_OK;
+    for (int rep = 0; rep < 2 && rc == SIML_OK; ++rep) {  // the first launch warms up
+        (void)hipEventRecord(a, ctx->stream);
+        k_valu_peak<<<dim3(blocks), dim3(kBlock), 0, ctx->stream>>>(ctx->scratch.as<uint32_t>(), 12345u, iters, op);
+        rc = launch_check("k_valu_peak");
+        (void)hipEventRecord(b, ctx->stream);
+        if (rc == SIML_OK && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = fail(SIML_EDEVICE, "timing k_valu_peak failed");
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    TRY(rc);
+    *lane_ops_per_s = (double)blocks * kBlock * iters * 16.0 * 8.0 / (ms * 1e-3);
+    return SIML_OK;
 }
 
 int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t* first_n, int64_t* first_d) {

@@ -440,3 +440,16 @@ def test_topk_seeded_real_smiles(demo_lines, k):
     assert np.array_equal(idx, oi)
     assert_same_f32(t, ot)
     assert g.launch_shape()[2] is True
+
+
+def test_valu_peak_microbenchmark():
+    """The measured int32 VALU rate of the device (independent add and min
+    chains) is positive and not above the vector peak: 256 CUs x 128 lanes
+    per clock (4 SIMDs, a wave64 instruction every 2 cycles) x 2.4 GHz."""
+    import ctypes
+    lib = _lib.load()
+    for op in (0, 1):
+        rate = ctypes.c_double()
+        _lib.check(lib.siml_selftest_valu(0, op, ctypes.byref(rate)))
+        print(f"op {op}: {rate.value:.4g} lane-ops/s", flush=True)
+        assert 0 < rate.value <= 256 * 128 * 2.4e9 * 1.02
