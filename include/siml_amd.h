@@ -184,6 +184,11 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
  * since the last reset. */
 int siml_set_timing(siml_ctx* ctx, int on);
 int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_t* pairs, int reset);
+/* Work the neighbour kernels actually did while timing was on, counted on
+ * the device: the pairs they evaluated (those the magnitude bound did not
+ * prune) and the sum of La + Lb (unique lingo counts) over those pairs, the
+ * merge work the pairs stand for.  Values since the last reset. */
+int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset);
 /* Kernel policy: AUTO serves sorted-unique sets with the bit-sliced
  * multi-row kernels (rows that do not qualify, and unsorted sets, use the
  * merge kernel); HASH8 uses the 8-row hashed kernels instead of the

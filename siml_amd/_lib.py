@@ -79,6 +79,7 @@ SIGNATURES = {
     "siml_set_layout": (_int, [_vp, _int]),
     "siml_kernel_stats": (_int, [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _i64p, _int]),
     "siml_last_kernel": (ctypes.c_char_p, [_vp]),
+    "siml_work_stats": (_int, [_vp, _i64p, _i64p, _int]),
     "siml_selftest_division": (_int, [_int, _int, _i64p, _i64p, _i64p]),
     "siml_set_autotune": (_int, [_vp, _int]),
     "siml_selftest_valu": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
@@ -186,6 +187,13 @@ class Handle:
         ms, n, pairs = ctypes.c_double(0), _i64(0), _i64(0)
         self.call("siml_kernel_stats", ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs), int(reset))
         return ms.value, n.value, pairs.value
+
+    def work_stats(self, reset=False):
+        """(pairs evaluated, sum of La + Lb over them) by the neighbour
+        kernels while timing was on, since the last reset."""
+        pairs, lsum = _i64(0), _i64(0)
+        self.call("siml_work_stats", ctypes.byref(pairs), ctypes.byref(lsum), int(reset))
+        return pairs.value, lsum.value
 
     def last_kernel(self):
         return self._lib.siml_last_kernel(self.h).decode()

@@ -177,6 +177,33 @@ __device__ __forceinline__ int rank_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Sum of v over the wave (every lane gets it).
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Evaluated-work counter: one pair of global atomics per wave.
+__device__ __forceinline__ void add_work(unsigned long long* work, unsigned long long pairs, unsigned long long lsum) {
+    pairs = wave_sum_u64(pairs);
+    lsum = wave_sum_u64(lsum);
+    if (lane_id() == 0 && pairs) {
+        atomicAdd(work, pairs);
+        atomicAdd(work + 1, lsum);
+    }
+}
+
 // Bin of doHistogram / gpuHistogrammedLingo (kernels.py:657,
 // _cpuutil.cpp:65-66); out-of-range products take the documented end bins.
 __device__ __forceinline__ int bin101_f32(float t) {
@@ -405,7 +432,9 @@ __global__ __launch_bounds__(kBlock) void k_hash_neighbors(GroupArgs G, int64_t 
                                                            ColArgs C, int64_t rb, int64_t rl, float lo, float hi,
                                                            int32_t* __restrict__ counts,
                                                            const int64_t* __restrict__ offs,
-                                                           int32_t* __restrict__ idx) {
+                                                           int32_t* __restrict__ idx,
+                                                           const int32_t* __restrict__ rlen,
+                                                           unsigned long long* __restrict__ work) {
     __shared__ GroupLds T;
     const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
     stage_group(T, G, hd);
@@ -415,19 +444,29 @@ __global__ __launch_bounds__(kBlock) void k_hash_neighbors(GroupArgs G, int64_t 
     int ma[siml::kGroupRows];
     bool live[siml::kGroupRows];
     int64_t pos[siml::kGroupRows];
+    unsigned nlive = 0, sum_la = 0;
 #pragma unroll
     for (int r = 0; r < siml::kGroupRows; ++r) {
         const int64_t row = (int64_t)hd.row0 + r;
         live[r] = r < hd.nrows && row >= rb && row < rl;
         ma[r] = live[r] ? rmag[row] : 0;
         pos[r] = (FILL && live[r]) ? offs[(row - rb) * nw + gw] : 0;
+        if (work && live[r]) {
+            nlive += 1;
+            sum_la += (unsigned)rlen[row];
+        }
     }
+    unsigned long long wpairs = 0, wlsum = 0;  // evaluated-work counter
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t c = t * kWave + lane;
         uint32_t acc[4];
         group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
         const bool valid = c < C.n;
         const int mb = valid ? C.mag[c] : 0;
+        if (work && valid) {
+            wpairs += nlive;
+            wlsum += (unsigned long long)nlive * (unsigned)C.len[c] + sum_la;
+        }
 #pragma unroll
         for (int r = 0; r < siml::kGroupRows; ++r) {
             if (!live[r]) continue;
@@ -449,6 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_hash_neighbors(GroupArgs G, int64_t 
         for (int r = 0; r < siml::kGroupRows; ++r)
             if (live[r]) counts[((int64_t)hd.row0 + r - rb) * nw + gw] = (int32_t)pos[r];
     }
+    if (work) add_work(work, wpairs, wlsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,10 +1239,18 @@ struct EpiNbr {
     // is needed
     int32_t* buf;
     int bcap;
+    // evaluated-work counter (null: off): [0] += pairs of the tiles that the
+    // magnitude band did not prune, [1] += their sum of La + Lb (unique
+    // lingo counts, the merge work the pairs stand for); rlen gives La
+    unsigned long long* work;
+    const int32_t* rlen;
     struct Lds {
         int64_t pos[kBlock / kWave][32];
     };
-    struct Reg {};
+    struct Reg {
+        unsigned long long pairs = 0, lsum = 0;
+        int nrows = 0, sum_la = 0;
+    };
     // True when no pair of the group's rows (magnitudes [amin, amax]) with
     // the tile's columns (magnitudes tm) can pass the bound prefilter:
     // fl(min/max) is monotone in both magnitudes, so the closest pair of
@@ -1214,10 +1262,22 @@ struct EpiNbr {
         return false;
     }
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void begin(Lds& L, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (lane < 32) L.pos[w][lane] = (FILL && ri.row[lane] >= 0) ? offs[((int64_t)ri.row[lane] - rb) * nw + gw] : 0;
+        if (work) {
+            const bool live = lane < siml::kBitRows && ri.row[lane < siml::kBitRows ? lane : 0] >= 0;
+            R.nrows = __popcll(__ballot(live));
+            R.sum_la = wave_sum_i32(live ? rlen[ri.row[lane]] : 0);
+        }
         wave_lds_sync();
+    }
+    // one tile survived the magnitude band: lane c's column meets every row
+    __device__ void count_tile(Reg& R, const ColArgs& C, bool valid, int64_t c) const {
+        if (work && valid) {
+            R.pairs += (unsigned)R.nrows;
+            R.lsum += (unsigned long long)R.nrows * (unsigned)C.len[c] + (unsigned)R.sum_la;
+        }
     }
     __device__ void apply(Lds& L, Reg&, int r, int32_t row, bool valid, int64_t col, int ma, int mb, float t,
                           int64_t gw, int64_t nw) const {
@@ -1240,9 +1300,10 @@ struct EpiNbr {
         if (lane_id() == 0) L.pos[w][r] = pos + __popcll(m);
         wave_lds_sync();
     }
-    __device__ void end(Lds& L, Reg&, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void end(Lds& L, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (!FILL && lane < 32 && ri.row[lane] >= 0) counts[((int64_t)ri.row[lane] - rb) * nw + gw] = (int32_t)L.pos[w][lane];
+        if (work) add_work(work, R.pairs, R.lsum);
     }
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
@@ -1454,6 +1515,7 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         transpose8(P);
         if constexpr (NB == 16) transpose8(P + 8);
         const bool valid = c < C.n;
+        if constexpr (Epi::kPrune) epi.count_tile(R, C, valid, c);
         const int mb = valid ? C.mag[c] : 0;
         const float mbf = (float)mb;
         const int64_t col = valid ? col_of(C, c) : 0;
@@ -1507,7 +1569,7 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
                                                       int64_t rowbase, float lo, float hi,
                                                       int32_t* __restrict__ counts, const int64_t* __restrict__ offs,
-                                                      int32_t* __restrict__ idx) {
+                                                      int32_t* __restrict__ idx, unsigned long long* __restrict__ work) {
     extern __shared__ int2 lds_row[];
     const int64_t row = rowlist ? (int64_t)rowlist[blockIdx.x] : rowbase + blockIdx.x;
     const int La = R.len[row];
@@ -1519,6 +1581,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, cons
     const int lane = lane_id();
     const int64_t slot = (row - rowbase) * nw + gw;
     int64_t pos = FILL ? offs[slot] : 0;
+    unsigned long long wpairs = 0, wlsum = 0;  // merges run (evaluated-work counter)
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t c = t * kWave + lane;
         bool nb = false;
@@ -1526,6 +1589,8 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, cons
             const int mb = C.mag[c];
             const float bound = (float)min(ma, mb) / (float)max(ma, mb);
             if (bound >= lo) {
+                wpairs += 1;
+                wlsum += (unsigned)(La + C.len[c]);
                 const int isct = merge_isct(lds_row, La, C.tiles + C.toff[t] + lane, C.len[c]);
                 const float tv = tanimoto_mags(isct, ma, mb);
                 nb = (tv >= lo) && (tv < hi);
@@ -1536,6 +1601,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(RowArgs R, ColArgs C, cons
         pos += __popcll(m);
     }
     if (!FILL && lane == 0) counts[slot] = (int32_t)pos;
+    if (work) add_work(work, wpairs, wlsum);
 }
 
 // rowHistogram101 of a float matrix (_cpuutil.cpp:52-72): one workgroup per
@@ -1892,6 +1958,10 @@ struct siml_ctx {
     std::vector<hipEvent_t> ev_pool;
     double kernel_ms = 0.0;
     int64_t launches = 0, pairs = 0, pending_pairs = 0;
+    // evaluated-work counter of the neighbour kernels (while timing is on):
+    // device [pairs, sum of La + Lb], and the values at the last reset
+    DevBuf work;
+    unsigned long long work_base[2] = {0, 0};
     std::string last_kernel = "none";
     int policy = SIML_POLICY_AUTO;
     int layout = SIML_LAYOUT_BLOCK;
@@ -2462,6 +2532,7 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->sort_tmp.release();
     ctx->glist.release();
     ctx->csr_dev.release();
+    ctx->work.release();
     for (auto& pr : ctx->pending_ev) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
@@ -2871,57 +2942,49 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     int32_t* d_nbuf = nullptr;
     const dim3 hgrid((unsigned)std::max<int64_t>(ng, 1), grid.y);
     const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y);
+    unsigned long long* work = ctx->timing ? ctx->work.as<unsigned long long>() : nullptr;
+    const int32_t* rlen = ctx->ref.args().len;
     auto launch = [&](bool fill, const int64_t* offs, int32_t* idx) -> int {
+        int32_t* counts = fill ? nullptr : d_counts_of(ctx);
         if (bp.use) {
             if (fill)
                 TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
-                                EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase, nullptr, 0}, "k_bits<nbr fill>"));
+                                EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase, nullptr, 0, work, rlen},
+                                "k_bits<nbr fill>"));
             else
                 TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
-                                EpiNbr<false>{lo, hi, d_counts_of(ctx), nullptr, nullptr, rowbase, d_nbuf, kNbrCap},
+                                EpiNbr<false>{lo, hi, counts, nullptr, nullptr, rowbase, d_nbuf, kNbrCap, work, rlen},
                                 "k_bits<nbr count>"));
-            if (!mrows.empty()) {
-                if (fill)
-                    k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
-                        ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, nullptr, offs, idx);
-                else
-                    k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
-                        ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, d_counts_of(ctx),
-                        nullptr, nullptr);
-                TRY(launch_check("k_neighbors<rowlist>"));
-            }
-            return SIML_OK;
-        }
-        if (plan.all_merge) {
+        } else if (plan.all_merge) {
             if (fill)
                 k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
-                                                                            rowbase, lo, hi, nullptr, offs, idx);
+                                                                            rowbase, lo, hi, nullptr, offs, idx, work);
             else
                 k_neighbors<false><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
-                                                                             rowbase, lo, hi, d_counts_of(ctx), nullptr,
-                                                                             nullptr);
+                                                                             rowbase, lo, hi, counts, nullptr, nullptr,
+                                                                             work);
             return launch_check("k_neighbors");
-        }
-        if (ng > 0) {
+        } else if (ng > 0) {
             if (fill)
                 k_hash_neighbors<true><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
                     ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
-                    nullptr, offs, idx);
+                    nullptr, offs, idx, rlen, work);
             else
                 k_hash_neighbors<false><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
                     ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
-                    d_counts_of(ctx), nullptr, nullptr);
+                    counts, nullptr, nullptr, rlen, work);
             TRY(launch_check("k_hash_neighbors"));
         }
-        if (!plan.merge.empty()) {
+        // rows the multi-row kernels do not serve: merge kernel over a row list
+        if (!mrows.empty()) {
             if (fill)
-                k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(),
-                                                                             ctx->rowlist.as<int32_t>(), rowbase, lo,
-                                                                             hi, nullptr, offs, idx);
+                k_neighbors<true><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
+                    ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, nullptr, offs, idx,
+                    work);
             else
-                k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(),
-                                                                              ctx->rowlist.as<int32_t>(), rowbase, lo,
-                                                                              hi, d_counts_of(ctx), nullptr, nullptr);
+                k_neighbors<false><<<mgrid, dim3(kBlock), lds, ctx->stream>>>(
+                    ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, lo, hi, counts, nullptr,
+                    nullptr, work);
             TRY(launch_check("k_neighbors<rowlist>"));
         }
         return SIML_OK;
@@ -3449,7 +3512,30 @@ int siml_set_layout(siml_ctx* ctx, int layout) {
 
 int siml_set_timing(siml_ctx* ctx, int on) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (on && !ctx->work.p) {
+        TRY(set_device(ctx));
+        TRY(ctx->work.reserve(2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(ctx->work.p, 0, 2 * sizeof(unsigned long long)));
+        ctx->work_base[0] = ctx->work_base[1] = 0;
+    }
     ctx->timing = on != 0;
+    return SIML_OK;
+}
+
+int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    unsigned long long v[2] = {0, 0};
+    if (ctx->work.p) {
+        TRY(set_device(ctx));
+        HIP_TRY(hipMemcpyAsync(v, ctx->work.p, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (pairs) *pairs = (int64_t)(v[0] - ctx->work_base[0]);
+    if (merge_len) *merge_len = (int64_t)(v[1] - ctx->work_base[1]);
+    if (reset) {
+        ctx->work_base[0] = v[0];
+        ctx->work_base[1] = v[1];
+    }
     return SIML_OK;
 }
 

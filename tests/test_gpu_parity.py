@@ -453,3 +453,41 @@ def test_valu_peak_microbenchmark():
         _lib.check(lib.siml_selftest_valu(0, op, ctypes.byref(rate)))
         print(f"op {op}: {rate.value:.4g} lane-ops/s", flush=True)
         assert 0 < rate.value <= 256 * 128 * 2.4e9 * 1.02
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_evaluated_work_counter(demo_lines, demo_set, policy):
+    """The device counter of evaluated neighbour pairs.  With lo = 0 nothing
+    is pruned: every pair is evaluated once per pass, and the merge work is
+    exactly nq * sum(La) + nr * sum(Lb) per pass.  With lo = 0.7 the pairs
+    whose magnitude bound fl(min/max) reaches lo must all be evaluated
+    (pruning is lossless); the merge kernel evaluates exactly those."""
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[500:700])
+    _set(g, ref, demo_set)
+    h = g._h
+    h.call("siml_set_timing", 1)
+    nr, nq = ref[0].shape[0], demo_set[0].shape[0]
+    la = ref[2].ravel().astype(np.int64)
+    lb = demo_set[2].ravel().astype(np.int64)
+    h.work_stats(reset=True)
+    g.getNeighborsCSR(0, nr, 0.0)
+    pairs, lsum = h.work_stats(reset=True)
+    passes = pairs // (nr * nq)
+    assert passes in (1, 2) and pairs == passes * nr * nq, (pairs, nr * nq)
+    assert lsum == passes * (nq * int(la.sum()) + nr * int(lb.sum()))
+    ma = ref[3].ravel().astype(np.float32)[:, None]
+    mb = demo_set[3].ravel().astype(np.float32)[None, :]
+    bound = np.minimum(ma, mb) / np.maximum(ma, mb)
+    need = int(np.count_nonzero(bound >= np.float32(0.7)))
+    offs, idx = g.getNeighborsCSR(0, nr, 0.7)
+    pairs, lsum = h.work_stats(reset=True)
+    lists, counts = oracle.neighbors(ref, demo_set, 0.7)
+    assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+    assert need <= pairs <= 2 * nr * nq
+    if policy == _lib.POLICY_MERGE:
+        assert pairs == 2 * need
+    h.call("siml_set_timing", 0)
+    g.getNeighborsCSR(0, nr, 0.7)
+    assert h.work_stats(reset=True) == (0, 0)
