@@ -1121,7 +1121,7 @@ struct EpiF32 {
     __device__ void init(Lds&) const {}
     __device__ void begin(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
     __device__ void apply(Lds&, Reg&, int, int32_t row, bool valid, int64_t col, int, int, float t, int64_t,
-                          int64_t) const {
+                          int64_t, int) const {
         if (valid) out[(row - rb) * opitch + col] = t;
     }
     __device__ void end(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
@@ -1167,7 +1167,7 @@ struct EpiHist101 {
         for (int r = 0; r < siml::kBitRows; ++r) R.zero[r] = 0u;
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t, bool valid, int64_t, int, int, float t, int64_t,
-                          int64_t) const {
+                          int64_t, int) const {
         const bool z = !(t * 100.0f >= 1.0f);
         R.zero[r] += (valid && z) ? 1u : 0u;
         const bool nz = valid && !z;
@@ -1192,29 +1192,49 @@ struct EpiHist1000 {
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     unsigned long long* hist;
+    // Pairs with I = 1 or 2 and U = ma + mb - I below kIU (most non-zero
+    // pairs of unrelated molecules) are counted by (I, U), T being a
+    // function of those two: the counters spread over many more LDS
+    // addresses than the few bins such T fall in, so fewer atomics of one
+    // instruction collide.  finish() adds them to their bins.
+    static constexpr int kIU = 512;
     struct Lds {
         uint32_t bins[1000];
         float edges[1001];
+        uint32_t iu[2 * kIU];
     };
     struct Reg {
         uint32_t zero;
     };
     __device__ void init(Lds& L) const {
         for (int k = threadIdx.x; k < 1000; k += blockDim.x) L.bins[k] = 0u;
+        for (int k = threadIdx.x; k < 2 * kIU; k += blockDim.x) L.iu[k] = 0u;
         stage_edges(L.edges);
     }
     __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const { R.zero = 0u; }
-    __device__ void apply(Lds& L, Reg& R, int, int32_t, bool valid, int64_t, int, int, float t, int64_t,
-                          int64_t) const {
+    __device__ void apply(Lds& L, Reg& R, int, int32_t, bool valid, int64_t, int ma, int mb, float t, int64_t,
+                          int64_t, int isct) const {
         const bool z = !(t >= 1.0f / 1000.0f);  // edge 1 of stage_edges, fl32(1/1000)
         R.zero += (valid && z) ? 1u : 0u;
         const bool nz = valid && !z;
-        if (__ballot(nz) != 0ull && nz) atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
+        if (__ballot(nz) == 0ull || !nz) return;
+        // t > 0 here, so isct >= 1; when u < kIU, ma + mb is small and t is
+        // exactly fl32(isct / u) (u = 0 gives +inf, as (float)isct / 0.0f)
+        const int64_t u = (int64_t)ma + mb - isct;
+        if (isct <= 2 && u >= 0 && u < kIU)
+            atomicAdd(&L.iu[(isct - 1) * kIU + u], 1u);
+        else
+            atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
     }
     __device__ void end(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
         if (R.zero) atomicAdd(&L.bins[0], R.zero);
     }
     __device__ void finish(Lds& L, const BitRowInfo&) const {
+        // the float expression of pair_tanimoto: (float)I / (float)U, both
+        // exact integers (short_div equals IEEE division there)
+        for (int k = threadIdx.x; k < 2 * kIU; k += blockDim.x)
+            if (L.iu[k]) atomicAdd(&L.bins[bin1000((float)(k / kIU + 1) / (float)(k % kIU), L.edges)], L.iu[k]);
+        __syncthreads();
         for (int k = threadIdx.x; k < 1000; k += blockDim.x)
             if (L.bins[k]) atomicAdd(&hist[k], (unsigned long long)L.bins[k]);
     }
@@ -1280,7 +1300,7 @@ struct EpiNbr {
         }
     }
     __device__ void apply(Lds& L, Reg&, int r, int32_t row, bool valid, int64_t col, int ma, int mb, float t,
-                          int64_t gw, int64_t nw) const {
+                          int64_t gw, int64_t nw, int) const {
         bool nb = valid && t >= lo && t < hi;
         if (__ballot(nb) == 0ull) return;
         if (nb) {
@@ -1343,7 +1363,7 @@ struct EpiTopk {
         R.cnt = 0;
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
-                          int64_t gw, int64_t nw) const {
+                          int64_t gw, int64_t nw, int) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         const uint64_t key = topk_key(t, col);
         uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr >> 32), r) << 32) |
@@ -1524,8 +1544,9 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
             const int32_t row = ri.row[r];
             if (row < 0) continue;
             const int ma = ri.ma[r];
-            const float tv = pair_tanimoto<FAST>(count_of<NB>(P, r), ri.maf[r], mbf, ma == 0 || mb == 0);
-            epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw);
+            const float fi = count_of<NB>(P, r);
+            const float tv = pair_tanimoto<FAST>(fi, ri.maf[r], mbf, ma == 0 || mb == 0);
+            epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw, (int)fi);
         }
     }
     epi.end(el, R, ri, gw, nw);

@@ -344,6 +344,25 @@ def test_global_histogram(demo_lines, demo_set, policy):
 
 
 @pytest.mark.parametrize("policy", POLICIES)
+def test_global_histogram_caller_magnitudes(demo_lines, policy):
+    """Magnitudes given by the caller below the count sums (the CPULingo
+    default of lengths, CPULingo.py:82): unions of 0 or less give T = +inf
+    or negative values; the bins still match the oracle, including the
+    pairs the kernels count by (I, U)."""
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", policy)
+    ref = oracle.pack(demo_lines[:200] + [b"CCCCCCCC\n", b"CCCCC\n", b"CCCCCCCCCCCC\n"])
+    qry = oracle.pack(demo_lines[1000:2500] + [b"CCCCCC\n", b"CCCCCCCCCC\n"])
+    ref = ref[:3] + (np.ascontiguousarray(ref[2].ravel().astype(np.int32)),)
+    qry = qry[:3] + (np.ascontiguousarray(qry[2].ravel().astype(np.int32)),)
+    _set(g, ref, qry)
+    n = ref[0].shape[0]
+    h = g.getGlobalHistogram(0, n)
+    assert h.sum() == n * qry[0].shape[0]
+    assert np.array_equal(h, oracle.hist1000(ref, qry))
+
+
+@pytest.mark.parametrize("policy", POLICIES)
 def test_hist_and_neighbors_both_policies(demo_lines, demo_set, policy):
     g = GPULingo(0)
     g._h.call("siml_set_policy", policy)
