@@ -1192,23 +1192,24 @@ struct EpiHist1000 {
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     unsigned long long* hist;
-    // Pairs with I = 1 or 2 and U = ma + mb - I below kIU (most non-zero
-    // pairs of unrelated molecules) are counted by (I, U), T being a
+    // Pairs with I in [1, kIMax] and U = ma + mb - I below kIU (most
+    // non-zero pairs of unrelated molecules) are counted by (I, U), T being a
     // function of those two: the counters spread over many more LDS
     // addresses than the few bins such T fall in, so fewer atomics of one
     // instruction collide.  finish() adds them to their bins.
     static constexpr int kIU = 512;
+    static constexpr int kIMax = 4;
     struct Lds {
         uint32_t bins[1000];
         float edges[1001];
-        uint32_t iu[2 * kIU];
+        uint32_t iu[kIMax * kIU];
     };
     struct Reg {
         uint32_t zero;
     };
     __device__ void init(Lds& L) const {
         for (int k = threadIdx.x; k < 1000; k += blockDim.x) L.bins[k] = 0u;
-        for (int k = threadIdx.x; k < 2 * kIU; k += blockDim.x) L.iu[k] = 0u;
+        for (int k = threadIdx.x; k < kIMax * kIU; k += blockDim.x) L.iu[k] = 0u;
         stage_edges(L.edges);
     }
     __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const { R.zero = 0u; }
@@ -1221,7 +1222,7 @@ struct EpiHist1000 {
         // t > 0 here, so isct >= 1; when u < kIU, ma + mb is small and t is
         // exactly fl32(isct / u) (u = 0 gives +inf, as (float)isct / 0.0f)
         const int64_t u = (int64_t)ma + mb - isct;
-        if (isct <= 2 && u >= 0 && u < kIU)
+        if (isct <= kIMax && u >= 0 && u < kIU)
             atomicAdd(&L.iu[(isct - 1) * kIU + u], 1u);
         else
             atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
@@ -1232,7 +1233,7 @@ struct EpiHist1000 {
     __device__ void finish(Lds& L, const BitRowInfo&) const {
         // the float expression of pair_tanimoto: (float)I / (float)U, both
         // exact integers (short_div equals IEEE division there)
-        for (int k = threadIdx.x; k < 2 * kIU; k += blockDim.x)
+        for (int k = threadIdx.x; k < kIMax * kIU; k += blockDim.x)
             if (L.iu[k]) atomicAdd(&L.bins[bin1000((float)(k / kIU + 1) / (float)(k % kIU), L.edges)], L.iu[k]);
         __syncthreads();
         for (int k = threadIdx.x; k < 1000; k += blockDim.x)
