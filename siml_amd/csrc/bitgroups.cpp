@@ -2,6 +2,7 @@
 #include "bitgroups.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <thread>
 
 namespace siml {
@@ -22,23 +23,6 @@ bool row_levels(const BitGroupInput& in, int64_t r, std::vector<uint32_t>& out) 
     }
     std::sort(out.begin(), out.end());
     return true;
-}
-
-int64_t union_count(const std::vector<uint32_t>& u, const std::vector<uint32_t>& v) {
-    size_t i = 0, j = 0;
-    int64_t n = 0;
-    while (i < u.size() && j < v.size()) {
-        if (u[i] == v[j]) {
-            ++i;
-            ++j;
-        } else if (u[i] < v[j]) {
-            ++i;
-        } else {
-            ++j;
-        }
-        ++n;
-    }
-    return n + (int64_t)(u.size() - i) + (int64_t)(v.size() - j);
 }
 
 // Two-choice cuckoo placement of `keys` with the fixed slot functions;
@@ -71,6 +55,25 @@ struct Built {
     BitGroupHdr hdr;
     std::vector<BitEntry> ents;
 };
+
+// Header and slot-ordered entries of a placed group.
+void emit_group(const std::vector<int32_t>& rows, const std::vector<uint32_t>& keys,
+                const std::vector<uint32_t>& masks, const std::vector<int32_t>& tab, int32_t maxsum,
+                std::vector<Built>& out) {
+    Built b;
+    b.rows = rows;
+    b.hdr.nrows = (int32_t)rows.size();
+    b.hdr.nkeys = (int32_t)keys.size();
+    b.hdr.nbits = maxsum <= 255 ? 8 : 16;
+    b.hdr.pad0 = 0;
+    b.hdr.entry_off = 0;
+    b.ents.reserve(keys.size());
+    for (int s = 0; s < kBitSlots; ++s) {
+        const int k = tab[(size_t)s];
+        if (k >= 0) b.ents.push_back(BitEntry{keys[(size_t)k], masks[(size_t)k], s, 0});
+    }
+    out.push_back(std::move(b));
+}
 
 // Builds the table of `rows`; on a placement failure the rows are split in
 // two halves, each built again.  A single row that cannot be placed goes to
@@ -106,18 +109,35 @@ void build_group(const BitGroupInput& in, const std::vector<int32_t>& rows, std:
         build_group(in, std::vector<int32_t>(rows.begin() + (long)h, rows.end()), out, merge);
         return;
     }
-    Built b;
-    b.rows = rows;
-    b.hdr.nrows = (int32_t)rows.size();
-    b.hdr.nkeys = (int32_t)keys.size();
-    b.hdr.nbits = maxsum <= 255 ? 8 : 16;
-    b.hdr.pad0 = 0;
-    b.hdr.entry_off = 0;
-    for (int s = 0; s < kBitSlots; ++s) {
-        const int k = tab[(size_t)s];
-        if (k >= 0) b.ents.push_back(BitEntry{keys[(size_t)k], masks[(size_t)k], s, 0});
+    emit_group(rows, keys, masks, tab, maxsum, out);
+}
+
+// build_group for rows whose level lists lvs[i] and their sorted union u
+// the caller already holds (the greedy grouping computes both): the keys
+// are placed first, then each level finds its key's index through the
+// placed table (slot h1, else h2) and sets its row's mask bit.
+void build_group_known(const BitGroupInput& in, const std::vector<int32_t>& rows,
+                       const std::vector<std::vector<uint32_t>>& lvs, const std::vector<uint32_t>& u,
+                       std::vector<int32_t>& tab, std::vector<uint32_t>& masks, std::vector<Built>& out,
+                       std::vector<int32_t>& merge) {
+    if (!place(u, tab)) {
+        build_group(in, rows, out, merge);  // splits the rows
+        return;
     }
-    out.push_back(std::move(b));
+    masks.assign(u.size(), 0u);
+    int32_t maxsum = 0;
+    for (size_t i = 0; i < rows.size(); ++i) {
+        const std::vector<uint32_t>& lv = lvs[i];
+        for (uint32_t k : lv) {
+            uint32_t h1, h2;
+            key_slots(k, h1, h2);
+            int32_t j = tab[h1];
+            if (j < 0 || u[(size_t)j] != k) j = tab[h2];
+            masks[(size_t)j] |= 1u << i;
+        }
+        maxsum = std::max(maxsum, (int32_t)lv.size());
+    }
+    emit_group(rows, u, masks, tab, maxsum, out);
 }
 
 }  // namespace
@@ -125,8 +145,17 @@ void build_group(const BitGroupInput& in, const std::vector<int32_t>& rows, std:
 void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out, int nthreads) {
     out = BitGroupTables();
     out.row_group.assign((size_t)in.n, -1);
-    unsigned hw = std::thread::hardware_concurrency();
-    const int nt = nthreads > 0 ? nthreads : (int)std::min<unsigned>(hw ? hw : 1, 32);
+    // default thread count: OMP_NUM_THREADS when set (the process's CPU
+    // share), else the hardware threads, at most 32
+    int nt = nthreads;
+    if (nt <= 0) {
+        const char* env = std::getenv("OMP_NUM_THREADS");
+        nt = env ? std::atoi(env) : 0;
+    }
+    if (nt <= 0) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        nt = (int)std::min<unsigned>(hw ? hw : 1, 32);
+    }
     auto run = [&](int64_t n, auto&& fn) {
         if (n < 256 || nt <= 1) {
             fn(0, n, 0);
@@ -156,44 +185,85 @@ void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out,
     }
     if (by_mag)
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return in.mag[a] < in.mag[b]; });
-    // 2. greedy grouping of consecutive rows, independently per chunk of
-    //    rows, then 3. tables
+    // 2. greedy grouping of consecutive rows, each group's table built as
+    //    soon as it closes.  Chunks of at least kChunkRows rows are grouped
+    //    independently (in parallel); each chunk ends with a partial group,
+    //    so chunks are kept long enough for that to cost at most a group in
+    //    64.
     const int64_t m = (int64_t)order.size();
-    const int nchunks = m < 256 ? 1 : nt;
+    constexpr int64_t kChunkRows = 64 * kBitRows;
+    const int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(nt, m / kChunkRows));
     std::vector<std::vector<Built>> built((size_t)nchunks);
     std::vector<std::vector<int32_t>> merged((size_t)nchunks);
-    run(m, [&](int64_t a, int64_t b, int t) {
+    auto greedy = [&](int64_t a, int64_t b, int t) {
         std::vector<uint32_t> u, tmp, lv;
-        std::vector<int32_t> cur;
+        std::vector<std::vector<uint32_t>> lvs((size_t)kBitRows);
+        std::vector<int32_t> cur, tab;
+        std::vector<uint32_t> masks;
         for (int64_t i = a; i < b; ++i) {
             const int32_t r = order[(size_t)i];
             row_levels(in, r, lv);
-            if (!cur.empty() && ((int)cur.size() == kBitRows || union_count(u, lv) > kBitMaxKeys)) {
-                build_group(in, cur, built[(size_t)t], merged[(size_t)t]);
-                cur.clear();
-                u.clear();
+            bool close = (int)cur.size() == kBitRows;
+            if (!close && !cur.empty()) {
+                tmp.resize(u.size() + lv.size());
+                tmp.resize((size_t)(std::set_union(u.begin(), u.end(), lv.begin(), lv.end(), tmp.begin()) -
+                                    tmp.begin()));
+                close = (int64_t)tmp.size() > kBitMaxKeys;
             }
-            tmp.resize(u.size() + lv.size());
-            tmp.resize((size_t)(std::set_union(u.begin(), u.end(), lv.begin(), lv.end(), tmp.begin()) - tmp.begin()));
-            u.swap(tmp);
+            if (close) {
+                build_group_known(in, cur, lvs, u, tab, masks, built[(size_t)t], merged[(size_t)t]);
+                cur.clear();
+            }
+            if (cur.empty())
+                u.assign(lv.begin(), lv.end());
+            else
+                u.swap(tmp);
+            lvs[cur.size()].swap(lv);
             cur.push_back(r);
         }
-        if (!cur.empty()) build_group(in, cur, built[(size_t)t], merged[(size_t)t]);
-    });
-    // 4. gather
+        if (!cur.empty()) build_group_known(in, cur, lvs, u, tab, masks, built[(size_t)t], merged[(size_t)t]);
+    };
+    if (nchunks == 1) {
+        greedy(0, m, 0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nchunks; ++t) th.emplace_back(greedy, m * t / nchunks, m * (t + 1) / nchunks, t);
+        for (auto& x : th) x.join();
+    }
+    // 3. gather: offsets first, then each chunk copies its groups in place
+    std::vector<int64_t> g_off((size_t)nchunks + 1, 0), e_off((size_t)nchunks + 1, 0);
     for (int t = 0; t < nchunks; ++t) {
+        int64_t ne = 0;
+        for (const Built& b : built[(size_t)t]) ne += (int64_t)b.ents.size();
+        g_off[(size_t)t + 1] = g_off[(size_t)t] + (int64_t)built[(size_t)t].size();
+        e_off[(size_t)t + 1] = e_off[(size_t)t] + ne;
         out.merge_rows.insert(out.merge_rows.end(), merged[(size_t)t].begin(), merged[(size_t)t].end());
+    }
+    out.hdr.resize((size_t)g_off[(size_t)nchunks]);
+    out.rows.resize((size_t)g_off[(size_t)nchunks] * kBitRowStride);
+    out.entries.resize((size_t)e_off[(size_t)nchunks]);
+    auto gather = [&](int t) {
+        int64_t gi = g_off[(size_t)t], eo = e_off[(size_t)t];
         for (Built& b : built[(size_t)t]) {
-            b.hdr.entry_off = (int64_t)out.entries.size();
-            out.entries.insert(out.entries.end(), b.ents.begin(), b.ents.end());
-            const int32_t gi = (int32_t)out.hdr.size();
+            b.hdr.entry_off = eo;
+            std::copy(b.ents.begin(), b.ents.end(), out.entries.begin() + eo);
+            eo += (int64_t)b.ents.size();
             for (int i = 0; i < kBitRowStride; ++i) {
                 const int32_t r = i < (int)b.rows.size() ? b.rows[(size_t)i] : -1;
-                out.rows.push_back(r);
-                if (r >= 0) out.row_group[(size_t)r] = gi;
+                out.rows[(size_t)(gi * kBitRowStride + i)] = r;
+                if (r >= 0) out.row_group[(size_t)r] = (int32_t)gi;
             }
-            out.hdr.push_back(b.hdr);
+            out.hdr[(size_t)gi] = b.hdr;
+            ++gi;
+            std::vector<BitEntry>().swap(b.ents);
         }
+    };
+    if (nchunks == 1) {
+        gather(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nchunks; ++t) th.emplace_back(gather, t);
+        for (auto& x : th) x.join();
     }
     std::sort(out.merge_rows.begin(), out.merge_rows.end());
 }

@@ -31,6 +31,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace siml {
@@ -93,10 +94,32 @@ struct BitGroupInput {
     int64_t n;
 };
 
+// Allocator whose value-less construct leaves trivial elements
+// uninitialised, so a resize that the caller overwrites at once does not
+// first fill the memory with zeros.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <class U>
+    void construct(U* p) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+
 struct BitGroupTables {
     std::vector<BitGroupHdr> hdr;
     std::vector<int32_t> rows;        // kBitRowStride per group, -1 for unused
-    std::vector<BitEntry> entries;
+    std::vector<BitEntry, NoInitAlloc<BitEntry>> entries;
     std::vector<int32_t> row_group;   // group of each row, -1 for merge rows
     std::vector<int32_t> merge_rows;  // rows served by the merge kernel, ascending
 };
