@@ -62,6 +62,12 @@ TRAFFIC_ALLPAIRS = (2.0 * 1051660 + 1.0 * 4264020) * 1024
 # VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 8.763e8
 # wave-instructions x 2 cycles / (1024 SIMDs x 2.4 GHz x its duration).
 VALU_INSTS_ALLPAIRS = 8.763e8
+# LDS busy fraction of the same kernel from its counter run: SQ_LDS_IDX_ACTIVE
+# 7.987e8 CU-cycles / (256 CUs x 4.64e6 cycles of GRBM_GUI_ACTIVE per XCC);
+# SQ_LDS_BANK_CONFLICT accounts for 5.089e8 of those cycles.  The table reads
+# (their bank conflicts and latency) bound this kernel, not the VALU.
+LDS_BUSY_ALLPAIRS = 7.987e8 / (256 * (37146851 / 8))
+LDS_CONFLICT_ALLPAIRS = 5.089e8 / (256 * (37146851 / 8))
 
 
 def log(msg):
@@ -175,7 +181,10 @@ def workload_allpairs(args):
                      "alg_ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
                      "hbm_frac": bytes_alg / kernel_s / B_HBM,
                      "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
-                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 2 / (1024 * 2.4e9 * kernel_s)) if traffic else None},
+                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 2 / (1024 * 2.4e9 * kernel_s)) if traffic else None,
+                     "lds_busy_frac": LDS_BUSY_ALLPAIRS if traffic else None,
+                     "lds_bank_conflict_frac": LDS_CONFLICT_ALLPAIRS if traffic else None,
+                     "limiter": "LDS table reads (bank conflicts and latency)" if traffic else None},
     })
     if not args.no_cpu:
         import oracle
