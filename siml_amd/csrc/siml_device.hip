@@ -1996,6 +1996,7 @@ struct siml_ctx {
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
+    std::vector<char> h_stage;       // staging of strided result copies
 };
 
 namespace {
@@ -2796,9 +2797,10 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     TRY(set_device(ctx));
     clear_result(ctx);
     const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
-    // device row pitch: a multiple of 4 floats, so a row's 256-column
-    // blocks are 16-byte aligned
-    const int64_t dp = (nq + 3) & ~int64_t(3);
+    // device row pitch: a multiple of 16 floats (16-byte aligned 256-column
+    // blocks, and the row pitch of the reference's padded result, so that
+    // GPULingo's retrieval is one linear copy)
+    const int64_t dp = (nq + 15) & ~int64_t(15);
     TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * dp, 1)));
     ctx->kind = SIML_RESULT_F32;
     ctx->rows = nr;
@@ -3438,14 +3440,44 @@ int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int
     return SIML_OK;
 }
 
+// Rows [0, rows) of a device matrix (row pitch dpitch elements) into host
+// memory with row pitch hpitch, cols elements per row; waits for the copy.
+// Equal pitches take one linear copy (the last row stops at cols).  Unequal
+// pitches go through a host staging buffer in slabs of rows: one linear
+// copy per slab, then a row-by-row host copy.  (A strided device-to-host
+// copy into pageable memory runs at a fraction of the linear rate.)
+static int copy_rows_d2h(siml_ctx* ctx, void* host, int64_t hpitch, const void* dev, int64_t dpitch, int64_t rows,
+                         int64_t cols, size_t esz) {
+    if (rows <= 0 || cols <= 0) return SIML_OK;
+    if (hpitch == dpitch) {
+        HIP_TRY(hipMemcpyAsync(host, dev, esz * (size_t)((rows - 1) * dpitch + cols), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return SIML_OK;
+    }
+    const size_t row_bytes = esz * (size_t)dpitch;
+    const int64_t slab = std::max<int64_t>(1, (int64_t)((size_t(64) << 20) / row_bytes));
+    std::vector<char>& stage = ctx->h_stage;
+    stage.resize(row_bytes * (size_t)std::min<int64_t>(slab, rows));
+    for (int64_t r0 = 0; r0 < rows; r0 += slab) {
+        const int64_t n = std::min<int64_t>(slab, rows - r0);
+        HIP_TRY(hipMemcpyAsync(stage.data(), static_cast<const char*>(dev) + row_bytes * (size_t)r0,
+                               esz * (size_t)((n - 1) * dpitch + cols), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int64_t r = 0; r < n; ++r)
+            std::memcpy(static_cast<char*>(host) + esz * (size_t)((r0 + r) * hpitch), stage.data() + row_bytes * (size_t)r,
+                        esz * (size_t)cols);
+    }
+    return SIML_OK;
+}
+
 static int retrieve_matrix(siml_ctx* ctx, void* host, int64_t pitch, int want, size_t esz) {
     if (ctx->kind != want) return fail(SIML_ESTATE, "no pending result of this kind");
     if (pitch < ctx->cols) return fail(SIML_EINVAL, "host pitch %lld < columns %lld", (long long)pitch, (long long)ctx->cols);
     TRY(set_device(ctx));
     if (ctx->rows > 0 && ctx->cols > 0) {
         if (!host) return fail(SIML_EINVAL, "null host buffer");
-        HIP_TRY(hipMemcpy2DAsync(host, esz * pitch, ctx->res.p, esz * ctx->dpitch, esz * ctx->cols, ctx->rows,
-                                 hipMemcpyDeviceToHost, ctx->stream));
+        TRY(copy_rows_d2h(ctx, host, pitch, ctx->res.p, ctx->dpitch, ctx->rows, ctx->cols, esz));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     clear_result(ctx);
@@ -3468,10 +3500,11 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
     if (r0 < 0 || r1 > ctx->rows || r1 < r0) return fail(SIML_EINVAL, "rows outside the pending result");
     if (pitch < ctx->cols) return fail(SIML_EINVAL, "host pitch smaller than columns");
     TRY(set_device(ctx));
-    if (r1 > r0 && ctx->cols > 0)
-        HIP_TRY(hipMemcpy2DAsync(host, sizeof(float) * pitch, ctx->res.as<float>() + r0 * ctx->dpitch,
-                                 sizeof(float) * ctx->dpitch, sizeof(float) * ctx->cols, r1 - r0,
-                                 hipMemcpyDeviceToHost, ctx->stream));
+    if (r1 > r0 && ctx->cols > 0) {
+        if (!host) return fail(SIML_EINVAL, "null host buffer");
+        TRY(copy_rows_d2h(ctx, host, pitch, ctx->res.as<float>() + r0 * ctx->dpitch, ctx->dpitch, r1 - r0, ctx->cols,
+                          sizeof(float)));
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
 }
