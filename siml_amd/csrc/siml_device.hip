@@ -2798,8 +2798,8 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     clear_result(ctx);
     const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
     // device row pitch: a multiple of 16 floats (16-byte aligned 256-column
-    // blocks, and the row pitch of the reference's padded result, so that
-    // GPULingo's retrieval is one linear copy)
+    // blocks, and the row pitch of the reference's padded result,
+    // GPULingo.py:378-379, so that GPULingo's retrieval is one linear copy)
     const int64_t dp = (nq + 15) & ~int64_t(15);
     TRY(ctx->res.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * dp, 1)));
     ctx->kind = SIML_RESULT_F32;
