@@ -74,10 +74,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(s, budget_s=10.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _oracle_rate(s, threads, budget_s):
     """Oracle pairs/s on R rows x all columns, R grown until ~budget_s."""
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
     rows = 8
     while True:
         sub = tuple(x[:rows] for x in s)
@@ -87,10 +97,20 @@ def cpu_baseline(s, budget_s=10.0):
         if dt >= budget_s or rows >= len(s[0]):
             break
         rows = min(len(s[0]), max(rows * 2, int(rows * budget_s / max(dt, 1e-3) * 1.05)))
-    pairs = rows * len(s[0])
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} rows x {len(s[0])} columns of the same workload, {dt:.1f} s",
-            "seconds": round(dt, 3)}
+    return rows, dt
+
+
+def cpu_baseline(s, budget_s=10.0):
+    """The oracle's C restatement of _CPULingo.c timed on this host: all
+    cores up to 16 (the reported value), and one core."""
+    threads = min(os.cpu_count() or 1, 16)
+    rows, dt = _oracle_rate(s, threads, budget_s)
+    rows1, dt1 = _oracle_rate(s, 1, budget_s / 4)
+    n = len(s[0])
+    return {"value": rows * n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows x {n} columns of the same workload, {dt:.1f} s",
+            "seconds": round(dt, 3), "value_1core": rows1 * n / dt1,
+            "sample_1core": f"{rows1} rows x {n} columns, {dt1:.1f} s", "cpu_model": cpu_model()}
 
 
 def copy_calibration(nbytes):
