@@ -1887,6 +1887,11 @@ struct RowSet {
             hdr.release();
             rows.release();
             ent.release();
+            reset();
+        }
+        // forget the grouping but keep the device buffers for the next one:
+        // no hipFree (which waits for the whole device) between sets
+        void reset() {
             h_hdr.clear();
             h_rows.clear();
             merge_rows.clear();
@@ -2119,8 +2124,8 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
         TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
         TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
     }
-    R.bits[0].release();
-    R.bits[1].release();
+    R.bits[0].reset();
+    R.bits[1].reset();
     R.h_ids.resize((size_t)total);
     R.h_counts.resize((size_t)total);
     for (int64_t i = 0; i < total; ++i) {
