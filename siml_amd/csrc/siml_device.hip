@@ -1478,6 +1478,13 @@ __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const 
             }
             transpose8(Q[q]);
             if constexpr (NB == 16) transpose8(Q[q] + 8);
+            (void)c;
+        }
+        // the columns' magnitudes and in-block positions, read after the
+        // accumulation so that they hold no registers during it
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t c = (blk * 4 + q) * kWave + lane;
             const bool valid = c < C.n;
             const int mb = valid ? C.mag[c] : 0;
             mbf[q] = (float)mb;
