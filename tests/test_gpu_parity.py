@@ -510,3 +510,31 @@ def test_evaluated_work_counter(demo_lines, demo_set, policy):
     h.call("siml_set_timing", 0)
     g.getNeighborsCSR(0, nr, 0.7)
     assert h.work_stats(reset=True) == (0, 0)
+
+
+def test_strided_result_copies(demo_lines):
+    """Results whose host row pitch differs from the device pitch (CPULingo's
+    contiguous arrays for a column count that is not a multiple of 16, and
+    explicit pitches) go through the staged copy, in more than one 64 MiB
+    slab here; every element lands in its row."""
+    ref = synth.packed_padded(17000, 20, 120, seed=31)
+    qry = synth.packed_padded(1001, 20, 120, seed=32)
+    c = CPULingo()
+    c.set_refsmiles(*ref)
+    c.set_qsmiles(*qry)
+    got = c.getMultipleRows(0, 17000)
+    assert got.shape == (17000, 1001) and got.flags["C_CONTIGUOUS"]
+    want = oracle.tanimoto_block(ref, qry, nthreads=0)
+    assert_same_f32(got, want)
+    g = GPULingo(0)
+    _set(g, ref, qry)
+    h = g._h
+    h.call("siml_multirow_async", 100, 140)
+    peek = np.full((40, 1003), -1.0, np.float32)
+    h.call("siml_peek_rows_f32", 5, 25, _lib.f32p(peek[5:]), 1003)
+    assert_same_f32(peek[5:25, :1001], want[105:125])
+    assert np.all(peek[5:25, 1001:] == -1.0) and np.all(peek[:5] == -1.0) and np.all(peek[25:] == -1.0)
+    out = np.full((40, 1020), -1.0, np.float32)
+    h.retrieve_f32(out, 1020)
+    assert_same_f32(out[:, :1001], want[100:140])
+    assert np.all(out[:, 1001:] == -1.0)
