@@ -136,15 +136,18 @@ class GPULingo:
 
     def launch_shape(self):
         """(workgroups aimed at per launch, most waves per top-k row,
-        top-k seeding) of the last call."""
+        top-k seed stride: 0 = no seeding) of the last call."""
         import ctypes
         t, c, sd = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
         self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c), ctypes.byref(sd))
-        return t.value, c.value, bool(sd.value)
+        return t.value, c.value, sd.value
 
-    def set_launch_shape(self, target_wg=8192, topk_wave_cap=256, topk_seed=False):
-        """Fixes the launch shape (turns the autotuner off)."""
-        self._h.call("siml_set_launch_shape", int(target_wg), int(topk_wave_cap), 1 if topk_seed else 0)
+    def set_launch_shape(self, target_wg=8192, topk_wave_cap=256, topk_seed=0):
+        """Fixes the launch shape (turns the autotuner off).  topk_seed: the
+        top-k seed stride (0 or False: none; True: every 16th query tile;
+        an int in [2, 64]: every topk_seed-th tile)."""
+        seed = 1 if topk_seed is True else int(topk_seed)
+        self._h.call("siml_set_launch_shape", int(target_wg), int(topk_wave_cap), seed)
 
     # -- helpers -----------------------------------------------------------
     def _rows(self, rowbase, rowlimit):

@@ -2003,7 +2003,9 @@ struct siml_ctx {
     // and, for top-k, the most waves per row (chosen by the autotuner when on)
     int64_t target_wg = 8192;
     unsigned topk_wave_cap = 256;
-    bool topk_seed = false;  // top-k: seed row thresholds from a column sample first
+    // top-k: seed row thresholds from every topk_seed-th query tile first
+    // (0: no seeding)
+    int topk_seed = 0;
     bool autotune = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
@@ -2395,7 +2397,7 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
 
 int launch_check(const char* name);
 
-// Top-k seeding samples every kSeedStride-th tile.
+// Top-k seeding samples every kSeedStride-th tile unless told otherwise.
 constexpr int64_t kSeedStride = 16;
 
 // Neighbour indices kept per (row, wave) slot by the count pass.
@@ -3168,14 +3170,15 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     };
     TimedRegion tr(ctx, nr * nq);
     const uint64_t* seed = nullptr;
-    if (bp.use && pos_only && ctx->topk_seed && ctx->qry->ntiles >= 4 * kSeedStride) {
-        // seeding: the k-th largest candidate key over every kSeedStride-th
-        // tile is reached by at least k keys of the full row (with pos_only,
+    const int64_t stride = ctx->topk_seed;
+    if (bp.use && pos_only && stride > 1 && ctx->qry->ntiles >= 4 * stride) {
+        // seeding: the k-th largest candidate key over every stride-th tile
+        // is reached by at least k keys of the full row (with pos_only,
         // a row with fewer than k sampled candidates gets seed 0: no limit);
         // the full pass then buffers only keys >= that seed
         ColArgs S = ctx->qry->args();
-        S.tstride = kSeedStride;
-        S.ntiles = (ctx->qry->ntiles + kSeedStride - 1) / kSeedStride;
+        S.tstride = stride;
+        S.ntiles = (ctx->qry->ntiles + stride - 1) / stride;
         const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid.y, (S.ntiles + 3) / 4));
         const int64_t nws = (int64_t)gys * (kBlock / kWave);
         TRY(launch_bits(ctx, bp, gys, rowbase, rowlimit,
@@ -3289,7 +3292,7 @@ enum TuneOp { kTuneMultirow = 1, kTuneHistRows, kTuneNeighbors, kTuneTopk, kTune
 struct TuneShape {
     int64_t target_wg;
     unsigned topk_wave_cap;
-    bool topk_seed;
+    int topk_seed;  // sample stride, 0: no seeding
 };
 
 std::mutex g_tune_mu;
@@ -3318,10 +3321,10 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
             return impl();
         }
     }
-    static const TuneShape kCands[] = {{8192, 256, false}, {2048, 256, false}, {4096, 256, false},
-                                       {16384, 256, false}, {32768, 1024, false}, {8192, 64, false},
-                                       {8192, 256, true}, {2048, 256, true}, {32768, 1024, true}};
-    const int ncand = op == kTuneTopk ? 9 : 5;
+    static const TuneShape kCands[] = {{8192, 256, 0},   {2048, 256, 0},  {4096, 256, 0},  {16384, 256, 0},
+                                       {32768, 1024, 0}, {8192, 64, 0},   {8192, 256, 16}, {2048, 256, 16},
+                                       {32768, 1024, 16}, {8192, 256, 4}, {2048, 256, 4}};
+    const int ncand = op == kTuneTopk ? 11 : 5;
     TRY(set_device(ctx));
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -3399,7 +3402,7 @@ int siml_set_autotune(siml_ctx* ctx, int on) {
     if (!ctx->autotune) {
         ctx->target_wg = 8192;
         ctx->topk_wave_cap = 256;
-        ctx->topk_seed = false;
+        ctx->topk_seed = 0;
     }
     return SIML_OK;
 }
@@ -3408,7 +3411,7 @@ int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap,
     if (!ctx || !target_wg || !topk_wave_cap || !topk_seed) return fail(SIML_EINVAL, "null pointer");
     *target_wg = ctx->target_wg;
     *topk_wave_cap = (int32_t)ctx->topk_wave_cap;
-    *topk_seed = ctx->topk_seed ? 1 : 0;
+    *topk_seed = ctx->topk_seed;
     return SIML_OK;
 }
 
@@ -3418,7 +3421,9 @@ int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_ca
     ctx->autotune = false;
     ctx->target_wg = target_wg;
     ctx->topk_wave_cap = (unsigned)topk_wave_cap;
-    ctx->topk_seed = topk_seed != 0;
+    if (topk_seed < 0 || topk_seed > 64) return fail(SIML_EINVAL, "top-k seed stride %d outside [0, 64]", topk_seed);
+    // 1 asks for seeding at the default stride
+    ctx->topk_seed = topk_seed == 1 ? (int)kSeedStride : topk_seed;
     return SIML_OK;
 }
 

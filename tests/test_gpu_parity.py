@@ -445,12 +445,13 @@ def test_neighbors_single_and_two_pass(demo_lines, demo_set, lo):
     assert np.array_equal(idx, np.concatenate(lists))
 
 
+@pytest.mark.parametrize("stride", [16, 4, 2])
 @pytest.mark.parametrize("k", [1, 50, 256])
-def test_topk_seeded_real_smiles(demo_lines, k):
+def test_topk_seeded_real_smiles(demo_lines, k, stride):
     """Real SMILES (dense similarities): top-k with the row thresholds seeded
-    from every 16th query tile equals the oracle."""
+    from every stride-th query tile equals the oracle."""
     g = GPULingo(0)
-    g.set_launch_shape(2048, 256, True)
+    g.set_launch_shape(2048, 256, stride)
     ref = oracle.pack(demo_lines[100:220])
     qry = oracle.pack(demo_lines)
     _set(g, ref, qry)
@@ -458,7 +459,7 @@ def test_topk_seeded_real_smiles(demo_lines, k):
     oi, ot = oracle.topk(ref, qry, k)
     assert np.array_equal(idx, oi)
     assert_same_f32(t, ot)
-    assert g.launch_shape()[2] is True
+    assert g.launch_shape()[2] == stride
 
 
 def test_valu_peak_microbenchmark():

@@ -181,7 +181,7 @@ def test_autotune_changes_no_result(db_and_queries):
     db.close()
 
 
-@pytest.mark.parametrize("shape", [(8192, 256, True), (2048, 64, True), (32768, 1024, False), (1024, 16, True)])
+@pytest.mark.parametrize("shape", [(8192, 256, 16), (2048, 64, 16), (32768, 1024, 0), (1024, 16, 16), (4096, 256, 4)])
 def test_topk_launch_shapes_and_seeding(db_and_queries, shape):
     """Top-k is the same for every launch shape, with and without seeding
     the row thresholds from a column sample, on sparse (random) and dense
