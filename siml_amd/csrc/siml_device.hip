@@ -1410,11 +1410,34 @@ __device__ __forceinline__ float pair_tanimoto(float fi, float maf, float mbf, b
 
 // Workgroup prologue of the bit-sliced kernels: the group's table and row
 // information into LDS.
+// Workgroup -> (group slot, column range) of a bit-sliced launch.  The
+// dispatcher deals consecutive workgroups to the 8 XCDs in turn; when the
+// number of column ranges Y is a multiple of 8, workgroup L goes to XCD
+// L % 8 and takes a column range y == L % 8 (mod 8), so each XCD's L2 only
+// ever sees an eighth of the column stream; within an XCD consecutive
+// workgroups take consecutive groups of the same range, so the workgroups
+// in flight share their column data.  Otherwise (x, y) = blockIdx.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void bits_wg(int& gslot, int& y, int& ny) {
+    ny = (int)gridDim.y;
+    if (ny % kXcds == 0) {
+        const unsigned L = blockIdx.x + gridDim.x * blockIdx.y;
+        const unsigned xcd = L % kXcds, slot = L / kXcds;
+        gslot = (int)(slot % gridDim.x);
+        y = (int)(xcd + kXcds * (slot / gridDim.x));
+    } else {
+        y = (int)blockIdx.y;
+        gslot = (int)blockIdx.x;
+    }
+}
+
 template <class Epi>
 __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
                                               int64_t rl, const Epi& epi, uint2* tab, BitRowInfo& ri,
                                               typename Epi::Lds& el) {
-    const int g = G.glist[blockIdx.x];
+    int gslot, y, ny;
+    bits_wg(gslot, y, ny);
+    const int g = G.glist[gslot];
     const siml::BitGroupHdr hd = G.hdr[g];
     for (int s = threadIdx.x; s < siml::kBitSlots; s += blockDim.x) tab[s] = make_uint2(0u, 0u);
     if (threadIdx.x < 32) {
@@ -1457,7 +1480,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const 
     const int lane = lane_id();
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
     const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
-    const int64_t gw = (int64_t)blockIdx.y * wpb + w, nw = (int64_t)gridDim.y * wpb;
+    int gslot, y, ny;
+    bits_wg(gslot, y, ny);
+    const int64_t gw = (int64_t)y * wpb + w, nw = (int64_t)ny * wpb;
     const int64_t nblocks = (C.ntiles + 3) / 4;
     const int64_t b0 = (nblocks * gw) / nw, b1 = (nblocks * (gw + 1)) / nw;
     float* stage = el.stage[w];  // 256 column slots + 64 for lanes past the end
@@ -1530,7 +1555,15 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
     typename Epi::Reg R;
     int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
+    {
+        int gslot, y, ny;
+        bits_wg(gslot, y, ny);
+        const int wpb = blockDim.x / kWave;
+        gw = (int64_t)y * wpb + (threadIdx.x / kWave);
+        nw = (int64_t)ny * wpb;
+        t0 = (C.ntiles * gw) / nw;
+        t1 = (C.ntiles * (gw + 1)) / nw;
+    }
     epi.begin(el, R, ri, gw, nw);
     for (int64_t ti = t0; ti < t1; ++ti) {
         const int64_t t = ti * C.tstride;
@@ -2439,6 +2472,17 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
     return launch_check(name);
 }
 
+// Column ranges of a bit-sliced launch rounded to a multiple of the XCD
+// count, so that bits_wg keeps each XCD on an eighth of the column stream,
+// when that stays within cap (small counts stay as they are).
+unsigned xcd_gy(unsigned gy, unsigned cap) {
+    if (gy < 4) return gy;
+    const unsigned up = (gy + kXcds - 1) / kXcds * kXcds;
+    if (up <= cap) return up;
+    const unsigned down = gy / kXcds * kXcds;
+    return down >= (unsigned)kXcds ? down : gy;
+}
+
 // Grid for row-parallel kernels: grid.x = rows, grid.y splits the query
 // tiles so the launch has enough workgroups to fill 256 CUs several times.
 dim3 row_grid(int64_t nrows, int64_t ntiles, int64_t target_wg = 8192, int64_t max_slots = 0) {
@@ -2827,7 +2871,8 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
-        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
+        const unsigned gy = xcd_gy(
+            row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y, 65535);
         if (ctx->qry->blocked)
             TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32Block{ctx->res.as<float>(), dp, rowbase},
                             "k_bits<f32 block>"));
@@ -2912,7 +2957,8 @@ static int hist_rows_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
-        const unsigned gy = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
+        const unsigned gy = xcd_gy(
+            row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y, 65535);
         TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiHist101{hist, rowbase}, "k_bits<hist101>"));
         if (!bp.merge.empty()) {
             k_hist_rows<<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
@@ -2973,7 +3019,8 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     if (bp.use) {
         // bit-sliced groups hold up to 28 rows: size grid.y by groups
         const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
-        grid.y = (unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, (int64_t(1) << 24) / (nr * (kBlock / kWave))));
+        const int64_t cap = std::min<int64_t>(65535, (int64_t(1) << 24) / (nr * (kBlock / kWave)));
+        grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
     }
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t ng = plan.g1 - plan.g0;
@@ -3117,6 +3164,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? ctx->target_wg : 2048);
     grid.y = std::min<unsigned>(grid.y, ctx->topk_wave_cap);
+    if (bp.use) grid.y = xcd_gy(grid.y, std::min<unsigned>(ctx->topk_wave_cap, 65535));
     int64_t nw = (int64_t)grid.y * (kBlock / kWave);
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
@@ -3246,6 +3294,7 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     // one workgroup counts at most 28 rows x its columns in uint32 LDS bins
     const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
+    if (bp.use) grid.y = std::max<unsigned>(xcd_gy(grid.y, 65535), (unsigned)std::min<int64_t>(min_gy, 65535));
     const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
     if (bp.use) {
