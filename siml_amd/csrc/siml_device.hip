@@ -3370,10 +3370,12 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
             return impl();
         }
     }
-    static const TuneShape kCands[] = {{8192, 256, 0},   {2048, 256, 0},  {4096, 256, 0},  {16384, 256, 0},
-                                       {32768, 1024, 0}, {8192, 64, 0},   {8192, 256, 16}, {2048, 256, 16},
-                                       {32768, 1024, 16}, {8192, 256, 4}, {2048, 256, 4}};
-    const int ncand = op == kTuneTopk ? 11 : 5;
+    // the first 6 apply to every operation, the rest to top-k only
+    static const TuneShape kCands[] = {{8192, 256, 0},   {2048, 256, 0},    {4096, 256, 0},
+                                       {16384, 256, 0},  {32768, 1024, 0},  {65536, 1024, 0},
+                                       {8192, 64, 0},    {8192, 256, 16},   {2048, 256, 16},
+                                       {32768, 1024, 16}, {8192, 256, 4},   {2048, 256, 4}};
+    const int ncand = op == kTuneTopk ? 12 : 6;
     TRY(set_device(ctx));
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
