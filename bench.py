@@ -636,7 +636,7 @@ def main():
     ap.add_argument("--traffic", type=float, default=None,
                     help="all-pairs: HBM bytes per launch from counter runs, reported as roofline.traffic")
     ap.add_argument("--policy", choices=["auto", "hash8", "merge"], default="auto",
-                    help="auto: bit-sliced 28-row kernels; hash8: 8-row hashed kernels; "
+                    help="auto: bit-sliced 32-row kernels; hash8: 8-row hashed kernels; "
                          "merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural", "sorted"], default="block",
                     help="query storage order: size-sorted 256-column blocks, the caller's order, or fully "

@@ -22,7 +22,7 @@
 // (cuckoo placement) holding (key, mask), bit r of mask saying that row r of
 // the group has the level.  Empty slots hold (0, 0).  Probing a column level
 // reads both candidate slots and takes the mask of the one whose key is
-// equal; summing the masks as 28 vertical binary counters gives every row's
+// equal; summing the masks as 32 vertical binary counters gives every row's
 // intersection with the column at once.
 //
 // Rows qualify when their ids are strictly ascending and ASCII, every count
@@ -36,7 +36,7 @@
 
 namespace siml {
 
-constexpr int kBitRows = 28;
+constexpr int kBitRows = 32;
 constexpr int kBitRowStride = 32;  // row indices stored per group
 constexpr int kBitSlotBits = 12;
 constexpr int kBitSlots = 1 << kBitSlotBits;  // 4096 slots x 8 B = 32 KiB of LDS

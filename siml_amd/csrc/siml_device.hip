@@ -976,7 +976,7 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
 // mask)) in LDS.  Each lane streams one query molecule's levels as (key,
 // slot word) pairs, two levels per 16-byte load; a level reads its two
 // candidate slots at the precomputed byte offsets and keeps the mask of the
-// slot whose key is equal.  The masks are added to 28 vertical binary
+// slot whose key is equal.  The masks are added to 32 vertical binary
 // counters held as bit planes P[0..NB): blocks of 8 masks go through a
 // carry-save adder tree (xor3 / majority, one v_bitop3 each) and the
 // block's carry ripples into the upper planes.  A bit-plane transpose then
@@ -3017,7 +3017,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
     if (bp.use) {
-        // bit-sliced groups hold up to 28 rows: size grid.y by groups
+        // bit-sliced groups hold up to 32 rows: size grid.y by groups
         const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
         const int64_t cap = std::min<int64_t>(65535, (int64_t(1) << 24) / (nr * (kBlock / kWave)));
         grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
@@ -3291,7 +3291,7 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
                          bp.use ? ctx->target_wg : 2048);
-    // one workgroup counts at most 28 rows x its columns in uint32 LDS bins
+    // one workgroup counts at most 32 rows x its columns in uint32 LDS bins
     const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
     if (bp.use) grid.y = std::max<unsigned>(xcd_gy(grid.y, 65535), (unsigned)std::min<int64_t>(min_gy, 65535));

@@ -89,7 +89,7 @@ int main(int argc, char** argv) {
                         const uint32_t k = level_key(C.ids[j], (uint32_t)t), sw = slot_word(k);
                         const uint32_t a1 = (sw & 0xFFFFu) / 8u, a2 = (sw >> 16) / 8u;
                         const uint32_t m = key[a1] == k ? mask[a1] : (key[a2] == k ? mask[a2] : 0u);
-                        for (int r = 0; r < 28; ++r) cnt[r] += (m >> r) & 1u;
+                        for (int r = 0; r < kBitRows; ++r) cnt[r] += (m >> r) & 1u;
                     }
                 }
                 for (int r = 0; r < h.nrows; ++r) {
