@@ -75,55 +75,14 @@ void emit_group(const std::vector<int32_t>& rows, const std::vector<uint32_t>& k
     out.push_back(std::move(b));
 }
 
-// Builds the table of `rows`; on a placement failure the rows are split in
-// two halves, each built again.  A single row that cannot be placed goes to
-// `merge`.
-void build_group(const BitGroupInput& in, const std::vector<int32_t>& rows, std::vector<Built>& out,
-                 std::vector<int32_t>& merge) {
-    std::vector<uint32_t> lv;
-    std::vector<std::pair<uint32_t, uint32_t>> kv;
-    int32_t maxsum = 0;
-    for (size_t i = 0; i < rows.size(); ++i) {
-        row_levels(in, rows[i], lv);
-        for (uint32_t k : lv) kv.emplace_back(k, 1u << i);
-        maxsum = std::max(maxsum, (int32_t)lv.size());
-    }
-    std::sort(kv.begin(), kv.end());
-    std::vector<uint32_t> keys, masks;
-    for (auto& e : kv) {
-        if (!keys.empty() && keys.back() == e.first) {
-            masks.back() |= e.second;
-        } else {
-            keys.push_back(e.first);
-            masks.push_back(e.second);
-        }
-    }
-    std::vector<int32_t> tab;
-    if (!place(keys, tab)) {
-        if (rows.size() == 1) {
-            merge.push_back(rows[0]);
-            return;
-        }
-        const size_t h = rows.size() / 2;
-        build_group(in, std::vector<int32_t>(rows.begin(), rows.begin() + (long)h), out, merge);
-        build_group(in, std::vector<int32_t>(rows.begin() + (long)h, rows.end()), out, merge);
-        return;
-    }
-    emit_group(rows, keys, masks, tab, maxsum, out);
-}
-
-// build_group for rows whose level lists lvs[i] and their sorted union u
-// the caller already holds (the greedy grouping computes both): the keys
-// are placed first, then each level finds its key's index through the
-// placed table (slot h1, else h2) and sets its row's mask bit.
-void build_group_known(const BitGroupInput& in, const std::vector<int32_t>& rows,
-                       const std::vector<std::vector<uint32_t>>& lvs, const std::vector<uint32_t>& u,
-                       std::vector<int32_t>& tab, std::vector<uint32_t>& masks, std::vector<Built>& out,
-                       std::vector<int32_t>& merge) {
-    if (!place(u, tab)) {
-        build_group(in, rows, out, merge);  // splits the rows
-        return;
-    }
+// Places the sorted key union u of `rows` (level lists lvs[i]) and emits the
+// group; false (nothing emitted) when the keys cannot be placed.  Each level
+// finds its key's index through the placed table (slot h1, else h2) and
+// sets its row's mask bit.
+bool try_group(const std::vector<int32_t>& rows, const std::vector<std::vector<uint32_t>>& lvs,
+               const std::vector<uint32_t>& u, std::vector<int32_t>& tab, std::vector<uint32_t>& masks,
+               std::vector<Built>& out) {
+    if (!place(u, tab)) return false;
     masks.assign(u.size(), 0u);
     int32_t maxsum = 0;
     for (size_t i = 0; i < rows.size(); ++i) {
@@ -138,6 +97,7 @@ void build_group_known(const BitGroupInput& in, const std::vector<int32_t>& rows
         maxsum = std::max(maxsum, (int32_t)lv.size());
     }
     emit_group(rows, u, masks, tab, maxsum, out);
+    return true;
 }
 
 }  // namespace
@@ -200,28 +160,43 @@ void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out,
         std::vector<std::vector<uint32_t>> lvs((size_t)kBitRows);
         std::vector<int32_t> cur, tab;
         std::vector<uint32_t> masks;
-        for (int64_t i = a; i < b; ++i) {
-            const int32_t r = order[(size_t)i];
-            row_levels(in, r, lv);
-            bool close = (int)cur.size() == kBitRows;
-            if (!close && !cur.empty()) {
-                tmp.resize(u.size() + lv.size());
-                tmp.resize((size_t)(std::set_union(u.begin(), u.end(), lv.begin(), lv.end(), tmp.begin()) -
-                                    tmp.begin()));
-                close = (int64_t)tmp.size() > kBitMaxKeys;
+        int64_t i = a;
+        while (i < b) {
+            // take rows while the group has room and its key union stays
+            // within kBitMaxKeys
+            cur.clear();
+            int64_t j = i;
+            for (; j < b && (int)cur.size() < kBitRows; ++j) {
+                row_levels(in, order[(size_t)j], lv);
+                if (cur.empty()) {
+                    u.assign(lv.begin(), lv.end());
+                } else {
+                    tmp.resize(u.size() + lv.size());
+                    tmp.resize((size_t)(std::set_union(u.begin(), u.end(), lv.begin(), lv.end(), tmp.begin()) -
+                                        tmp.begin()));
+                    if ((int64_t)tmp.size() > kBitMaxKeys) break;
+                    u.swap(tmp);
+                }
+                lvs[cur.size()].swap(lv);
+                cur.push_back(order[(size_t)j]);
             }
-            if (close) {
-                build_group_known(in, cur, lvs, u, tab, masks, built[(size_t)t], merged[(size_t)t]);
-                cur.clear();
+            // a failed placement gives the group's last row back to the next
+            // group until the keys fit; a single row that does not fit goes
+            // to the merge kernel
+            while (!try_group(cur, lvs, u, tab, masks, built[(size_t)t])) {
+                if (cur.size() == 1) {
+                    merged[(size_t)t].push_back(cur[0]);
+                    break;
+                }
+                cur.pop_back();
+                --j;
+                u.clear();
+                for (size_t r = 0; r < cur.size(); ++r) u.insert(u.end(), lvs[r].begin(), lvs[r].end());
+                std::sort(u.begin(), u.end());
+                u.erase(std::unique(u.begin(), u.end()), u.end());
             }
-            if (cur.empty())
-                u.assign(lv.begin(), lv.end());
-            else
-                u.swap(tmp);
-            lvs[cur.size()].swap(lv);
-            cur.push_back(r);
+            i = j;
         }
-        if (!cur.empty()) build_group_known(in, cur, lvs, u, tab, masks, built[(size_t)t], merged[(size_t)t]);
     };
     if (nchunks == 1) {
         greedy(0, m, 0);

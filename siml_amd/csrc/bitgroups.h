@@ -40,7 +40,7 @@ constexpr int kBitRows = 32;
 constexpr int kBitRowStride = 32;  // row indices stored per group
 constexpr int kBitSlotBits = 12;
 constexpr int kBitSlots = 1 << kBitSlotBits;  // 4096 slots x 8 B = 32 KiB of LDS
-constexpr int kBitMaxKeys = 1800;             // load factor <= 0.44
+constexpr int kBitMaxKeys = 2000;             // load factor <= 0.49
 constexpr int kBitMaxLevel = 15;
 
 inline bool ascii_id(int32_t x) { return ((uint32_t)x & 0x80808080u) == 0u; }
@@ -126,7 +126,8 @@ struct BitGroupTables {
 
 // Consecutive rows form groups; with by_mag the eligible rows are first
 // ordered by (magnitude, index) so that a group's rows have close
-// magnitudes.  A group whose keys cannot be placed is split in two.
+// magnitudes.  A group whose keys cannot be placed gives its last rows
+// back to the next group until they can.
 void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out, int nthreads);
 
 }  // namespace siml
