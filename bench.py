@@ -55,20 +55,20 @@ P_VALU = 256 * 128 * 2.4e9
 B_HBM = 8.0e12                     # bytes/s
 N_ALLPAIRS = 32768
 # HBM bytes per launch of the default all-pairs kernel, from rocprofv3 counter
-# runs of this workload: FETCH_SIZE 632,502 KiB x 2.0 + WRITE_SIZE
+# runs of this workload: FETCH_SIZE 648,849 KiB x 2.0 + WRITE_SIZE
 # 4,194,304 KiB x 1.0 (the matrix exactly), the factors being what the same
 # counters read for a device copy of known size
 # (profiles/k_bits_f32block_allpairs32768.md).
-TRAFFIC_ALLPAIRS = (2.0 * 632502 + 1.0 * 4194304) * 1024
-# VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 8.767e8
+TRAFFIC_ALLPAIRS = (2.0 * 648849 + 1.0 * 4194304) * 1024
+# VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 7.964e8
 # wave-instructions x 2 cycles / (1024 SIMDs x 2.4 GHz x its duration).
-VALU_INSTS_ALLPAIRS = 8.767e8
+VALU_INSTS_ALLPAIRS = 7.964e8
 # LDS busy fraction of the same kernel from its counter run: SQ_LDS_IDX_ACTIVE
-# 7.994e8 CU-cycles / (256 CUs x 4.053e6 cycles of GRBM_GUI_ACTIVE per XCC);
-# SQ_LDS_BANK_CONFLICT accounts for 5.091e8 of those cycles.  The table reads
+# 7.269e8 CU-cycles / (256 CUs x 3.915e6 cycles of GRBM_GUI_ACTIVE per XCC);
+# SQ_LDS_BANK_CONFLICT accounts for 4.605e8 of those cycles.  The table reads
 # (their bank conflicts and latency) bound this kernel, not the VALU.
-LDS_BUSY_ALLPAIRS = 7.994e8 / (256 * (32425770 / 8))
-LDS_CONFLICT_ALLPAIRS = 5.091e8 / (256 * (32425770 / 8))
+LDS_BUSY_ALLPAIRS = 7.269e8 / (256 * (31318790 / 8))
+LDS_CONFLICT_ALLPAIRS = 4.605e8 / (256 * (31318790 / 8))
 
 
 def log(msg):
