@@ -1803,6 +1803,29 @@ struct DevBuf {
     }
 };
 
+// Pinned host staging buffer (hipHostMalloc), grown on demand.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes) {
+        if (bytes <= cap && p) return SIML_OK;
+        release();
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SIML_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        }
+        cap = bytes;
+        return SIML_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 // Host view of a SMILES set given either as padded matrices or ragged.
 struct HostSet {
     const int32_t* ids = nullptr;
@@ -2044,6 +2067,7 @@ struct siml_ctx {
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
     std::vector<char> h_stage;       // staging of strided result copies
+    PinnedBuf pinned[2];             // double-buffered staging of large result copies
 };
 
 namespace {
@@ -2614,6 +2638,8 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->glist.release();
     ctx->csr_dev.release();
     ctx->work.release();
+    ctx->pinned[0].release();
+    ctx->pinned[1].release();
     for (auto& pr : ctx->pending_ev) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
@@ -3514,16 +3540,91 @@ int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int
 // pitches go through a host staging buffer in slabs of rows: one linear
 // copy per slab, then a row-by-row host copy.  (A strided device-to-host
 // copy into pageable memory runs at a fraction of the linear rate.)
+// Host threads of the copy from pinned staging into the caller's memory:
+// OMP_NUM_THREADS when set (the process's CPU share), at most 16.
+static int copy_threads() {
+    const char* env = std::getenv("OMP_NUM_THREADS");
+    int n = env ? std::atoi(env) : 0;
+    if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::min(n, 16);
+}
+
+// Rows [r0, r0 + n) of a staged slab (row pitch dpitch) into host memory
+// (row pitch hpitch), cols elements each, split over host threads.
+static void scatter_rows(void* host, int64_t hpitch, const char* stage, int64_t dpitch, int64_t r0, int64_t n,
+                         int64_t cols, size_t esz, int nthreads) {
+    auto work = [&](int64_t a, int64_t b) {
+        if (hpitch == dpitch) {
+            const size_t bytes_a = esz * (size_t)(a * dpitch), bytes_b = esz * (size_t)((b - 1) * dpitch + cols);
+            std::memcpy(static_cast<char*>(host) + esz * (size_t)(r0 * hpitch) + bytes_a, stage + bytes_a,
+                        bytes_b - bytes_a);
+            return;
+        }
+        for (int64_t r = a; r < b; ++r)
+            std::memcpy(static_cast<char*>(host) + esz * (size_t)((r0 + r) * hpitch), stage + esz * (size_t)(r * dpitch),
+                        esz * (size_t)cols);
+    };
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, n));
+    if (nt == 1) {
+        work(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, n * t / nt, n * (t + 1) / nt);
+    work(0, n / nt);
+    for (auto& x : th) x.join();
+}
+
 static int copy_rows_d2h(siml_ctx* ctx, void* host, int64_t hpitch, const void* dev, int64_t dpitch, int64_t rows,
                          int64_t cols, size_t esz) {
     if (rows <= 0 || cols <= 0) return SIML_OK;
+    const size_t row_bytes = esz * (size_t)dpitch;
+    const size_t total = esz * (size_t)((rows - 1) * dpitch + cols);
+    if (total >= (size_t(32) << 20)) {
+        // large results: device -> pinned slab (at PCIe rate) while host
+        // threads copy the previous slab into the caller's memory, which
+        // also spreads its first-touch page faults over the threads
+        const int64_t slab = std::max<int64_t>(1, (int64_t)((size_t(64) << 20) / row_bytes));
+        const size_t slab_bytes = row_bytes * (size_t)std::min<int64_t>(slab, rows);
+        TRY(ctx->pinned[0].reserve(slab_bytes));
+        TRY(ctx->pinned[1].reserve(slab_bytes));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        if (hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(ev[0]);
+            return fail(SIML_EDEVICE, "hipEventCreate failed");
+        }
+        const int nthreads = copy_threads();
+        const int64_t nslabs = (rows + slab - 1) / slab;
+        auto issue = [&](int64_t s) -> hipError_t {
+            const int64_t r0 = s * slab, n = std::min<int64_t>(slab, rows - r0);
+            hipError_t e = hipMemcpyAsync(ctx->pinned[s % 2].p, static_cast<const char*>(dev) + row_bytes * (size_t)r0,
+                                          esz * (size_t)((n - 1) * dpitch + cols), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[s % 2], ctx->stream);
+            return e;
+        };
+        hipError_t e = issue(0);
+        for (int64_t s = 0; s < nslabs && e == hipSuccess; ++s) {
+            if (s + 1 < nslabs) e = issue(s + 1);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[s % 2]);
+            if (e == hipSuccess) {
+                const int64_t r0 = s * slab, n = std::min<int64_t>(slab, rows - r0);
+                scatter_rows(host, hpitch, static_cast<const char*>(ctx->pinned[s % 2].p), dpitch, r0, n, cols, esz,
+                             nthreads);
+            }
+        }
+        if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (e != hipSuccess) return fail(SIML_EDEVICE, "result copy failed: %s", hipGetErrorString(e));
+        return SIML_OK;
+    }
     if (hpitch == dpitch) {
         HIP_TRY(hipMemcpyAsync(host, dev, esz * (size_t)((rows - 1) * dpitch + cols), hipMemcpyDeviceToHost,
                                ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         return SIML_OK;
     }
-    const size_t row_bytes = esz * (size_t)dpitch;
     const int64_t slab = std::max<int64_t>(1, (int64_t)((size_t(64) << 20) / row_bytes));
     std::vector<char>& stage = ctx->h_stage;
     stage.resize(row_bytes * (size_t)std::min<int64_t>(slab, rows));

@@ -516,8 +516,9 @@ def test_evaluated_work_counter(demo_lines, demo_set, policy):
 def test_strided_result_copies(demo_lines):
     """Results whose host row pitch differs from the device pitch (CPULingo's
     contiguous arrays for a column count that is not a multiple of 16, and
-    explicit pitches) go through the staged copy, in more than one 64 MiB
-    slab here; every element lands in its row."""
+    explicit pitches) and large results of equal pitch go through the
+    staged copies (pinned slabs of 64 MiB above 32 MiB, two slabs here);
+    every element lands in its row."""
     ref = synth.packed_padded(17000, 20, 120, seed=31)
     qry = synth.packed_padded(1001, 20, 120, seed=32)
     c = CPULingo()
@@ -529,6 +530,8 @@ def test_strided_result_copies(demo_lines):
     assert_same_f32(got, want)
     g = GPULingo(0)
     _set(g, ref, qry)
+    # GPULingo's padded host array has the device pitch: the contiguous path
+    assert_same_f32(g.getMultipleRows(0, 17000), want)
     h = g._h
     h.call("siml_multirow_async", 100, 140)
     peek = np.full((40, 1003), -1.0, np.float32)
