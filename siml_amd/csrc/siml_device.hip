@@ -3640,6 +3640,22 @@ static int copy_rows_d2h(siml_ctx* ctx, void* host, int64_t hpitch, const void* 
     return SIML_OK;
 }
 
+// n contiguous elements of esz bytes from the device: the bulk as rows of
+// 16384 elements through copy_rows_d2h (pinned slabs when large), the tail
+// with one copy.
+static int copy_d2h(siml_ctx* ctx, void* host, const void* dev, int64_t n, size_t esz) {
+    constexpr int64_t W = 16384;
+    const int64_t rows = n / W, done = rows * W;
+    if (rows > 0) TRY(copy_rows_d2h(ctx, host, W, dev, W, rows, W, esz));
+    if (n > done) {
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(host) + esz * (size_t)done,
+                               static_cast<const char*>(dev) + esz * (size_t)done, esz * (size_t)(n - done),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return SIML_OK;
+}
+
 static int retrieve_matrix(siml_ctx* ctx, void* host, int64_t pitch, int want, size_t esz) {
     if (ctx->kind != want) return fail(SIML_ESTATE, "no pending result of this kind");
     if (pitch < ctx->cols) return fail(SIML_EINVAL, "host pitch %lld < columns %lld", (long long)pitch, (long long)ctx->cols);
@@ -3686,7 +3702,7 @@ int siml_retrieve_csr(siml_ctx* ctx, int64_t* offsets, int32_t* indices) {
     std::memcpy(offsets, ctx->csr_rows.data(), sizeof(int64_t) * ctx->csr_rows.size());
     if (ctx->nnz > 0) {
         if (!indices) return fail(SIML_EINVAL, "null indices buffer");
-        HIP_TRY(hipMemcpyAsync(indices, ctx->res.p, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+        TRY(copy_d2h(ctx, indices, ctx->res.p, ctx->nnz, sizeof(int32_t)));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     clear_result(ctx);
@@ -3700,8 +3716,8 @@ int siml_retrieve_topk(siml_ctx* ctx, int32_t* idx, float* t) {
     const size_t n = (size_t)(ctx->rows * ctx->cols);
     if (n) {
         if (!idx || !t) return fail(SIML_EINVAL, "null output buffer");
-        HIP_TRY(hipMemcpyAsync(idx, ctx->res.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(t, ctx->res2.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        TRY(copy_d2h(ctx, idx, ctx->res.p, (int64_t)n, sizeof(int32_t)));
+        TRY(copy_d2h(ctx, t, ctx->res2.p, (int64_t)n, sizeof(float)));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     clear_result(ctx);
