@@ -1,5 +1,6 @@
 // bitgroups.cpp -- host builder of the bit-sliced row groups (bitgroups.h).
 #include "bitgroups.h"
+#include "host_threads.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -105,17 +106,7 @@ bool try_group(const std::vector<int32_t>& rows, const std::vector<std::vector<u
 void build_bit_groups(const BitGroupInput& in, bool by_mag, BitGroupTables& out, int nthreads) {
     out = BitGroupTables();
     out.row_group.assign((size_t)in.n, -1);
-    // default thread count: OMP_NUM_THREADS when set (the process's CPU
-    // share), else the hardware threads, at most 32
-    int nt = nthreads;
-    if (nt <= 0) {
-        const char* env = std::getenv("OMP_NUM_THREADS");
-        nt = env ? std::atoi(env) : 0;
-    }
-    if (nt <= 0) {
-        const unsigned hw = std::thread::hardware_concurrency();
-        nt = (int)std::min<unsigned>(hw ? hw : 1, 32);
-    }
+    const int nt = nthreads > 0 ? nthreads : default_threads(32);
     auto run = [&](int64_t n, auto&& fn) {
         if (n < 256 || nt <= 1) {
             fn(0, n, 0);

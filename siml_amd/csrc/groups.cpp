@@ -1,5 +1,6 @@
 // groups.cpp -- host builder of the row-group cuckoo tables (see groups.h).
 #include "groups.h"
+#include "host_threads.h"
 
 #include <algorithm>
 #include <climits>
@@ -188,8 +189,7 @@ bool build_groups(const GroupInput& in, GroupTables& out, int nthreads) {
             h.empty_key = (int32_t)e;
         }
     };
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = nthreads > 0 ? nthreads : (int)std::min<unsigned>(hw ? hw : 1, 32);
+    const int nt = nthreads > 0 ? nthreads : default_threads(32);
     if (G < 256 || nt <= 1) {
         work(0, G);
     } else {

@@ -12,6 +12,8 @@
 // The reference reads one stale byte past its zeroed buffer when a string
 // of length 0 mod 4 has no terminator (:120-125, :304); only real windows
 // are produced here.
+#include "host_threads.h"
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -108,9 +110,7 @@ struct Bag {
 
 template <class F>
 void parallel_rows(int64_t n, int nthreads, F&& f) {
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = nthreads > 0 ? nthreads : (int)(hw ? hw : 1);
-    if (nt > 64) nt = 64;
+    const int nt = nthreads > 0 ? std::min(nthreads, 64) : siml::default_threads(64);
     if (n < 4096 || nt <= 1) {
         f(0, n);
         return;

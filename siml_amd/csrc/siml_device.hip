@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "bitgroups.h"
+#include "host_threads.h"
 #include "groups.h"
 #include "siml_internal.h"
 
@@ -1848,10 +1849,11 @@ struct SetInfo {
     bool mags_small = true;
 };
 
+int host_threads() { return siml::default_threads(32); }
+
 template <class F>
 void parallel_for(int64_t n, F&& f) {
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = (int)std::min<unsigned>(hw ? hw : 1, 32);
+    const int nt = host_threads();
     if (n < 65536 || nt <= 1) {
         f(0, n);
         return;
@@ -2194,10 +2196,12 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     R.bits[1].reset();
     R.h_ids.resize((size_t)total);
     R.h_counts.resize((size_t)total);
-    for (int64_t i = 0; i < total; ++i) {
-        R.h_ids[(size_t)i] = el[(size_t)i].x;
-        R.h_counts[(size_t)i] = el[(size_t)i].y;
-    }
+    parallel_for(total, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i) {
+            R.h_ids[(size_t)i] = el[(size_t)i].x;
+            R.h_counts[(size_t)i] = el[(size_t)i].y;
+        }
+    });
     R.h_start.assign(off.begin(), off.end());
     R.h_mag = info.mag;
     R.h_csum = info.csum;
@@ -3540,14 +3544,8 @@ int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int
 // pitches go through a host staging buffer in slabs of rows: one linear
 // copy per slab, then a row-by-row host copy.  (A strided device-to-host
 // copy into pageable memory runs at a fraction of the linear rate.)
-// Host threads of the copy from pinned staging into the caller's memory:
-// OMP_NUM_THREADS when set (the process's CPU share), at most 16.
-static int copy_threads() {
-    const char* env = std::getenv("OMP_NUM_THREADS");
-    int n = env ? std::atoi(env) : 0;
-    if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
-    return std::min(n, 16);
-}
+// Host threads of the copy from pinned staging into the caller's memory.
+static int copy_threads() { return std::min(host_threads(), 16); }
 
 // Rows [r0, r0 + n) of a staged slab (row pitch dpitch) into host memory
 // (row pitch hpitch), cols elements each, split over host threads.
