@@ -141,9 +141,14 @@ def configure(g, args):
                                "merge": _lib.POLICY_MERGE}[args.policy])
 
 
-def base_record(args, metric, value, unit, wall, config, dtype="int32 intersections, float32 division"):
-    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True,
+def base_record(args, metric, value, unit, wall, config, dtype="int32 intersections, float32 division", steps=None,
+                warmup=None):
+    """`wall` is the timed region in seconds over `steps` steps (default
+    --steps)."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": wall * 1e3 / max(steps, 1), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic (seeded SMILES-like strings, packed with the native packer)", "config": config}
 
@@ -428,9 +433,8 @@ def workload_batched(args):
                       t_batched, {"workload": f"batched uneven query sets: {nsets} sets (geometric 1/128, clipped "
                                               f"[1, 4096], {nq} molecules) x {ndb}-molecule resident database, "
                                               f"top-{k}", "sets": nsets, "queries": nq, "database": ndb, "k": k,
-                                  "seed": 3})
+                                  "seed": 3}, steps=1, warmup=1)
     rec.update({
-        "steps": 1, "warmup": 1,
         "plain": {"wall_s": t_plain, "pairs_per_s": pairs / t_plain, "kernel_ms": kms_plain, "launches": n_plain},
         "batched": {"wall_s": t_batched, "pairs_per_s": pairs / t_batched, "kernel_ms": kms_b, "launches": n_b},
         "speedup_wall": t_plain / t_batched,
@@ -605,8 +609,8 @@ def workload_serve(args):
                       best["wall_s"], {"workload": f"serving: {ndb}-molecule resident database, Poisson arrivals "
                                                    f"of query batches (geometric 1/32 sizes clipped [1, 512]), "
                                                    f"half top-50, half threshold 0.8", "database": ndb,
-                                       "requests": nreq, "slots": args.slots, "seed": 5})
-    rec.update({"steps": 1, "warmup": 1, "rates": results, "best_rate_with_p99_below_50ms": best["rate_per_s"]
+                                       "requests": nreq, "slots": args.slots, "seed": 5}, steps=1, warmup=1)
+    rec.update({"rates": results, "best_rate_with_p99_below_50ms": best["rate_per_s"]
                 if ok else None})
     db.close()
     return rec

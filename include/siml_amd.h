@@ -165,7 +165,12 @@ int siml_query_done(siml_ctx* ctx, int* done);   /* asyncOperationsDone */
 int siml_synchronize(siml_ctx* ctx);
 int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int64_t* nnz);
 /* Copy the pending result to host buffers and clear it.  Matrix results use
- * the host row pitch `pitch`; CSR fills offsets (rows+1) and indices (nnz);
+ * the host row pitch `pitch`; when it equals the device row pitch
+ * (16 * ceil(cols / 16), as GPULingo.py:378-379 allocates) the rows are
+ * copied as one span, so elements [cols, pitch) of each row but the last
+ * are overwritten with unspecified values, as in the reference's padded
+ * result array; any other pitch leaves them untouched.
+ * CSR fills offsets (rows+1) and indices (nnz);
  * top-k fills idx and t with pitch k; I64 fills `cols` values. */
 int siml_retrieve_f32(siml_ctx* ctx, float* host, int64_t pitch);
 int siml_retrieve_i32(siml_ctx* ctx, int32_t* host, int64_t pitch);

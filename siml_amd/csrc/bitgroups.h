@@ -116,12 +116,24 @@ struct NoInitAlloc : std::allocator<T> {
     }
 };
 
+using BitEntryVec = std::vector<BitEntry, NoInitAlloc<BitEntry>>;
+
+// The groups' entries are the concatenation of `chunks` (chunk t starts at
+// entry chunk_off[t]; hdr[g].entry_off indexes the concatenation).  A
+// caller that builds into the same tables again reuses their memory.
 struct BitGroupTables {
     std::vector<BitGroupHdr> hdr;
     std::vector<int32_t> rows;        // kBitRowStride per group, -1 for unused
-    std::vector<BitEntry, NoInitAlloc<BitEntry>> entries;
+    std::vector<BitEntryVec> chunks;
+    std::vector<int64_t> chunk_off;   // chunks.size() + 1 offsets
     std::vector<int32_t> row_group;   // group of each row, -1 for merge rows
     std::vector<int32_t> merge_rows;  // rows served by the merge kernel, ascending
+    int64_t nentries() const { return chunk_off.empty() ? 0 : chunk_off.back(); }
+    const BitEntry& entry(int64_t i) const {
+        size_t t = 0;
+        while (i >= chunk_off[t + 1]) ++t;
+        return chunks[t][(size_t)(i - chunk_off[t])];
+    }
 };
 
 // Consecutive rows form groups; with by_mag the eligible rows are first

@@ -1851,6 +1851,34 @@ struct SetInfo {
 
 int host_threads() { return siml::default_threads(32); }
 
+// Host threads of copies between pinned staging and the caller's memory.
+int copy_threads() { return std::min(host_threads(), 16); }
+
+// parts[i] (pointer, bytes) copied to dst + at[i], split by bytes over up
+// to nt threads.
+void parallel_copy(char* dst, const std::vector<std::pair<const char*, size_t>>& parts, const std::vector<size_t>& at,
+                   int nt) {
+    size_t total = 0;
+    for (const auto& p : parts) total += p.second;
+    nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)nt, total >> 20));
+    auto work = [&](size_t lo, size_t hi) {  // bytes [lo, hi) of the concatenation
+        size_t base = 0;
+        for (size_t i = 0; i < parts.size() && base < hi; ++i) {
+            const size_t a = std::max(lo, base), b = std::min(hi, base + parts[i].second);
+            if (a < b) std::memcpy(dst + at[i] + (a - base), parts[i].first + (a - base), b - a);
+            base += parts[i].second;
+        }
+    };
+    if (nt == 1) {
+        work(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, total * t / nt, total * (t + 1) / nt);
+    work(0, total / nt);
+    for (auto& x : th) x.join();
+}
+
 template <class F>
 void parallel_for(int64_t n, F&& f) {
     const int nt = host_threads();
@@ -2069,7 +2097,8 @@ struct siml_ctx {
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
     std::vector<char> h_stage;       // staging of strided result copies
-    PinnedBuf pinned[2];             // double-buffered staging of large result copies
+    PinnedBuf pinned[2];             // double-buffered staging of large copies
+    siml::BitGroupTables bit_tables; // host tables of the last row-group build
 };
 
 namespace {
@@ -2136,21 +2165,91 @@ int collect_timing(siml_ctx* ctx) {
 
 // Builds bit-sliced grouping `which` of the reference set (0: consecutive
 // rows, 1: rows ordered by magnitude) unless it exists.
+// Host spans copied back to back into device memory at dev, on the handle's
+// stream; returns when the copy is complete.  Large copies go through the
+// two pinned slabs: host threads fill one slab while the other is in
+// flight (a copy from pageable memory runs at a fraction of the DMA rate).
+struct HostSpan {
+    const void* p;
+    size_t bytes;
+};
+
+int h2d_spans(siml_ctx* ctx, void* dev, const std::vector<HostSpan>& spans) {
+    size_t total = 0;
+    for (const HostSpan& s : spans) total += s.bytes;
+    char* d = static_cast<char*>(dev);
+    if (total < (size_t(32) << 20)) {
+        for (const HostSpan& s : spans) {
+            if (s.bytes) HIP_TRY(hipMemcpyAsync(d, s.p, s.bytes, hipMemcpyHostToDevice, ctx->stream));
+            d += s.bytes;
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return SIML_OK;
+    }
+    constexpr size_t kSlab = size_t(64) << 20;
+    TRY(ctx->pinned[0].reserve(kSlab));
+    TRY(ctx->pinned[1].reserve(kSlab));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    if (hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return fail(SIML_EDEVICE, "hipEventCreate failed");
+    }
+    const int nt = copy_threads();
+    size_t si = 0, soff = 0, done = 0;
+    hipError_t e = hipSuccess;
+    for (int64_t k = 0; done < total && e == hipSuccess; ++k) {
+        const int b = (int)(k % 2);
+        if (k >= 2) e = hipEventSynchronize(ev[b]);  // slab b's previous copy has landed
+        if (e != hipSuccess) break;
+        // gather up to kSlab bytes of the spans into slab b
+        char* dst = static_cast<char*>(ctx->pinned[b].p);
+        size_t fill = 0;
+        std::vector<std::pair<const char*, size_t>> parts;
+        std::vector<size_t> at;
+        while (fill < kSlab && si < spans.size()) {
+            const size_t take = std::min(kSlab - fill, spans[si].bytes - soff);
+            if (take) {
+                parts.emplace_back(static_cast<const char*>(spans[si].p) + soff, take);
+                at.push_back(fill);
+            }
+            fill += take;
+            soff += take;
+            if (soff == spans[si].bytes) {
+                ++si;
+                soff = 0;
+            }
+        }
+        parallel_copy(dst, parts, at, nt);
+        e = hipMemcpyAsync(d + done, dst, fill, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[b], ctx->stream);
+        done += fill;
+    }
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = es;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (e != hipSuccess) return fail(SIML_EDEVICE, "host-to-device copy failed: %s", hipGetErrorString(e));
+    return SIML_OK;
+}
+
 int ensure_bits(siml_ctx* ctx, int which) {
     RowSet& R = ctx->ref;
     RowSet::BitGrouping& B = R.bits[which];
     if (B.built) return SIML_OK;
     siml::BitGroupInput bin{R.h_ids.data(), R.h_counts.data(), R.h_start.data(), R.h_len.data(), R.h_mag.data(),
                             R.n};
-    siml::BitGroupTables bt;
+    siml::BitGroupTables& bt = ctx->bit_tables;  // reused: no fresh pages per build
     siml::build_bit_groups(bin, which == 1, bt, 0);
     TRY(upload(B.hdr, bt.hdr.data(), bt.hdr.size(), ctx->stream));
     TRY(upload(B.rows, bt.rows.data(), bt.rows.size(), ctx->stream));
-    TRY(upload(B.ent, reinterpret_cast<const uint4*>(bt.entries.data()), bt.entries.size(), ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    B.h_hdr = std::move(bt.hdr);
-    B.h_rows = std::move(bt.rows);
-    B.merge_rows = std::move(bt.merge_rows);
+    TRY(B.ent.reserve(sizeof(uint4) * (size_t)std::max<int64_t>(bt.nentries(), 1)));
+    std::vector<HostSpan> spans;
+    for (const siml::BitEntryVec& c : bt.chunks) spans.push_back({c.data(), sizeof(uint4) * c.size()});
+    TRY(h2d_spans(ctx, B.ent.p, spans));
+    B.h_hdr = bt.hdr;
+    B.h_rows = bt.rows;
+    B.merge_rows = bt.merge_rows;
     B.built = true;
     return SIML_OK;
 }
@@ -2175,7 +2274,8 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
             for (int32_t k = 0; k < info.len[(size_t)r]; ++k) d[k] = make_int2(id[k], ct[k]);
         }
     });
-    TRY(upload(R.elems, el.data(), (size_t)total, ctx->stream));
+    TRY(R.elems.reserve(sizeof(int2) * (size_t)std::max<int64_t>(total, 1)));
+    TRY(h2d_spans(ctx, R.elems.p, {HostSpan{el.data(), sizeof(int2) * (size_t)total}}));
     TRY(upload(R.off, off.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.len, info.len.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.mag, info.mag.data(), (size_t)s.n, ctx->stream));
@@ -3544,8 +3644,6 @@ int siml_result_info(siml_ctx* ctx, int* kind, int64_t* rows, int64_t* cols, int
 // pitches go through a host staging buffer in slabs of rows: one linear
 // copy per slab, then a row-by-row host copy.  (A strided device-to-host
 // copy into pageable memory runs at a fraction of the linear rate.)
-// Host threads of the copy from pinned staging into the caller's memory.
-static int copy_threads() { return std::min(host_threads(), 16); }
 
 // Rows [r0, r0 + n) of a staged slab (row pitch dpitch) into host memory
 // (row pitch hpitch), cols elements each, split over host threads.

@@ -6,12 +6,19 @@
 // candidate slots, key equality) and sums the row masks; each row's count
 // must equal the multiset intersection computed by a sorted merge.
 //
-// usage: bitgroups_replay SEED NROWS NCOLS NONASCII_PER_MILLE BIG_PER_MILLE
+// usage: bitgroups_replay SEED NROWS NCOLS NONASCII_PER_MILLE BIG_PER_MILLE [PLANT]
+//
+// With PLANT = 1, three ASCII ids whose level-1 keys share both candidate
+// slots are planted: a row holding two of them followed by a row holding
+// the third (the group's cuckoo insertion runs out of displacements and
+// the second row is given back to the next group), and a row holding all
+// three (it cannot be placed at all and goes to the merge kernel).
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <random>
+#include <unordered_map>
 #include <vector>
 
 #include "bitgroups.h"
@@ -44,11 +51,49 @@ static Set make_set(std::mt19937& rng, int n, int nonascii, int big) {
     return s;
 }
 
+// Three ASCII ids whose level-1 keys have the same two candidate slots.
+static std::vector<int32_t> colliding_ids() {
+    std::unordered_map<uint32_t, std::vector<int32_t>> by_pair;
+    for (int32_t b1 = 0x21; b1 < 0x7F; ++b1)
+        for (int32_t b2 = 0x21; b2 < 0x7F; ++b2)
+            for (int32_t b3 = 0x21; b3 < 0x7F; ++b3) {
+                const int32_t x = 0x43 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                uint32_t h1, h2;
+                key_slots(level_key(x, 1u), h1, h2);
+                std::vector<int32_t>& v = by_pair[std::min(h1, h2) * (uint32_t)kBitSlots + std::max(h1, h2)];
+                v.push_back(x);
+                if (v.size() == 3) return v;
+            }
+    return {};
+}
+
+static void add_row(Set& s, std::vector<int32_t> ids) {
+    std::sort(ids.begin(), ids.end());
+    for (int32_t x : ids) {
+        s.ids.push_back(x);
+        s.counts.push_back(1);
+    }
+    s.len.push_back((int32_t)ids.size());
+    s.start.push_back((int64_t)s.ids.size());
+}
+
 int main(int argc, char** argv) {
     if (argc < 6) return 2;
     std::mt19937 rng((unsigned)atoi(argv[1]));
-    const int nr = atoi(argv[2]), nc = atoi(argv[3]), nonascii = atoi(argv[4]), big = atoi(argv[5]);
+    int nr = atoi(argv[2]), nc = atoi(argv[3]);
+    const int nonascii = atoi(argv[4]), big = atoi(argv[5]), plant = argc > 6 ? atoi(argv[6]) : 0;
     Set R = make_set(rng, nr, nonascii, big), C = make_set(rng, nc, nonascii, big);
+    if (plant) {
+        const std::vector<int32_t> x = colliding_ids();
+        if (x.size() != 3) return 3;
+        add_row(R, {x[0], x[1], 0x01010101});
+        add_row(R, {x[2], 0x02010101});
+        add_row(R, {x[0], x[1], x[2]});
+        add_row(C, {x[0], x[1], x[2]});
+        add_row(C, {x[2], 0x02010101});
+        nr += 3;
+        nc += 2;
+    }
     std::vector<int32_t> mag(nr);
     for (int r = 0; r < nr; ++r)
         for (int64_t j = R.start[r]; j < R.start[r + 1]; ++j) mag[r] += R.counts[j];
@@ -73,7 +118,7 @@ int main(int argc, char** argv) {
             grouped_rows += h.nrows;
             std::vector<uint32_t> key(kBitSlots, 0), mask(kBitSlots, 0);
             for (int k = 0; k < h.nkeys; ++k) {
-                const BitEntry& e = bt.entries[(size_t)(h.entry_off + k)];
+                const BitEntry& e = bt.entry(h.entry_off + k);
                 if (e.slot < 0 || e.slot >= kBitSlots || key[e.slot] != 0 || e.key == 0) return 1;
                 uint32_t s1, s2;
                 key_slots(e.key, s1, s2);
@@ -110,6 +155,18 @@ int main(int argc, char** argv) {
                     ++checked;
                     if (isct != cnt[r]) ++bad;
                 }
+            }
+        }
+        if (plant) {
+            // the row with all three colliding keys cannot be placed
+            if (std::find(bt.merge_rows.begin(), bt.merge_rows.end(), nr - 1) == bt.merge_rows.end()) {
+                printf("row with three colliding keys is not a merge row\n");
+                return 1;
+            }
+            if (bt.row_group[nr - 3] < 0 || bt.row_group[nr - 2] < 0 ||
+                bt.row_group[nr - 3] == bt.row_group[nr - 2]) {
+                printf("colliding rows share a group\n");
+                return 1;
             }
         }
     }

@@ -22,14 +22,16 @@ def replay(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("seed,nrows,ncols,nonascii,big", [
-    (1, 300, 200, 0, 0),        # every row groups
-    (2, 700, 150, 2, 2),        # some non-ASCII ids and counts above 15 -> merge rows
-    (3, 64, 300, 0, 30),        # many repeated Lingos
-    (4, 1000, 60, 0, 0),        # several groups per chunk of the parallel builder
+@pytest.mark.parametrize("seed,nrows,ncols,nonascii,big,plant", [
+    (1, 300, 200, 0, 0, 0),        # every row groups
+    (2, 700, 150, 2, 2, 0),        # some non-ASCII ids and counts above 15 -> merge rows
+    (3, 64, 300, 0, 30, 0),        # many repeated Lingos
+    (4, 1000, 60, 0, 0, 0),        # several groups per chunk of the parallel builder
+    (5, 300, 200, 0, 0, 1),        # keys sharing both slots: give-back and merge-row paths
+    (6, 5000, 40, 0, 0, 1),
 ])
-def test_group_tables_reproduce_intersections(replay, seed, nrows, ncols, nonascii, big):
-    res = subprocess.run([replay, str(seed), str(nrows), str(ncols), str(nonascii), str(big)],
+def test_group_tables_reproduce_intersections(replay, seed, nrows, ncols, nonascii, big, plant):
+    res = subprocess.run([replay, str(seed), str(nrows), str(ncols), str(nonascii), str(big), str(plant)],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stdout + res.stderr
     assert " 0 differ" in res.stdout

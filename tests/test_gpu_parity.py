@@ -542,3 +542,64 @@ def test_strided_result_copies(demo_lines):
     h.retrieve_f32(out, 1020)
     assert_same_f32(out[:, :1001], want[100:140])
     assert np.all(out[:, 1001:] == -1.0)
+
+
+def _key_slots(keys):
+    """The two candidate table slots of level keys (the row-group hash of
+    siml_amd/csrc/bitgroups.h), vectorised over a uint32 array."""
+    h = keys.astype(np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    h = (h * np.uint64(0x9E3779B1)) & m
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x85EBCA77)) & m
+    h ^= h >> np.uint64(13)
+    h = (h * np.uint64(0xC2B2AE3D)) & m
+    h ^= h >> np.uint64(16)
+    h1 = h >> np.uint64(20)
+    h2 = (h >> np.uint64(4)) & np.uint64(4095)
+    h2 = np.where(h2 == h1, h2 ^ np.uint64(1), h2)
+    return h1, h2
+
+
+def _colliding_ids():
+    """Three ASCII Lingo ids whose level-1 keys (id | 0x80) share both slots."""
+    b = np.arange(0x21, 0x7F, dtype=np.int64)
+    x = (0x43 | (b[:, None, None] << 8) | (b[None, :, None] << 16) | (b[None, None, :] << 24)).reshape(-1)
+    h1, h2 = _key_slots((x | 0x80).astype(np.uint32))
+    pair = np.minimum(h1, h2) * np.uint64(4096) + np.maximum(h1, h2)
+    order = np.argsort(pair, kind="stable")
+    ps = pair[order]
+    first = np.nonzero(ps[2:] == ps[:-2])[0][0]
+    return [int(v) for v in sorted(x[order[first:first + 3]])]
+
+
+def test_colliding_level_keys(demo_lines):
+    """Rows whose level keys share both table slots: a group whose insertion
+    runs out of displacements gives rows back, and a row that cannot be
+    placed at all goes to the merge kernel; every pair stays bit-exact."""
+    a, b, c = _colliding_ids()
+    ids, counts, offs, mags = synth.packed_ragged(300, 20, 80, seed=41)
+    extra_rows = [[a, b, 0x01010101], [c, 0x02010101], [a, b, c]]
+    extra_cols = [[a, b, c], [c, 0x02010101], [a]]
+
+    def with_rows(rows):
+        I, C, O = [ids], [counts], [offs]
+        end = int(offs[-1])
+        for r in rows:
+            r = sorted(r)
+            I.append(np.array(r, np.int32))
+            C.append(np.ones(len(r), np.int32))
+            end += len(r)
+            O.append(np.array([end], np.int64))
+        i, cnt, o = np.concatenate(I), np.concatenate(C), np.concatenate(O)
+        m = np.add.reduceat(cnt, o[:-1]).astype(np.int32)
+        return synth.ragged_to_padded(i, cnt, o, m)
+
+    ref, qry = with_rows(extra_rows), with_rows(extra_cols)
+    g = GPULingo(0)
+    _set(g, ref, qry)
+    got = g.getMultipleRows(0, ref[0].shape[0])
+    want = oracle.tanimoto_block(ref, qry, nthreads=0)
+    assert_same_f32(got, want)
+    # the planted columns meet the planted rows
+    assert want[-1, -3] == np.float32(1.0) and want[-2, -2] == np.float32(1.0)
