@@ -86,6 +86,7 @@ class GPULingo:
         self.qshape = None
         self._h = _lib.Handle(deviceID)
         self._retrieve = None
+        self._pending_kind = None
 
     # -- set-up ------------------------------------------------------------
     def set_refsmiles(self, refsmilesmat, refcountsmat, reflengths, refmags=None):
@@ -156,8 +157,9 @@ class GPULingo:
             raise ValueError(f"rows [{rowbase}, {rowlimit}) outside [0, {self.nref})")
         return rowbase, rowlimit
 
-    def _pend(self, fn):
+    def _pend(self, fn, kind="matrix"):
         self._retrieve = fn
+        self._pending_kind = kind
 
     # -- similarity rows -----------------------------------------------------
     def getTanimotoRow(self, rowidx):
@@ -195,7 +197,7 @@ class GPULingo:
         (rows, 101), bin (unsigned)(T*100)."""
         rowbase, rowlimit = self._rows(rowbase, rowlimit)
         self._h.call("siml_hist_rows_async", rowbase, rowlimit)
-        self._pend(self._retrieve_hist)
+        self._pend(self._retrieve_hist, "histogram")
 
     def getNeighbors(self, rowbase, rowlimit, lowerbound, upperbound=1.1, maxneighbors=None):
         """GPULingo.py:516-535."""
@@ -212,7 +214,7 @@ class GPULingo:
         pitch = 16 * ((int(maxneighbors) + 15) // 16)
         self._h.call("siml_neighbors_async", rowbase, rowlimit, float(np.float32(lowerbound)),
                      float(np.float32(upperbound)))
-        self._pend(lambda: self._retrieve_neighbors(pitch))
+        self._pend(lambda trimmed=True: self._retrieve_neighbors(pitch, maxneighbors, trimmed), "neighbors")
 
     # -- additive -------------------------------------------------------------
     def getNeighborsCSR(self, rowbase, rowlimit, lowerbound, upperbound=1.1):
@@ -254,11 +256,14 @@ class GPULingo:
         self._h.retrieve_i32(out, 112)
         return out[:, :101]
 
-    def _retrieve_neighbors(self, pitch):
+    def _retrieve_neighbors(self, pitch, maxneighbors, trimmed):
         offs, idx = self._h.retrieve_csr()
         counts = reference_neighbor_counts(offs, idx, pitch)
         rows = len(offs) - 1
-        width = int(min(counts.max(), pitch)) if rows else 0
+        if trimmed:
+            width = int(min(counts.max(), pitch)) if rows else 0
+        else:
+            width = int(maxneighbors)
         mat = np.full((rows, width), -1, dtype=np.int32)
         for r in range(rows):
             lst = idx[offs[r]:offs[r + 1]][:width]
@@ -272,6 +277,37 @@ class GPULingo:
         if fn is None:
             raise RuntimeError("Error: tried to call retrieveAsyncResult without a pending asynchronous result")
         return fn()
+
+    def _retrieve_kind(self, kinds, what):
+        if self._retrieve is not None and self._pending_kind not in kinds:
+            raise ValueError(f"{what}: the pending result is a {self._pending_kind} result")
+
+    def retrieveAsyncMatrixResult(self):
+        """GPULingo.py:538-556: the pending single-row, multi-row or
+        histogrammed-row matrix."""
+        self._retrieve_kind(("matrix", "histogram"), "retrieveAsyncMatrixResult")
+        return self.retrieveAsyncResult()
+
+    def retrieveAsyncMatrixVectorResult(self):
+        """GPULingo.py:559-584: the pending neighbour search as (matrix with
+        maxneighbors columns, counts); entries past a row's count are -1."""
+        self._retrieve_kind(("neighbors",), "retrieveAsyncMatrixVectorResult")
+        fn, self._retrieve = self._retrieve, None
+        if fn is None:
+            raise RuntimeError("Error: tried to call retrieveAsyncResult without a pending asynchronous result")
+        return fn(trimmed=False)
+
+    def retrieveAsyncMatrixVectorResult_trimmed(self):
+        """GPULingo.py:586-623: the pending neighbour search as (matrix with
+        max(counts) columns, counts) -- what retrieveAsyncResult returns after
+        getNeighbors_async (GPULingo.py:513)."""
+        self._retrieve_kind(("neighbors",), "retrieveAsyncMatrixVectorResult_trimmed")
+        return self.retrieveAsyncResult()
+
+    def retrieveAsyncResultBase(self):
+        """GPULingo.py:625-628: the retrieve of an object with no pending
+        result."""
+        raise RuntimeError("Error: tried to call retrieveAsyncResult without a pending asynchronous result")
 
     def asyncOperationsDone(self):
         """True once all work queued on this object has completed

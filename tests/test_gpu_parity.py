@@ -603,3 +603,33 @@ def test_colliding_level_keys(demo_lines):
     assert_same_f32(got, want)
     # the planted columns meet the planted rows
     assert want[-1, -3] == np.float32(1.0) and want[-2, -2] == np.float32(1.0)
+
+
+def test_named_retrieve_methods(gpu, demo_set, demo_lines):
+    """GPULingo.py:538-628: the retrieve methods the reference binds per
+    operation, called by name."""
+    ref = oracle.pack(demo_lines[:32])
+    _set(gpu, ref, demo_set)
+    gpu.getMultipleRows_async(0, 32)
+    with pytest.raises(ValueError):
+        gpu.retrieveAsyncMatrixVectorResult()
+    m = gpu.retrieveAsyncMatrixResult()
+    assert_same_f32(m, oracle.tanimoto_block(ref, demo_set, nthreads=0))
+    gpu.getMultipleHistogrammedRows_async(0, 32)
+    h = gpu.retrieveAsyncMatrixResult()
+    assert h.shape == (32, 101) and np.all(h.sum(axis=1) == 4096)
+    lists, _ = oracle.neighbors(ref, demo_set, 0.3)
+    gpu.getNeighbors_async(0, 32, 0.3, maxneighbors=40)
+    full, cnt = gpu.retrieveAsyncMatrixVectorResult()
+    assert full.shape == (32, 40)
+    gpu.getNeighbors_async(0, 32, 0.3, maxneighbors=40)
+    trimmed, cnt2 = gpu.retrieveAsyncMatrixVectorResult_trimmed()
+    assert np.array_equal(cnt, cnt2) and trimmed.shape[1] == min(int(cnt.max()), 48)
+    for r in range(32):
+        k = min(int(cnt[r]), 40)
+        assert np.array_equal(full[r, :k], lists[r][:k])
+        assert np.array_equal(trimmed[r, :min(k, trimmed.shape[1])], lists[r][:min(k, trimmed.shape[1])])
+    with pytest.raises(RuntimeError):
+        gpu.retrieveAsyncResultBase()
+    with pytest.raises(RuntimeError):
+        gpu.retrieveAsyncMatrixResult()
