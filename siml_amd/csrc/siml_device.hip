@@ -775,25 +775,52 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
     __shared__ uint32_t hist[256];
     __shared__ int32_t s_base[kTopkFan + 1];
     __shared__ uint64_t s_prefix, s_pmask;
-    __shared__ uint32_t s_need, s_fill;
-    if (threadIdx.x == 0) {
-        int32_t tot = 0;
-        for (int w = s0; w < s1; ++w) {
-            s_base[w - s0] = tot;
-            tot += cnt[w];
+    __shared__ unsigned long long s_min;
+    __shared__ uint32_t s_need, s_fill, s_exact;
+    const int nseg = s1 - s0;  // <= kTopkFan < kWave
+    if (threadIdx.x < kWave) {
+        // segment offsets: one wave prefix sum over the counts
+        const int lane = threadIdx.x;
+        const int n = lane < nseg ? cnt[s0 + lane] : 0;
+        int S = n;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int t = __shfl_up(S, off);
+            if (lane >= off) S += t;
         }
-        s_base[s1 - s0] = tot;
-        s_prefix = 0ull;
-        s_pmask = 0ull;
-        s_need = (uint32_t)k;
-        s_fill = 0u;
+        if (lane <= nseg) s_base[lane] = S - n;
+        if (lane == 0) {
+            s_prefix = 0ull;
+            s_pmask = 0ull;
+            s_need = (uint32_t)k;
+            s_fill = 0u;
+        }
     }
     __syncthreads();
-    const uint32_t total = (uint32_t)s_base[s1 - s0];
-    for (int w = s0; w < s1; ++w) {
-        const int n = s_base[w - s0 + 1] - s_base[w - s0];
-        uint64_t* d = cand + s_base[w - s0];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = buf[(int64_t)w * cap + i];
+    const uint32_t total = (uint32_t)s_base[nseg];
+    // keys to LDS: every element load is independent of the others, and a
+    // thread has four in flight
+    constexpr int kUnroll = 4;
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kUnroll * blockDim.x) {
+        uint64_t v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t i = i0 + u * blockDim.x;
+            if (i >= total) break;
+            int lo = 0, hi = nseg - 1;  // the segment w with s_base[w] <= i < s_base[w + 1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((uint32_t)s_base[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            v[u] = buf[(int64_t)(s0 + lo) * cap + (i - (uint32_t)s_base[lo])];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t i = i0 + u * blockDim.x;
+            if (i < total) cand[i] = v[u];
+        }
     }
     __syncthreads();
     uint64_t K = 0ull;  // every key is > 0
@@ -804,7 +831,20 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
             const uint64_t prefix = s_prefix, pmask = s_pmask;
             for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
                 const uint64_t key = cand[i];
-                if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+                const bool act = (key & pmask) == prefix;
+                const uint32_t dig = (uint32_t)(key >> shift) & 255u;
+                // the high digits of most keys are equal: when every active
+                // lane has the same digit, one atomic adds them all (equal-
+                // address atomics of one instruction serialise)
+                const uint64_t am = __ballot(act);
+                if (am == 0ull) continue;
+                const int leader = __builtin_ctzll(am);
+                const uint32_t d0 = (uint32_t)__shfl((int)dig, leader);
+                if (__ballot(act && dig == d0) == am) {
+                    if (lane_id() == leader) atomicAdd(&hist[d0], (uint32_t)__popcll(am));
+                } else if (act) {
+                    atomicAdd(&hist[dig], 1u);
+                }
             }
             __syncthreads();
             if (threadIdx.x < kWave) {
@@ -825,9 +865,11 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
                 const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
                                         (uint32_t)__shfl(h3, L)};
                 int d = 4 * L;
+                uint32_t cnt_d = hb[0];
                 for (int q = 3; q >= 0; --q) {
                     if (above + hb[q] >= need) {
                         d = 4 * L + q;
+                        cnt_d = hb[q];
                         break;
                     }
                     above += hb[q];
@@ -836,11 +878,26 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
                     s_need = need - above;
                     s_prefix = prefix | ((uint64_t)d << shift);
                     s_pmask = pmask | ((uint64_t)255 << shift);
+                    s_exact = cnt_d == need - above ? 1u : 0u;
+                    s_min = ~0ull;
                 }
             }
             __syncthreads();
+            if (s_exact) {
+                // every key of the chosen bucket is needed: the k-th largest
+                // is the bucket's smallest key
+                const uint64_t bp = s_prefix, bm = s_pmask;
+                uint64_t mn = ~0ull;
+                for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+                    const uint64_t key = cand[i];
+                    if ((key & bm) == bp && key < mn) mn = key;
+                }
+                atomicMin(&s_min, (unsigned long long)mn);
+                __syncthreads();
+                break;
+            }
         }
-        K = s_prefix;
+        K = s_exact ? (uint64_t)s_min : s_prefix;
     }
     if (kth) {
         // only the k-th largest key is wanted (0 when there are fewer than k)
