@@ -759,10 +759,14 @@ __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int
 // Final top-k of each row from its nw candidate buffers: a workgroup radix
 // select of the k-th best key, then a bitonic sort of the (at most 256)
 // keys at or above it.  Output rows are padded with (-1, -inf).
-// Candidate buffers one selection workgroup takes (its keys are staged in
-// LDS: kTopkFan x (SIML_MAX_TOPK + 64) keys = 40 KiB).
+// Candidate buffers one selection workgroup takes (their keys are staged in
+// LDS).
 constexpr int kTopkFan = 16;
-constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kTopkSlack);
+
+// Dynamic LDS of a selection launch whose segments hold at most segcap
+// keys (at most 16 x 384 keys = 48 KiB, for k = 256): sized to the launch,
+// so more selection workgroups fit per CU for small k.
+inline size_t topk_lds(int segcap) { return sizeof(uint64_t) * (size_t)kTopkFan * (size_t)segcap; }
 
 // Workgroup radix select over the candidate segments [s0, s1) of one row
 // (segment w holds cnt[w] keys at buf + w * cap, s1 - s0 <= kTopkFan): the
@@ -770,8 +774,8 @@ constexpr int kTopkLdsKeys = kTopkFan * (SIML_MAX_TOPK + kTopkSlack);
 // k-th largest, and `sorted` receives the min(k, total) largest keys
 // (unordered, rest 0).  Returns their number.
 __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32_t* __restrict__ cnt, int s0,
-                                   int s1, int cap, int k, uint64_t* sorted, uint64_t* kth = nullptr) {
-    __shared__ uint64_t cand[kTopkLdsKeys];
+                                   int s1, int cap, int k, uint64_t* sorted, uint64_t* cand,
+                                   uint64_t* kth = nullptr) {
     __shared__ uint32_t hist[256];
     __shared__ int32_t s_base[kTopkFan + 1];
     __shared__ uint64_t s_prefix, s_pmask;
@@ -932,11 +936,13 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
 __global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restrict__ buf,
                                                        const int32_t* __restrict__ bufcnt, int nw, int cap, int k,
                                                        uint64_t* __restrict__ out, int32_t* __restrict__ outcnt) {
+    extern __shared__ uint64_t topk_cand[];  // kTopkFan x cap keys (launch's dynamic LDS)
     __shared__ uint64_t sorted[kBlock];
     const int64_t r = blockIdx.x;
     const int s = blockIdx.y, ns = gridDim.y;
     const int s0 = s * kTopkFan, s1 = min(nw, s0 + kTopkFan);
-    const uint32_t have = wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, s0, s1, cap, k, sorted);
+    const uint32_t have = wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, s0, s1, cap, k, sorted,
+                                         topk_cand);
     uint64_t* o = out + (r * ns + s) * (int64_t)k;
     for (int i = threadIdx.x; i < (int)have; i += blockDim.x) o[i] = sorted[i];
     if (threadIdx.x == 0) outcnt[r * ns + s] = (int32_t)have;
@@ -953,10 +959,11 @@ __global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restric
 __global__ __launch_bounds__(kBlock) void k_topk_kth(const uint64_t* __restrict__ buf,
                                                      const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
                                                      uint64_t* __restrict__ seed) {
+    extern __shared__ uint64_t topk_cand[];  // kTopkFan x cap keys (launch's dynamic LDS)
     __shared__ uint64_t sorted[kBlock];
     const int64_t r = blockIdx.x;
     uint64_t K = 0ull;
-    wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, 0, nw, cap, k, sorted, &K);
+    wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, 0, nw, cap, k, sorted, topk_cand, &K);
     if (threadIdx.x == 0) seed[r] = K;
 }
 
@@ -964,11 +971,12 @@ __global__ __launch_bounds__(kBlock) void k_topk_final(const uint64_t* __restric
                                                        const int32_t* __restrict__ bufcnt, int nw, int k, int cap,
                                                        int zero_fill, int64_t ncols, int32_t* __restrict__ out_idx,
                                                        float* __restrict__ out_t) {
+    extern __shared__ uint64_t topk_cand[];  // kTopkFan x cap keys (launch's dynamic LDS)
     __shared__ uint64_t sorted[kBlock];
     __shared__ uint32_t poscol[kBlock];
     const int64_t rloc = blockIdx.x;
     const uint32_t have =
-        wg_select_topk(buf + rloc * nw * (int64_t)cap, bufcnt + rloc * nw, 0, nw, cap, k, sorted);
+        wg_select_topk(buf + rloc * nw * (int64_t)cap, bufcnt + rloc * nw, 0, nw, cap, k, sorted, topk_cand);
     // bitonic sort, descending, of kBlock keys (k <= kBlock)
     for (int size = 2; size <= kBlock; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -3387,7 +3395,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
             const int64_t ns = (nseg + kTopkFan - 1) / kTopkFan;
             uint64_t* dst = level % 2 ? d_sbuf2 : d_sbuf;
             int32_t* dstcnt = level % 2 ? d_scnt2 : d_scnt;
-            k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), 0, ctx->stream>>>(
+            k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), topk_lds(segcap), ctx->stream>>>(
                 src, srccnt, (int)nseg, segcap, k, dst, dstcnt);
             TRY(launch_check("k_topk_stage"));
             src = dst;
@@ -3396,10 +3404,11 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
             segcap = k;
         }
         if (kth) {
-            k_topk_kth<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap, kth);
+            k_topk_kth<<<dim3((unsigned)nr), dim3(kBlock), topk_lds(segcap), ctx->stream>>>(src, srccnt, (int)nseg,
+                                                                                            k, segcap, kth);
             return launch_check("k_topk_kth");
         }
-        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), 0, ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
+        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), topk_lds(segcap), ctx->stream>>>(src, srccnt, (int)nseg, k, segcap,
                                                                          pos_only ? 1 : 0, nq, out_idx, out_t);
         return launch_check("k_topk_final");
     };
