@@ -74,9 +74,10 @@ class Server:
 
     Each slot is a handle with its own HIP stream, so the kernels of
     different slots run concurrently.  Two slots (the default) measured the
-    lowest p99 on the 10M-molecule database: one slot leaves the device idle
-    between batches, four let large batches share the device with three
-    others and stretch the tail.
+    lowest p99 on the 10M-molecule database up to 2000 batches/s: one slot
+    leaves the device idle between batches, four let large batches share
+    the device with three others and stretch the tail; three gain only
+    near saturation (2500 batches/s).
     """
 
     def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False):
