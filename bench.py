@@ -629,7 +629,7 @@ def main():
     ap.add_argument("--db", type=int, default=0, help="database molecules (batched/threshold/serve)")
     ap.add_argument("--sets", type=int, default=2000, help="batched: query sets")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per arrival rate")
-    ap.add_argument("--rates", default="30,100,300,1000", help="serve: Poisson arrival rates, batches/s")
+    ap.add_argument("--rates", default="30,300,1000,2000,2500", help="serve: Poisson arrival rates, batches/s")
     ap.add_argument("--slots", type=int, default=2, help="serve: concurrent comparators")
     ap.add_argument("--max-rows", type=int, default=1024, help="serve: query molecules per launch")
     ap.add_argument("--mem-budget", type=int, default=16, help="serve: admission budget, GiB")
