@@ -137,8 +137,7 @@ def configure(g, args):
     h = g._h
     h.call("siml_set_layout", {"natural": _lib.LAYOUT_NATURAL, "block": _lib.LAYOUT_BLOCK,
                                "sorted": _lib.LAYOUT_SORTED}[args.layout])
-    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "hash8": _lib.POLICY_HASH8,
-                               "merge": _lib.POLICY_MERGE}[args.policy])
+    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "merge": _lib.POLICY_MERGE}[args.policy])
 
 
 def base_record(args, metric, value, unit, wall, config, dtype="int32 intersections, float32 division", steps=None,
@@ -639,9 +638,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--traffic", type=float, default=None,
                     help="all-pairs: HBM bytes per launch from counter runs, reported as roofline.traffic")
-    ap.add_argument("--policy", choices=["auto", "hash8", "merge"], default="auto",
-                    help="auto: bit-sliced 32-row kernels; hash8: 8-row hashed kernels; "
-                         "merge: one-row-per-workgroup merge kernel")
+    ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
+                    help="auto: bit-sliced 32-row kernels; merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural", "sorted"], default="block",
                     help="query storage order: size-sorted 256-column blocks, the caller's order, or fully "
                          "size-sorted")

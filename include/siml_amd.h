@@ -196,14 +196,12 @@ int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_
 int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset);
 /* Kernel policy: AUTO serves sorted-unique sets with the bit-sliced
  * multi-row kernels (rows that do not qualify, and unsorted sets, use the
- * merge kernel); HASH8 uses the 8-row hashed kernels instead of the
- * bit-sliced ones; MERGE uses the merge kernel for everything.  All give the
- * same bits.  The multi-row tables are built by set_ref for the policy in
- * force then; a reference set given under another policy is served by the
+ * merge kernel); MERGE uses the merge kernel for everything.  Both give the
+ * same bits.  The bit-sliced row groups are built by set_ref when the policy
+ * in force then is AUTO; a reference set given under MERGE is served by the
  * merge kernel. */
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
-#define SIML_POLICY_HASH8 2
 int siml_set_policy(siml_ctx* ctx, int policy);
 /* Storage order used by the next set_query: NATURAL keeps the caller's
  * order; BLOCK (default) sorts the query molecules by size inside blocks

@@ -33,8 +33,7 @@ class CPULingo:
         self.qmags = None
         self.lastasyncresult = None
         self._h = None
-        self._ref_key = None
-        self._qry_key = None
+        self._snap = {}   # set name -> contents last uploaded
 
     def supportsParallel(self):
         """CPULingo.py:61-68; the device path is always parallel."""
@@ -48,7 +47,6 @@ class CPULingo:
         self.rcounts = refcountsmat
         self.rlengths = reflengths
         self.rmags = refmags if refmags is not None else np.int32(reflengths.sum(1))
-        self._ref_key = None
 
     def set_qsmiles(self, qsmilesmat, qcountsmat, querylengths, querymags=None):
         """CPULingo.py:86-101."""
@@ -56,7 +54,6 @@ class CPULingo:
         self.qcounts = qcountsmat
         self.qlengths = querylengths
         self.qmags = querymags if querymags is not None else np.int32(querylengths.sum(1))
-        self._qry_key = None
 
     def _checkInitialized(self):
         """CPULingo.py:103-108."""
@@ -69,6 +66,22 @@ class CPULingo:
             self._h = _lib.Handle(int(os.environ.get("SIML_DEVICE", "0")))
         return self._h
 
+    def _sync(self, which):
+        """Uploads set `which` ("ref" or "query") unless the device already
+        holds exactly its current contents.  The reference reads the arrays
+        on every call (CPULingo.py:126-138), so an in-place edit after
+        set_refsmiles/set_qsmiles is seen by the next call here too."""
+        if which == "ref":
+            arrays, label = (self.rlingos, self.rcounts, self.rlengths, self.rmags), "Reference"
+        else:
+            arrays, label = (self.qlingos, self.qcounts, self.qlengths, self.qmags), "Query"
+        cur = normalise_set(*arrays, label)
+        snap = self._snap.get(which)
+        if snap is not None and all(a.shape == b.shape and np.array_equal(a, b) for a, b in zip(snap, cur)):
+            return
+        self._handle().set_padded(which, *cur)
+        self._snap[which] = tuple(np.array(a, copy=True) for a in cur)
+
     def getMultipleRows_async(self, rowbase, rowlimit, nprocs=1):
         """CPULingo.py:111-141: rows self.rlingos[rowbase:rowlimit] (Python
         slice semantics) against the whole query set; synchronous."""
@@ -77,14 +90,8 @@ class CPULingo:
         nref = np.asarray(self.rlingos).shape[0]
         start, stop, _ = slice(rowbase, rowlimit).indices(nref)
         stop = max(stop, start)
-        if self._ref_key is None:
-            l, c, n, m = normalise_set(self.rlingos, self.rcounts, self.rlengths, self.rmags, "Reference")
-            h.set_padded("ref", l, c, n, m)
-            self._ref_key = True
-        if self._qry_key is None:
-            l, c, n, m = normalise_set(self.qlingos, self.qcounts, self.qlengths, self.qmags, "Query")
-            h.set_padded("query", l, c, n, m)
-            self._qry_key = True
+        self._sync("ref")
+        self._sync("query")
         h.call("siml_multirow_async", start, stop)
         kind, rows, cols, _ = h.result_info()
         out = np.empty((rows, cols), dtype=np.float32)

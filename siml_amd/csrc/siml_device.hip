@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
@@ -32,7 +33,6 @@
 
 #include "bitgroups.h"
 #include "host_threads.h"
-#include "groups.h"
 #include "siml_internal.h"
 
 namespace siml {
@@ -252,247 +252,6 @@ __global__ __launch_bounds__(kBlock) void k_rows_f32(RowArgs R, ColArgs C, const
 }
 
 // ---------------------------------------------------------------------------
-// Hashed multi-row kernels (row groups, see groups.h)
-// ---------------------------------------------------------------------------
-
-typedef unsigned short us2 __attribute__((ext_vector_type(2)));
-
-struct GroupArgs {
-    const siml::GroupHdr* hdr;
-    const int2* ekey;   // (id, slot) per entry
-    const uint4* ecnt;  // eight u16 counts per entry
-};
-
-// acc += min(cv, cb2) on two u16 lanes (v_pk_min_u16 + v_pk_add_u16).
-__device__ __forceinline__ uint32_t pk_minadd(uint32_t acc, uint32_t cv, uint32_t cb2) {
-    us2 a = __builtin_bit_cast(us2, acc);
-    a += __builtin_elementwise_min(__builtin_bit_cast(us2, cv), __builtin_bit_cast(us2, cb2));
-    return __builtin_bit_cast(uint32_t, a);
-}
-
-struct GroupLds {
-    int32_t keys[siml::kSlots];
-    uint4 cnt[siml::kSlots];
-};
-
-// Fills the group's cuckoo table into LDS: empty slots hold empty_key with
-// all-zero counts (slot 0 always stays empty).
-__device__ __forceinline__ void stage_group(GroupLds& T, const GroupArgs& G, const siml::GroupHdr& hd) {
-    for (int s = threadIdx.x; s < siml::kSlots; s += blockDim.x) {
-        T.keys[s] = hd.empty_key;
-        T.cnt[s] = make_uint4(0u, 0u, 0u, 0u);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
-        const int2 e = G.ekey[hd.entry_off + k];
-        T.keys[e.y] = e.x;
-        T.cnt[e.y] = G.ecnt[hd.entry_off + k];
-    }
-    __syncthreads();
-}
-
-// Intersections of one query molecule (a lane) with all rows of the group:
-// acc[w] holds rows 2w (low u16) and 2w+1 (high u16).  The loop runs over
-// the tile's uniform step count; padding entries carry count 0 and add 0.
-// One column entry against the group's table: both cuckoo slots are read
-// unconditionally and the matching one (or the all-zero slot 0) selected
-// with masks, so the probe has no branch.
-struct HashParams {
-    uint32_t a1, b1, a2, b2;
-};
-
-// umul24 as in groups.h.  __umul24 returns a signed int: the sum is formed
-// unsigned, so the shifted slot is always in [0, kSlots).
-__device__ __forceinline__ uint32_t umul24u(uint32_t u, uint32_t v) { return (uint32_t)__umul24(u, v); }
-
-__device__ __forceinline__ void group_probe(const GroupLds& T, const HashParams& hp, int2 b, uint32_t acc[4]) {
-    const uint32_t x = (uint32_t)b.x;
-    const uint32_t xh = x >> 8;
-    const uint32_t h1 = (umul24u(x, hp.a1) + umul24u(xh, hp.b1)) >> (32 - siml::kSlotBits);
-    const uint32_t h2 = (umul24u(x, hp.a2) + umul24u(xh, hp.b2)) >> (32 - siml::kSlotBits);
-    const int32_t k1 = T.keys[h1];
-    const int32_t k2 = T.keys[h2];
-    const uint32_t m1 = 0u - (uint32_t)(k1 == b.x);
-    const uint32_t m2 = (0u - (uint32_t)(k2 == b.x)) & ~m1;
-    const uint4 cv = T.cnt[(h1 & m1) | (h2 & m2)];
-    uint32_t cb = min((uint32_t)b.y, siml::kCountMax);
-    cb |= cb << 16;
-    acc[0] = pk_minadd(acc[0], cv.x, cb);
-    acc[1] = pk_minadd(acc[1], cv.y, cb);
-    acc[2] = pk_minadd(acc[2], cv.z, cb);
-    acc[3] = pk_minadd(acc[3], cv.w, cb);
-}
-
-__device__ __forceinline__ void group_isct(const GroupLds& T, const siml::GroupHdr& hd, const int2* __restrict__ p,
-                                           int Lt, uint32_t acc[4]) {
-    acc[0] = acc[1] = acc[2] = acc[3] = 0u;
-    const HashParams hp{hd.a1, hd.b1, hd.a2, hd.b2};
-    int j = 0;
-    if (Lt >= 8) {
-        // loads run four steps ahead of the probes
-        int2 b0 = p[0], b1 = p[kWave], b2 = p[2 * kWave], b3 = p[3 * kWave];
-        for (; j + 8 <= Lt; j += 4) {
-            const int2* q = p + (size_t)(j + 4) * kWave;
-            const int2 n0 = q[0], n1 = q[kWave], n2 = q[2 * kWave], n3 = q[3 * kWave];
-            group_probe(T, hp, b0, acc);
-            group_probe(T, hp, b1, acc);
-            group_probe(T, hp, b2, acc);
-            group_probe(T, hp, b3, acc);
-            b0 = n0;
-            b1 = n1;
-            b2 = n2;
-            b3 = n3;
-        }
-        group_probe(T, hp, b0, acc);
-        group_probe(T, hp, b1, acc);
-        group_probe(T, hp, b2, acc);
-        group_probe(T, hp, b3, acc);
-        j += 4;
-    }
-    for (; j < Lt; ++j) group_probe(T, hp, p[(size_t)j * kWave], acc);
-}
-
-__device__ __forceinline__ int acc_row(const uint32_t acc[4], int r) {
-    return (int)((acc[r >> 1] >> (16 * (r & 1))) & 0xFFFFu);
-}
-
-// Multi-row Tanimoto block: one workgroup per row group (grid.x), query
-// tiles split across waves (grid.y * 4).  Each query entry is loaded once
-// and probed once for all rows of the group.
-__global__ __launch_bounds__(kBlock) void k_hash_rows_f32(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
-                                                          ColArgs C, int64_t rb, int64_t rl, float* __restrict__ out,
-                                                          int64_t opitch) {
-    __shared__ GroupLds T;
-    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
-    stage_group(T, G, hd);
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
-    const int lane = lane_id();
-    int ma[siml::kGroupRows];
-#pragma unroll
-    for (int r = 0; r < siml::kGroupRows; ++r) ma[r] = r < hd.nrows ? rmag[hd.row0 + r] : 0;
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t acc[4];
-        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
-        if (c >= C.n) continue;
-        const int mb = C.mag[c];
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r) {
-            const int64_t row = (int64_t)hd.row0 + r;
-            if (r < hd.nrows && row >= rb && row < rl)
-                out[(row - rb) * opitch + col_of(C, c)] = tanimoto_mags(acc_row(acc, r), ma[r], mb);
-        }
-    }
-}
-
-// gpuHistogrammedLingo_multirow (kernels.py:606-667) on row groups: per-row
-// LDS bins, flushed to the int32 (rows, 101) result with global atomics.
-__global__ __launch_bounds__(kBlock) void k_hash_hist_rows(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
-                                                           ColArgs C, int64_t rb, int64_t rl,
-                                                           int32_t* __restrict__ hist) {
-    __shared__ GroupLds T;
-    __shared__ int bins[siml::kGroupRows][101];
-    for (int k = threadIdx.x; k < siml::kGroupRows * 101; k += blockDim.x) (&bins[0][0])[k] = 0;
-    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
-    stage_group(T, G, hd);
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
-    const int lane = lane_id();
-    int ma[siml::kGroupRows];
-    bool live[siml::kGroupRows];
-#pragma unroll
-    for (int r = 0; r < siml::kGroupRows; ++r) {
-        const int64_t row = (int64_t)hd.row0 + r;
-        live[r] = r < hd.nrows && row >= rb && row < rl;
-        ma[r] = live[r] ? rmag[row] : 0;
-    }
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t acc[4];
-        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
-        if (c >= C.n) continue;
-        const int mb = C.mag[c];
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r)
-            if (live[r]) atomicAdd(&bins[r][bin101_f32(tanimoto_mags(acc_row(acc, r), ma[r], mb))], 1);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < siml::kGroupRows * 101; k += blockDim.x) {
-        const int r = k / 101, b = k - r * 101;
-        if (live[r] && bins[r][b]) atomicAdd(&hist[((int64_t)hd.row0 + r - rb) * 101 + b], bins[r][b]);
-    }
-}
-
-// gpuGetNeighbors_multirow (kernels.py:393-475) on row groups.  Same
-// criterion and ordering as k_neighbors: each wave owns a contiguous tile
-// range and keeps one running count per row; pass 1 counts, pass 2 writes
-// ascending column indices at the scanned offsets.
-template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_hash_neighbors(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
-                                                           ColArgs C, int64_t rb, int64_t rl, float lo, float hi,
-                                                           int32_t* __restrict__ counts,
-                                                           const int64_t* __restrict__ offs,
-                                                           int32_t* __restrict__ idx,
-                                                           const int32_t* __restrict__ rlen,
-                                                           unsigned long long* __restrict__ work) {
-    __shared__ GroupLds T;
-    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
-    stage_group(T, G, hd);
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
-    const int lane = lane_id();
-    int ma[siml::kGroupRows];
-    bool live[siml::kGroupRows];
-    int64_t pos[siml::kGroupRows];
-    unsigned nlive = 0, sum_la = 0;
-#pragma unroll
-    for (int r = 0; r < siml::kGroupRows; ++r) {
-        const int64_t row = (int64_t)hd.row0 + r;
-        live[r] = r < hd.nrows && row >= rb && row < rl;
-        ma[r] = live[r] ? rmag[row] : 0;
-        pos[r] = (FILL && live[r]) ? offs[(row - rb) * nw + gw] : 0;
-        if (work && live[r]) {
-            nlive += 1;
-            sum_la += (unsigned)rlen[row];
-        }
-    }
-    unsigned long long wpairs = 0, wlsum = 0;  // evaluated-work counter
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t acc[4];
-        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
-        const bool valid = c < C.n;
-        const int mb = valid ? C.mag[c] : 0;
-        if (work && valid) {
-            wpairs += nlive;
-            wlsum += (unsigned long long)nlive * (unsigned)C.len[c] + sum_la;
-        }
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r) {
-            if (!live[r]) continue;
-            bool nb = false;
-            if (valid) {
-                const float bound = (float)min(ma[r], mb) / (float)max(ma[r], mb);
-                if (bound >= lo) {
-                    const float tv = tanimoto_mags(acc_row(acc, r), ma[r], mb);
-                    nb = (tv >= lo) && (tv < hi);
-                }
-            }
-            const uint64_t m = __ballot(nb);
-            if (FILL && nb) idx[pos[r] + rank_below(m)] = (int32_t)col_of(C, c);
-            pos[r] += __popcll(m);
-        }
-    }
-    if (!FILL && lane == 0) {
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r)
-            if (live[r]) counts[((int64_t)hd.row0 + r - rb) * nw + gw] = (int32_t)pos[r];
-    }
-    if (work) add_work(work, wpairs, wlsum);
-}
-
-// ---------------------------------------------------------------------------
 // 1000-bin global histogram and top-k (no counterpart in the reference)
 // ---------------------------------------------------------------------------
 
@@ -627,42 +386,6 @@ __device__ __forceinline__ void topk_push(TopkBuf& B, bool valid, uint64_t key, 
     B.cnt += __popcll(m);
 }
 
-__global__ __launch_bounds__(kBlock) void k_hash_hist1000(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
-                                                          ColArgs C, int64_t rb, int64_t rl,
-                                                          unsigned long long* __restrict__ hist) {
-    __shared__ GroupLds T;
-    __shared__ uint32_t bins[1000];
-    __shared__ float edges[1001];
-    for (int k = threadIdx.x; k < 1000; k += blockDim.x) bins[k] = 0u;
-    stage_edges(edges);
-    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
-    stage_group(T, G, hd);
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
-    const int lane = lane_id();
-    int ma[siml::kGroupRows];
-    bool live[siml::kGroupRows];
-#pragma unroll
-    for (int r = 0; r < siml::kGroupRows; ++r) {
-        const int64_t row = (int64_t)hd.row0 + r;
-        live[r] = r < hd.nrows && row >= rb && row < rl;
-        ma[r] = live[r] ? rmag[row] : 0;
-    }
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t acc[4];
-        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
-        if (c >= C.n) continue;
-        const int mb = C.mag[c];
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r)
-            if (live[r]) atomicAdd(&bins[bin1000(tanimoto_mags(acc_row(acc, r), ma[r], mb), edges)], 1u);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 1000; k += blockDim.x)
-        if (bins[k]) atomicAdd(&hist[k], (unsigned long long)bins[k]);
-}
-
 __global__ __launch_bounds__(kBlock) void k_hist1000(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
                                                      int64_t rowbase, unsigned long long* __restrict__ hist) {
     extern __shared__ int2 lds_row[];
@@ -687,48 +410,6 @@ __global__ __launch_bounds__(kBlock) void k_hist1000(RowArgs R, ColArgs C, const
     __syncthreads();
     for (int k = threadIdx.x; k < 1000; k += blockDim.x)
         if (bins[k]) atomicAdd(&hist[k], (unsigned long long)bins[k]);
-}
-
-// Top-k candidates per (row, wave): buffers at ((row-rb)*nw + gw) * cap.
-__global__ __launch_bounds__(kBlock) void k_hash_topk(GroupArgs G, int64_t g0, const int32_t* __restrict__ rmag,
-                                                      ColArgs C, int64_t rb, int64_t rl, int k, int cap,
-                                                      uint64_t* __restrict__ buf, int32_t* __restrict__ bufcnt) {
-    __shared__ GroupLds T;
-    __shared__ uint32_t rhist[kBlock / kWave][256];
-    const siml::GroupHdr hd = G.hdr[g0 + blockIdx.x];
-    stage_group(T, G, hd);
-    int64_t t0, t1, gw, nw;
-    wave_tiles(C.ntiles, t0, t1, gw, nw);
-    const int lane = lane_id();
-    uint32_t* hist = rhist[threadIdx.x / kWave];
-    int ma[siml::kGroupRows];
-    bool live[siml::kGroupRows];
-    TopkBuf B[siml::kGroupRows];
-#pragma unroll
-    for (int r = 0; r < siml::kGroupRows; ++r) {
-        const int64_t row = (int64_t)hd.row0 + r;
-        live[r] = r < hd.nrows && row >= rb && row < rl;
-        ma[r] = live[r] ? rmag[row] : 0;
-        B[r].buf = buf + ((live[r] ? row - rb : 0) * nw + gw) * (int64_t)cap;
-        B[r].cnt = 0;
-        B[r].armed = false;
-        B[r].theta = 0ull;
-    }
-    for (int64_t t = t0; t < t1; ++t) {
-        const int64_t c = t * kWave + lane;
-        uint32_t acc[4];
-        group_isct(T, hd, C.tiles + C.toff[t] + lane, C.tlen[t], acc);
-        const bool valid = c < C.n;
-        const int mb = valid ? C.mag[c] : 0;
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r)
-            if (live[r]) topk_push(B[r], valid, topk_key(tanimoto_mags(acc_row(acc, r), ma[r], mb), valid ? col_of(C, c) : 0), k, cap, hist);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < siml::kGroupRows; ++r)
-            if (live[r]) bufcnt[((int64_t)hd.row0 + r - rb) * nw + gw] = B[r].cnt;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int32_t* __restrict__ rowlist,
@@ -2026,13 +1707,7 @@ struct RowSet {
     bool sorted_unique = true;
     DevBuf elems, off, len, mag, csum;
     std::vector<int32_t> h_len;
-    // row groups of the hashed kernels
-    DevBuf g_hdr, g_ekey, g_ecnt;
-    std::vector<siml::GroupHdr> h_hdr;
-    std::vector<int32_t> merge_rows;  // rows served by the merge kernel
-    bool hash_built = false;
     bool mags_small = false;
-    GroupArgs gargs() const { return {g_hdr.as<siml::GroupHdr>(), g_ekey.as<int2>(), g_ecnt.as<uint4>()}; }
     // bit-sliced row groups: [0] consecutive rows (built by set_ref under the
     // AUTO policy), [1] rows ordered by magnitude (built on first use by the
     // neighbour search, whose tile pruning needs narrow magnitude bands)
@@ -2068,11 +1743,6 @@ struct RowSet {
         len.release();
         mag.release();
         csum.release();
-        g_hdr.release();
-        g_ekey.release();
-        g_ecnt.release();
-        h_hdr.clear();
-        merge_rows.clear();
         bits[0].release();
         bits[1].release();
         h_ids.clear();
@@ -2080,7 +1750,6 @@ struct RowSet {
         h_mag.clear();
         h_csum.clear();
         h_start.clear();
-        hash_built = false;
         n = -1;
     }
 };
@@ -2345,18 +2014,6 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     TRY(upload(R.len, info.len.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.mag, info.mag.data(), (size_t)s.n, ctx->stream));
     TRY(upload(R.csum, info.csum.data(), (size_t)s.n, ctx->stream));
-    // multi-row tables of the policy in force: bit-sliced groups (AUTO) or
-    // 8-row cuckoo groups (HASH8)
-    std::vector<int64_t> starts((size_t)std::max<int64_t>(s.n, 1));
-    for (int64_t r = 0; r < s.n; ++r) starts[(size_t)r] = s.start(r);
-    siml::GroupTables gt;
-    if (ctx->policy == SIML_POLICY_HASH8) {
-        siml::GroupInput gin{s.ids, s.counts, starts.data(), info.len.data(), info.csum.data(), s.n};
-        siml::build_groups(gin, gt, 0);
-        TRY(upload(R.g_hdr, gt.hdr.data(), gt.hdr.size(), ctx->stream));
-        TRY(upload(R.g_ekey, reinterpret_cast<const int2*>(gt.ekey.data()), gt.ekey.size(), ctx->stream));
-        TRY(upload(R.g_ecnt, reinterpret_cast<const uint4*>(gt.ecnt.data()), gt.ecnt.size() / 4, ctx->stream));
-    }
     R.bits[0].reset();
     R.bits[1].reset();
     R.h_ids.resize((size_t)total);
@@ -2370,14 +2027,11 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     R.h_start.assign(off.begin(), off.end());
     R.h_mag = info.mag;
     R.h_csum = info.csum;
-    R.hash_built = ctx->policy == SIML_POLICY_HASH8;
     R.mags_small = info.mags_small;
     R.n = s.n;
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
     R.h_len = std::move(info.len);
-    R.h_hdr = std::move(gt.hdr);
-    R.merge_rows = std::move(gt.merge_rows);
     if (ctx->policy == SIML_POLICY_AUTO) TRY(ensure_bits(ctx, 0));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
@@ -2554,29 +2208,6 @@ int32_t max_len_in(const RowSet& R, int64_t r0, int64_t r1) {
     return m;
 }
 
-// How rows [rb, rl) are served: hashed groups [g0, g1) plus the rows in
-// `merge` on the merge kernel, or every row on the merge kernel.
-struct RowPlan {
-    bool all_merge = true;
-    int64_t g0 = 0, g1 = 0;
-    std::vector<int32_t> merge;
-};
-
-void plan_rows(const siml_ctx* ctx, int64_t rb, int64_t rl, RowPlan& p) {
-    const RowSet& R = ctx->ref;
-    const ColSet& C = *ctx->qry;
-    p = RowPlan();
-    if (ctx->policy == SIML_POLICY_MERGE || !C.sorted_unique || !C.counts_nonneg || !R.hash_built) return;
-    p.all_merge = false;
-    const auto& H = R.h_hdr;
-    p.g0 = std::partition_point(H.begin(), H.end(), [&](const siml::GroupHdr& h) { return h.row0 + h.nrows <= rb; }) -
-           H.begin();
-    p.g1 = std::partition_point(H.begin(), H.end(), [&](const siml::GroupHdr& h) { return h.row0 < rl; }) - H.begin();
-    auto a = std::lower_bound(R.merge_rows.begin(), R.merge_rows.end(), (int32_t)rb);
-    auto b = std::lower_bound(R.merge_rows.begin(), R.merge_rows.end(), (int64_t)rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
-    p.merge.assign(a, b);
-}
-
 int32_t* d_counts_of(siml_ctx* ctx) { return ctx->scratch.as<int32_t>(); }
 
 int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
@@ -2747,6 +2378,160 @@ std::vector<DbArray> db_arrays(ColSet& C) {
 }
 
 size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
+
+// Array order of db_arrays.
+enum DbArrayIndex {
+    kDbTiles, kDbToff, kDbTlen, kDbLen, kDbMag, kDbCsum, kDbPerm, kDbTmag, kDbBtiles, kDbBtoff, kDbBtlen, kDbArrays
+};
+
+void db_header_to_set(const DbFileHeader& h, ColSet& C) {
+    C.n = h.n;
+    C.ntiles = h.ntiles;
+    C.tiles_total = h.tiles_total;
+    C.btiles_total = h.btiles_total;
+    C.maxlen = h.maxlen;
+    C.layout = h.layout;
+    C.identity = h.identity;
+    C.blocked = h.blocked;
+    C.levels_ok = h.levels_ok;
+    C.sorted_unique = h.sorted_unique;
+    C.counts_nonneg = h.counts_nonneg;
+    C.mags_small = h.mags_small;
+}
+
+// The per-tile and per-molecule arrays of a database file.
+struct DbHostArrays {
+    std::vector<int64_t> toff, btoff;
+    std::vector<int32_t> tlen, btlen, len, mag, csum, perm;
+    std::vector<int2> tmag;
+    void resize(const DbFileHeader& h) {
+        const size_t n = (size_t)h.n, nt = (size_t)h.ntiles;
+        toff.resize(nt + 1);
+        tlen.resize(nt);
+        len.resize(n);
+        mag.resize(n);
+        csum.resize(n);
+        perm.resize(n);
+        tmag.resize(nt);
+        if (h.levels_ok) {
+            btoff.resize(nt + 1);
+            btlen.resize(nt);
+        }
+    }
+};
+
+// Sizes and flags of a header that the arrays do not need to confirm.
+int check_db_header(const DbFileHeader& h, const char* path) {
+    auto flag = [](uint8_t v) { return v <= 1; };
+    const bool ok = h.n >= 0 && h.n <= 0x7FFFFFFF && h.ntiles == (h.n + kWave - 1) / kWave && h.tiles_total >= 0 &&
+                    h.btiles_total >= 0 && h.maxlen >= 0 &&
+                    (h.layout == SIML_LAYOUT_NATURAL || h.layout == SIML_LAYOUT_BLOCK ||
+                     h.layout == SIML_LAYOUT_SORTED) &&
+                    flag(h.identity) && flag(h.blocked) && flag(h.levels_ok) && flag(h.sorted_unique) &&
+                    flag(h.counts_nonneg) && flag(h.mags_small) &&
+                    (!h.levels_ok || (h.sorted_unique && h.counts_nonneg)) && (h.levels_ok || h.btiles_total == 0);
+    if (!ok) return fail(SIML_EINVAL, "siml_db_load: %s has an inconsistent header", path);
+    return SIML_OK;
+}
+
+// Tile offsets, lengths, magnitudes, permutation and tile magnitude ranges:
+// every index the kernels derive from them stays inside its array, and the
+// values match what the builder derives from the molecules.
+int check_db_index(const DbFileHeader& h, const DbHostArrays& H, const char* path) {
+    const int64_t n = h.n, nt = h.ntiles;
+    if (H.toff[0] != 0) return fail(SIML_EINVAL, "siml_db_load: %s: tile offsets do not start at 0", path);
+    for (int64_t t = 0; t < nt; ++t)
+        if (H.tlen[(size_t)t] < 0 || H.toff[(size_t)t + 1] != H.toff[(size_t)t] + (int64_t)kWave * H.tlen[(size_t)t])
+            return fail(SIML_EINVAL, "siml_db_load: %s: tile %lld offsets do not match its length", path,
+                        (long long)t);
+    if (H.toff[(size_t)nt] != h.tiles_total)
+        return fail(SIML_EINVAL, "siml_db_load: %s: tile offsets do not end at the stream size", path);
+    int32_t maxlen = 0;
+    std::vector<char> seen((size_t)std::max<int64_t>(n, 1), 0);
+    for (int64_t p = 0; p < n; ++p) {
+        const int32_t L = H.len[(size_t)p];
+        if (L < 0 || L > H.tlen[(size_t)(p / kWave)])
+            return fail(SIML_EINVAL, "siml_db_load: %s: molecule %lld is longer than its tile", path, (long long)p);
+        maxlen = std::max(maxlen, L);
+        const int32_t c = H.perm[(size_t)p];
+        if (c < 0 || c >= n || seen[(size_t)c])
+            return fail(SIML_EINVAL, "siml_db_load: %s: the column permutation is not a permutation", path);
+        seen[(size_t)c] = 1;
+        if ((h.identity && c != p) || (h.blocked && c / kSortBlock != p / kSortBlock))
+            return fail(SIML_EINVAL, "siml_db_load: %s: the column permutation breaks its layout flags", path);
+        const int32_t m = H.mag[(size_t)p];
+        if (h.mags_small && (m < 0 || m >= 65536 || m < H.csum[(size_t)p]))
+            return fail(SIML_EINVAL, "siml_db_load: %s: molecule %lld's magnitude breaks the small-magnitude flag",
+                        path, (long long)p);
+    }
+    if (maxlen != h.maxlen) return fail(SIML_EINVAL, "siml_db_load: %s: the maximum length does not match", path);
+    for (int64_t t = 0; t < nt; ++t) {
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (int64_t p = t * kWave; p < std::min(n, (t + 1) * kWave); ++p) {
+            lo = std::min(lo, H.mag[(size_t)p]);
+            hi = std::max(hi, H.mag[(size_t)p]);
+        }
+        if (H.tmag[(size_t)t].x != lo || H.tmag[(size_t)t].y != hi)
+            return fail(SIML_EINVAL, "siml_db_load: %s: tile %lld's magnitude range does not match", path,
+                        (long long)t);
+    }
+    if (h.levels_ok) {
+        if (H.btoff[0] != 0) return fail(SIML_EINVAL, "siml_db_load: %s: level offsets do not start at 0", path);
+        for (int64_t t = 0; t < nt; ++t) {
+            const int32_t m = H.btlen[(size_t)t];
+            if (m < 0 || m % 8 != 0 || H.btoff[(size_t)t + 1] != H.btoff[(size_t)t] + (int64_t)kWave * m)
+                return fail(SIML_EINVAL, "siml_db_load: %s: tile %lld's level offsets do not match its length", path,
+                            (long long)t);
+        }
+        // the stream ends with one spare block of 8 levels per lane
+        if (H.btoff[(size_t)nt] + 8 * kWave != h.btiles_total)
+            return fail(SIML_EINVAL, "siml_db_load: %s: level offsets do not end at the stream size", path);
+    }
+    return SIML_OK;
+}
+
+// The merge stream: count sums equal csum, and ids ascending / counts
+// non-negative where the header says so.
+int check_db_tiles(const DbFileHeader& h, const DbHostArrays& H, const std::vector<int2>& tiles, const char* path) {
+    std::atomic<int64_t> bad{-1};
+    parallel_for(h.n, [&](int64_t a, int64_t b) {
+        for (int64_t p = a; p < b && bad.load(std::memory_order_relaxed) < 0; ++p) {
+            const int64_t t = p / kWave;
+            const int2* e = tiles.data() + H.toff[(size_t)t] + (p - t * kWave);
+            uint32_t sum = 0;
+            bool ok = true;
+            for (int32_t j = 0; j < H.len[(size_t)p]; ++j) {
+                const int2 v = e[(size_t)j * kWave];
+                sum += (uint32_t)v.y;
+                if (h.counts_nonneg && v.y < 0) ok = false;
+                if (h.sorted_unique && j > 0 && v.x <= e[(size_t)(j - 1) * kWave].x) ok = false;
+            }
+            if (!ok || (int32_t)sum != H.csum[(size_t)p]) bad.store(p);
+        }
+    });
+    if (bad.load() >= 0)
+        return fail(SIML_EINVAL, "siml_db_load: %s: molecule %lld's entries do not match its flags or count sum", path,
+                    (long long)bad.load());
+    return SIML_OK;
+}
+
+// The level stream: every (key, slot word) is an empty level (0, 0) or a
+// key with the slot word the probe rule derives from it, so every table
+// read of the kernels stays inside the table.
+int check_db_levels(const DbFileHeader& h, const DbHostArrays& H, const std::vector<int2>& levels, const char* path) {
+    (void)H;
+    std::atomic<int64_t> bad{-1};
+    parallel_for(h.btiles_total, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b && bad.load(std::memory_order_relaxed) < 0; ++i) {
+            const uint32_t key = (uint32_t)levels[(size_t)i].x, sw = (uint32_t)levels[(size_t)i].y;
+            if (key == 0u ? sw != 0u : sw != siml::slot_word(key)) bad.store(i);
+        }
+    });
+    if (bad.load() >= 0)
+        return fail(SIML_EINVAL, "siml_db_load: %s: level %lld has a slot word its key does not give", path,
+                    (long long)bad.load());
+    return SIML_OK;
+}
 
 }  // namespace
 
@@ -2984,45 +2769,73 @@ int siml_db_load(int device, const char* path, siml_db** out) {
     if (h.version != kDbVersion)
         return fail(SIML_EINVAL, "siml_db_load: %s has format version %d, this build reads %d", path, h.version,
                     kDbVersion);
-    if (h.n < 0 || h.ntiles != (h.n + kWave - 1) / kWave || h.tiles_total < 0 || h.btiles_total < 0 ||
-        (h.layout != SIML_LAYOUT_NATURAL && h.layout != SIML_LAYOUT_BLOCK && h.layout != SIML_LAYOUT_SORTED))
-        return fail(SIML_EINVAL, "siml_db_load: %s has an inconsistent header", path);
+    TRY(check_db_header(h, path));
+    // every array is read and checked on the host before anything reaches
+    // the device: the small per-tile and per-molecule arrays first, then the
+    // two entry streams against them
+    auto C = std::make_shared<ColSet>();
+    db_header_to_set(h, *C);
+    const std::vector<DbArray> arrays = db_arrays(*C);
+    std::vector<int64_t> at(arrays.size());
+    {
+        size_t pos = sizeof(h);
+        for (size_t i = 0; i < arrays.size(); ++i) {
+            pos = align64(pos);
+            at[i] = (int64_t)pos;
+            pos += arrays[i].bytes;
+        }
+    }
+    // the arrays must lie inside the file before any host memory is sized
+    // by the header's counts
+    if (fseeko(f, 0, SEEK_END) != 0) return fail(SIML_EINVAL, "siml_db_load: cannot seek in %s", path);
+    const off_t fsize = ftello(f);
+    if (fsize < 0 || (uint64_t)at.back() + arrays.back().bytes > (uint64_t)fsize)
+        return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
+    auto read_at = [&](size_t i, void* dst) -> int {
+        if (arrays[i].bytes == 0) return SIML_OK;
+        if (fseeko(f, (off_t)at[i], SEEK_SET) != 0 || std::fread(dst, 1, arrays[i].bytes, f) != arrays[i].bytes)
+            return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
+        return SIML_OK;
+    };
+    DbHostArrays H;
+    H.resize(h);
+    TRY(read_at(kDbToff, H.toff.data()));
+    TRY(read_at(kDbTlen, H.tlen.data()));
+    TRY(read_at(kDbLen, H.len.data()));
+    TRY(read_at(kDbMag, H.mag.data()));
+    TRY(read_at(kDbCsum, H.csum.data()));
+    TRY(read_at(kDbPerm, H.perm.data()));
+    TRY(read_at(kDbTmag, H.tmag.data()));
+    if (h.levels_ok) {
+        TRY(read_at(kDbBtoff, H.btoff.data()));
+        TRY(read_at(kDbBtlen, H.btlen.data()));
+    }
+    TRY(check_db_index(h, H, path));
+    std::vector<int2> stream((size_t)std::max<int64_t>(h.tiles_total, 1));
+    TRY(read_at(kDbTiles, stream.data()));
+    TRY(check_db_tiles(h, H, stream, path));
+    std::vector<int2> levels;
+    if (h.levels_ok) {
+        levels.resize((size_t)std::max<int64_t>(h.btiles_total, 1));
+        TRY(read_at(kDbBtiles, levels.data()));
+        TRY(check_db_levels(h, H, levels, path));
+    }
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) {
         (void)hipGetLastError();
         return fail(SIML_EDEVICE, "siml_db_load: device %d is not visible", device);
     }
     HIP_TRY(hipSetDevice(device));
-    auto set = std::make_shared<ColSet>();
-    ColSet& C = *set;
-    C.n = h.n;
-    C.ntiles = h.ntiles;
-    C.tiles_total = h.tiles_total;
-    C.btiles_total = h.btiles_total;
-    C.maxlen = h.maxlen;
-    C.layout = h.layout;
-    C.identity = h.identity;
-    C.blocked = h.blocked;
-    C.levels_ok = h.levels_ok;
-    C.sorted_unique = h.sorted_unique;
-    C.counts_nonneg = h.counts_nonneg;
-    C.mags_small = h.mags_small;
-    size_t pos = sizeof(h);
-    std::vector<char> host;
-    for (const DbArray& a : db_arrays(C)) {
-        const size_t next = align64(pos);
-        if (std::fseek(f, (long)next, SEEK_SET) != 0) return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
-        pos = next;
-        host.resize(std::max<size_t>(a.bytes, 1));
-        if (a.bytes && std::fread(host.data(), 1, a.bytes, f) != a.bytes)
-            return fail(SIML_EINVAL, "siml_db_load: %s is truncated", path);
-        TRY(a.buf->reserve(a.bytes));
-        if (a.bytes) HIP_TRY(hipMemcpy(a.buf->p, host.data(), a.bytes, hipMemcpyHostToDevice));
-        pos += a.bytes;
+    const void* src[kDbArrays] = {stream.data(), H.toff.data(), H.tlen.data(), H.len.data(), H.mag.data(),
+                                  H.csum.data(), H.perm.data(), H.tmag.data(), levels.data(), H.btoff.data(),
+                                  H.btlen.data()};
+    for (size_t i = 0; i < arrays.size(); ++i) {
+        TRY(arrays[i].buf->reserve(arrays[i].bytes));
+        if (arrays[i].bytes) HIP_TRY(hipMemcpy(arrays[i].buf->p, src[i], arrays[i].bytes, hipMemcpyHostToDevice));
     }
     siml_db* db = new siml_db();
     db->device = device;
-    db->set = std::move(set);
+    db->set = std::move(C);
     *out = db;
     return SIML_OK;
 }
@@ -3083,31 +2896,11 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
         ctx->last_kernel = bp.merge.empty() ? "rows_bits" : (bp.groups.empty() ? "rows_merge" : "rows_bits+merge");
         return SIML_OK;
     }
-    RowPlan plan;
-    plan_rows(ctx, rowbase, rowlimit, plan);
-    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
     TimedRegion tr(ctx, nr * nq);
-    if (plan.all_merge) {
-        k_rows_f32<false><<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
-            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, ctx->res.as<float>(), dp);
-        TRY(launch_check("k_rows_f32"));
-        ctx->last_kernel = "rows_merge";
-    } else {
-        const int64_t ng = plan.g1 - plan.g0;
-        if (ng > 0) {
-            k_hash_rows_f32<<<row_grid(ng, ctx->qry->ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit,
-                ctx->res.as<float>(), dp);
-            TRY(launch_check("k_hash_rows_f32"));
-        }
-        if (!plan.merge.empty()) {
-            k_rows_f32<false><<<row_grid((int64_t)plan.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
-                                ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
-                                               ctx->res.as<float>(), dp);
-            TRY(launch_check("k_rows_f32<rowlist>"));
-        }
-        ctx->last_kernel = plan.merge.empty() ? "rows_hash" : (ng > 0 ? "rows_hash+merge" : "rows_merge");
-    }
+    k_rows_f32<false><<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+        ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, ctx->res.as<float>(), dp);
+    TRY(launch_check("k_rows_f32"));
+    ctx->last_kernel = "rows_merge";
     tr.done();
     return SIML_OK;
 }
@@ -3164,29 +2957,12 @@ static int hist_rows_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
         ctx->last_kernel = "hist_rows_bits";
         return SIML_OK;
     }
-    RowPlan plan;
-    plan_rows(ctx, rowbase, rowlimit, plan);
-    if (!plan.merge.empty()) TRY(upload_rowlist(ctx, plan.merge));
     TimedRegion tr(ctx, nr * nq);
-    if (plan.all_merge) {
-        k_hist_rows<<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
-            ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, hist);
-        TRY(launch_check("k_hist_rows"));
-    } else {
-        const int64_t ng = plan.g1 - plan.g0;
-        if (ng > 0) {
-            k_hash_hist_rows<<<row_grid(ng, ctx->qry->ntiles, 2048), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, hist);
-            TRY(launch_check("k_hash_hist_rows"));
-        }
-        if (!plan.merge.empty()) {
-            k_hist_rows<<<row_grid((int64_t)plan.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
-            TRY(launch_check("k_hist_rows<rowlist>"));
-        }
-    }
+    k_hist_rows<<<row_grid(nr, ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+        ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, hist);
+    TRY(launch_check("k_hist_rows"));
     tr.done();
-    ctx->last_kernel = plan.all_merge ? "hist_rows_merge" : "hist_rows_hash";
+    ctx->last_kernel = "hist_rows_merge";
     return SIML_OK;
 }
 
@@ -3205,9 +2981,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     }
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp, true));
-    RowPlan plan;
-    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
-    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
@@ -3218,9 +2992,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
         grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
     }
     const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
-    const int64_t ng = plan.g1 - plan.g0;
     int32_t* d_nbuf = nullptr;
-    const dim3 hgrid((unsigned)std::max<int64_t>(ng, 1), grid.y);
     const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y);
     unsigned long long* work = ctx->timing ? ctx->work.as<unsigned long long>() : nullptr;
     const int32_t* rlen = ctx->ref.args().len;
@@ -3235,7 +3007,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
                 TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
                                 EpiNbr<false>{lo, hi, counts, nullptr, nullptr, rowbase, d_nbuf, kNbrCap, work, rlen},
                                 "k_bits<nbr count>"));
-        } else if (plan.all_merge) {
+        } else {
             if (fill)
                 k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
                                                                             rowbase, lo, hi, nullptr, offs, idx, work);
@@ -3244,16 +3016,6 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
                                                                              rowbase, lo, hi, counts, nullptr, nullptr,
                                                                              work);
             return launch_check("k_neighbors");
-        } else if (ng > 0) {
-            if (fill)
-                k_hash_neighbors<true><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
-                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
-                    nullptr, offs, idx, rlen, work);
-            else
-                k_hash_neighbors<false><<<hgrid, dim3(kBlock), 0, ctx->stream>>>(
-                    ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, lo, hi,
-                    counts, nullptr, nullptr, rlen, work);
-            TRY(launch_check("k_hash_neighbors"));
         }
         // rows the multi-row kernels do not serve: merge kernel over a row list
         if (!mrows.empty()) {
@@ -3325,7 +3087,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     tr.done();
     // h_offs and csr_rows must outlive the async copies above.
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->last_kernel = bp.use ? "neighbors_bits" : (plan.all_merge ? "neighbors_merge" : "neighbors_hash");
+    ctx->last_kernel = bp.use ? "neighbors_bits" : "neighbors_merge";
     return SIML_OK;
 }
 
@@ -3344,9 +3106,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     if (nr == 0) return SIML_OK;
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp));
-    RowPlan plan;
-    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
-    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const int cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
@@ -3383,7 +3143,6 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     uint64_t* d_sbuf2 = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes);
     int32_t* d_scnt = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes);
     int32_t* d_scnt2 = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes);
-    const int64_t ng = plan.g1 - plan.g0;
     // reduce kTopkFan buffers per workgroup until one workgroup per row can
     // take the rest, then select and sort
     auto select = [&](int32_t* out_idx, float* out_t, uint64_t* kth) -> int {
@@ -3442,26 +3201,14 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
                 ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
             TRY(launch_check("k_topk<rowlist>"));
         }
-    } else if (plan.all_merge) {
+    } else {
         k_topk<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, k, cap,
                                                          d_buf, d_cnt);
         TRY(launch_check("k_topk"));
-    } else {
-        if (ng > 0) {
-            k_hash_topk<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, k, cap,
-                d_buf, d_cnt);
-            TRY(launch_check("k_hash_topk"));
-        }
-        if (!plan.merge.empty()) {
-            k_topk<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
-            TRY(launch_check("k_topk<rowlist>"));
-        }
     }
     TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>(), nullptr));
     tr.done();
-    ctx->last_kernel = bp.use ? "topk_bits" : (plan.all_merge ? "topk_merge" : "topk_hash");
+    ctx->last_kernel = bp.use ? "topk_bits" : "topk_merge";
     return SIML_OK;
 }
 
@@ -3480,9 +3227,7 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     if (nr == 0 || nq == 0) return SIML_OK;
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp));
-    RowPlan plan;
-    if (!bp.use) plan_rows(ctx, rowbase, rowlimit, plan);
-    const std::vector<int32_t>& mrows = bp.use ? bp.merge : plan.merge;
+    const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
@@ -3491,7 +3236,6 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
     if (bp.use) grid.y = std::max<unsigned>(xcd_gy(grid.y, 65535), (unsigned)std::min<int64_t>(min_gy, 65535));
-    const int64_t ng = plan.g1 - plan.g0;
     TimedRegion tr(ctx, nr * nq);
     if (bp.use) {
         TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiHist1000{hist}, "k_bits<hist1000>"));
@@ -3500,23 +3244,12 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
                 ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
             TRY(launch_check("k_hist1000<rowlist>"));
         }
-    } else if (plan.all_merge) {
+    } else {
         k_hist1000<<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr, rowbase, hist);
         TRY(launch_check("k_hist1000"));
-    } else {
-        if (ng > 0) {
-            k_hash_hist1000<<<dim3((unsigned)ng, grid.y), dim3(kBlock), 0, ctx->stream>>>(
-                ctx->ref.gargs(), plan.g0, ctx->ref.mag.as<int32_t>(), ctx->qry->args(), rowbase, rowlimit, hist);
-            TRY(launch_check("k_hash_hist1000"));
-        }
-        if (!plan.merge.empty()) {
-            k_hist1000<<<dim3((unsigned)plan.merge.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
-            TRY(launch_check("k_hist1000<rowlist>"));
-        }
     }
     tr.done();
-    ctx->last_kernel = bp.use ? "hist1000_bits" : (plan.all_merge ? "hist1000_merge" : "hist1000_hash");
+    ctx->last_kernel = bp.use ? "hist1000_bits" : "hist1000_merge";
     return SIML_OK;
 }
 
@@ -3899,8 +3632,7 @@ int siml_retrieve_i64(siml_ctx* ctx, int64_t* host) {
 
 int siml_set_policy(siml_ctx* ctx, int policy) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE && policy != SIML_POLICY_HASH8)
-        return fail(SIML_EINVAL, "unknown policy %d", policy);
+    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE) return fail(SIML_EINVAL, "unknown policy %d", policy);
     ctx->policy = policy;
     return SIML_OK;
 }
@@ -3962,15 +3694,69 @@ const char* siml_last_kernel(siml_ctx* ctx) { return ctx ? ctx->last_kernel.c_st
 // Stateless entry points
 // ---------------------------------------------------------------------------
 
-static std::mutex g_shared_mu;
-static std::vector<siml_ctx*> g_shared;
+// One handle per device serves the stateless entry points.  ctypes releases
+// the GIL around C calls (the reference ran these calls under it), so calls
+// from several host threads can overlap: each call holds its device's lock
+// from set-up to the copy of its result, and the handle's buffers are never
+// resized while another call's kernels or copies use them.
+// The molecules of a padded set as the kernels read them: per row its
+// length, magnitude and first `length` (id, count) entries.
+struct SetSnapshot {
+    bool valid = false;
+    std::vector<int32_t> len, mag, ids, counts;
+    // whether (lingos, counts, mags, lengths) holds exactly these molecules
+    bool same(const int32_t* L, const int32_t* C, const int32_t* M, const int32_t* N, int64_t n, int64_t width,
+              int64_t pitch) const {
+        if (!valid || n != (int64_t)len.size() || !L || !C || !M || !N || pitch < width) return false;
+        size_t k = 0;
+        for (int64_t r = 0; r < n; ++r) {
+            const int32_t l = N[r];
+            if (l != len[(size_t)r] || M[r] != mag[(size_t)r] || l < 0 || l > width) return false;
+            if (std::memcmp(L + r * pitch, ids.data() + k, sizeof(int32_t) * (size_t)l) != 0 ||
+                std::memcmp(C + r * pitch, counts.data() + k, sizeof(int32_t) * (size_t)l) != 0)
+                return false;
+            k += (size_t)l;
+        }
+        return true;
+    }
+    void take(const int32_t* L, const int32_t* C, const int32_t* M, const int32_t* N, int64_t n, int64_t pitch) {
+        len.assign(N, N + n);
+        mag.assign(M, M + n);
+        ids.clear();
+        counts.clear();
+        for (int64_t r = 0; r < n; ++r) {
+            ids.insert(ids.end(), L + r * pitch, L + r * pitch + N[r]);
+            counts.insert(counts.end(), C + r * pitch, C + r * pitch + N[r]);
+        }
+        valid = true;
+    }
+};
 
-static int shared_ctx(int device, siml_ctx** out) {
-    std::lock_guard<std::mutex> lk(g_shared_mu);
+struct SharedCtx {
+    std::mutex mu;
+    siml_ctx* ctx = nullptr;
+    // the query set last uploaded by siml_tanimoto_block: the reference's
+    // callers pass the same query set with every block of rows
+    // (CPULingo.py:126-138), so it is rebuilt only when its contents change
+    SetSnapshot query;
+};
+static std::mutex g_shared_mu;
+static std::map<int, std::unique_ptr<SharedCtx>> g_shared;
+
+// Locks device `device`'s shared handle for the caller's lifetime of `lk`.
+static int shared_ctx(int device, std::unique_lock<std::mutex>& lk, siml_ctx** out, SharedCtx** shared = nullptr) {
     if (device < 0) return fail(SIML_EINVAL, "negative device");
-    if ((size_t)device >= g_shared.size()) g_shared.resize((size_t)device + 1, nullptr);
-    if (!g_shared[(size_t)device]) TRY(siml_create(device, &g_shared[(size_t)device]));
-    *out = g_shared[(size_t)device];
+    SharedCtx* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_shared_mu);
+        std::unique_ptr<SharedCtx>& slot = g_shared[device];
+        if (!slot) slot.reset(new SharedCtx());
+        sc = slot.get();
+    }
+    lk = std::unique_lock<std::mutex>(sc->mu);
+    if (!sc->ctx) TRY(siml_create(device, &sc->ctx));
+    *out = sc->ctx;
+    if (shared) *shared = sc;
     return SIML_OK;
 }
 
@@ -3981,9 +3767,15 @@ int siml_tanimoto_block(int device, const int32_t* rlingos, const int32_t* rcoun
                         int64_t opitch) {
     if (!rmags || !qmags) return fail(SIML_EINVAL, "tanimoto_block: magnitude vectors are required");
     siml_ctx* ctx = nullptr;
-    TRY(shared_ctx(device, &ctx));
+    SharedCtx* sc = nullptr;
+    std::unique_lock<std::mutex> lk;
+    TRY(shared_ctx(device, lk, &ctx, &sc));
     TRY(siml_set_ref(ctx, rlingos, rcounts, rmags, rlengths, nr, rwidth, rpitch));
-    TRY(siml_set_query(ctx, qlingos, qcounts, qmags, qlengths, nq, qwidth, qpitch));
+    if (!sc->query.same(qlingos, qcounts, qmags, qlengths, nq, qwidth, qpitch)) {
+        sc->query.valid = false;
+        TRY(siml_set_query(ctx, qlingos, qcounts, qmags, qlengths, nq, qwidth, qpitch));
+        sc->query.take(qlingos, qcounts, qmags, qlengths, nq, qpitch);
+    }
     TRY(siml_multirow_async(ctx, 0, nr));
     return siml_retrieve_f32(ctx, out, opitch);
 }
@@ -3994,7 +3786,8 @@ static int row_hist_common(int device, const void* data, size_t esz, int64_t row
     if (rows == 0) return SIML_OK;
     if (!data || !hist) return fail(SIML_EINVAL, "rowHistogram101: null buffer");
     siml_ctx* ctx = nullptr;
-    TRY(shared_ctx(device, &ctx));
+    std::unique_lock<std::mutex> lk;
+    TRY(shared_ctx(device, lk, &ctx));
     TRY(set_device(ctx));
     TRY(ctx->scratch.reserve(esz * (size_t)std::max<int64_t>(rows * pitch, 1)));
     TRY(ctx->res2.reserve(sizeof(uint32_t) * (size_t)rows * 101));
@@ -4025,7 +3818,8 @@ int siml_selftest_valu(int device, int op, double* lane_ops_per_s) {
     if (!lane_ops_per_s) return fail(SIML_EINVAL, "null pointer");
     if (op != 0 && op != 1) return fail(SIML_EINVAL, "unknown operation %d", op);
     siml_ctx* ctx = nullptr;
-    TRY(shared_ctx(device, &ctx));
+    std::unique_lock<std::mutex> lk;
+    TRY(shared_ctx(device, lk, &ctx));
     TRY(set_device(ctx));
     const int blocks = 256 * 4 * 8;  // 8 workgroups of 4 waves per CU
     const int iters = 2048;
@@ -4057,7 +3851,8 @@ int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t*
     if (!mismatches || !first_n || !first_d) return fail(SIML_EINVAL, "null pointer");
     if (method != 1 && method != 2) return fail(SIML_EINVAL, "unknown division method %d", method);
     siml_ctx* ctx = nullptr;
-    TRY(shared_ctx(device, &ctx));
+    std::unique_lock<std::mutex> lk;
+    TRY(shared_ctx(device, lk, &ctx));
     TRY(set_device(ctx));
     TRY(ctx->scratch.reserve(2 * sizeof(unsigned long long)));
     unsigned long long* d = ctx->scratch.as<unsigned long long>();

@@ -257,8 +257,8 @@ def test_ragged_sets_equal_padded(gpu):
     assert_same_f32(got, oracle.tanimoto_block(tuple(x[:300] for x in pad), pad, 0))
 
 
-KERNEL_NAMES = {_lib.POLICY_AUTO: "rows_bits", _lib.POLICY_HASH8: "rows_hash", _lib.POLICY_MERGE: "rows_merge"}
-POLICIES = [_lib.POLICY_AUTO, _lib.POLICY_HASH8, _lib.POLICY_MERGE]
+KERNEL_NAMES = {_lib.POLICY_AUTO: "rows_bits", _lib.POLICY_MERGE: "rows_merge"}
+POLICIES = [_lib.POLICY_AUTO, _lib.POLICY_MERGE]
 
 
 @pytest.mark.parametrize("policy", POLICIES)
@@ -275,12 +275,10 @@ def test_policies_bit_identical(demo_lines, policy):
     assert g._h.last_kernel() == KERNEL_NAMES[policy]
 
 
-@pytest.mark.parametrize("policy", [_lib.POLICY_AUTO, _lib.POLICY_HASH8])
-def test_mixed_hash_and_merge_rows(demo_lines, policy):
+def test_mixed_grouped_and_merge_rows(demo_lines):
     """Rows that cannot use the multi-row kernels (count > 65535, unsorted
     ids, too many ids) are served by the merge kernel in the same call."""
     g = GPULingo(0)
-    g._h.call("siml_set_policy", policy)
     ref = [x.copy() for x in oracle.pack(demo_lines[:100])]
     L, C, n, m = ref
     C[3, 0] = 70000                      # count beyond u16
@@ -300,7 +298,7 @@ def test_mixed_hash_and_merge_rows(demo_lines, policy):
     qry = oracle.pack(demo_lines[:2000])
     _set(g, ref, qry)
     got = g.getMultipleRows(0, 100)
-    assert g._h.last_kernel() == ("rows_bits+merge" if policy == _lib.POLICY_AUTO else "rows_hash+merge")
+    assert g._h.last_kernel() == "rows_bits+merge"
     assert_same_f32(got, oracle.tanimoto_block(ref, qry, nthreads=0))
 
 
@@ -420,15 +418,17 @@ def test_repeated_lingos(demo_lines, policy):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", [1, 2])
-def test_short_division_exhaustive(method):
-    """The short division candidates against IEEE float32 division on every
-    integer numerator in [0, 65536) and denominator in [1, 131072)."""
+def test_short_division_exhaustive():
+    """The short division the bit-sliced kernels use when magnitudes are
+    small and consistent (method 1) equals IEEE float32 division on every
+    integer numerator in [0, 65536) and denominator in [1, 131072): the
+    whole domain of (I, U) those magnitudes allow."""
     import ctypes
     lib = _lib.load()
     bad, fn, fd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-    _lib.check(lib.siml_selftest_division(0, method, ctypes.byref(bad), ctypes.byref(fn), ctypes.byref(fd)))
-    print(f"method {method}: {bad.value} mismatches, first n={fn.value} d={fd.value}", flush=True)
+    _lib.check(lib.siml_selftest_division(0, 1, ctypes.byref(bad), ctypes.byref(fn), ctypes.byref(fd)))
+    print(f"method 1: {bad.value} mismatches, first n={fn.value} d={fd.value}", flush=True)
+    assert bad.value == 0 and fn.value == -1 and fd.value == -1
 
 
 @pytest.mark.parametrize("lo", [0.0, 0.05, 0.6, 0.95])

@@ -67,6 +67,11 @@ def test_batched_equals_per_set(db_and_queries):
     bi, bt = g.getTopK(0, int(bounds[-1]), 30)
     assert np.array_equal(np.concatenate([p[0] for p in per]), bi)
     assert np.array_equal(bits(np.concatenate([p[1] for p in per])), bits(bt))
+    # and the oracle over the whole database, for every row
+    Q = synth.padded_columns(*q, 0, int(bounds[-1]))
+    D = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+    oi, ot = oracle.topk(Q, D, 30)
+    assert np.array_equal(bi, oi) and np.array_equal(bits(bt), bits(ot))
     db.close()
 
 
@@ -133,24 +138,50 @@ def test_serving_admission_budget(db_and_queries):
     db.close()
 
 
-def test_db_save_load_roundtrip(db_and_queries, tmp_path):
+def test_db_file_format_and_load(db_and_queries, tmp_path):
+    """The file siml_db_save writes for a NATURAL-layout database is byte
+    for byte the file tests/db_file_format.py writes from the format's
+    definition; a database loaded from that independent file, and a BLOCK
+    layout database saved and loaded again, give the oracle's results."""
+    import db_file_format as fmt
     db_set, q = db_and_queries
-    db = LingoDB(*db_set, layout=_lib.LAYOUT_BLOCK)
-    path = str(tmp_path / "db.siml")
-    db.save(path)
-    db2 = LingoDB.load(path)
-    assert len(db2) == len(db)
-    g1, g2 = GPULingo(0), GPULingo(0)
-    g1.attach_db(db)
-    g2.attach_db(db2)
-    sub = synth.ragged_slice(*q, 100, 160)
-    g1.set_refsmiles_ragged(*sub)
-    g2.set_refsmiles_ragged(*sub)
-    assert np.array_equal(bits(g1.getMultipleRows(0, 60)), bits(g2.getMultipleRows(0, 60)))
-    a, b = g1.getTopK(0, 60, 10), g2.getTopK(0, 60, 10)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(bits(a[1]), bits(b[1]))
+    ids, counts, offs, mags = db_set
+    n = 3000
+    sub_db = synth.ragged_slice(ids, counts, offs, mags, 0, n)
+    db = LingoDB(*sub_db, layout=_lib.LAYOUT_NATURAL)
+    mine = str(tmp_path / "lib.siml")
+    db.save(mine)
+    spec = str(tmp_path / "spec.siml")
+    fmt.write(spec, *sub_db[:2], sub_db[3], sub_db[2])
+    with open(mine, "rb") as f1, open(spec, "rb") as f2:
+        assert f1.read() == f2.read()
     db.close()
-    db2.close()
+    Q = synth.padded_columns(*q, 100, 160)
+    D = synth.padded_columns(*sub_db, 0, n)
+    want = oracle.tanimoto_block(Q, D, nthreads=0)
+    oi, ot = oracle.topk(Q, D, 10)
+    lists, counts_nb = oracle.neighbors(Q, D, 0.5)
+    for layout, path in ((None, spec), (_lib.LAYOUT_BLOCK, str(tmp_path / "block.siml"))):
+        if layout is not None:
+            b = LingoDB(*db_set, layout=layout)
+            b.save(path)
+            b.close()
+            D2 = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+            want2 = oracle.tanimoto_block(Q, D2, nthreads=0)
+            oi2, ot2 = oracle.topk(Q, D2, 10)
+        else:
+            want2, oi2, ot2 = want, oi, ot
+        loaded = LingoDB.load(path)
+        g = GPULingo(0)
+        g.attach_db(loaded)
+        g.set_refsmiles_ragged(*synth.ragged_slice(*q, 100, 160))
+        assert np.array_equal(bits(g.getMultipleRows(0, 60)), bits(want2))
+        ti, tt = g.getTopK(0, 60, 10)
+        assert np.array_equal(ti, oi2) and np.array_equal(bits(tt), bits(ot2))
+        if layout is None:
+            o, i = g.getNeighborsCSR(0, 60, 0.5)
+            assert np.array_equal(np.diff(o), counts_nb) and np.array_equal(i, np.concatenate(lists))
+        loaded.close()
 
 
 def test_autotune_changes_no_result(db_and_queries):
