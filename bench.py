@@ -1,42 +1,39 @@
 """Benchmark: Tanimoto pairs/s of the LINGO similarity path on one MI355X.
 
-Default workload (one step): the full 32768 x 32768 float32 Tanimoto matrix
-of a seeded synthetic set (strings U[20,120] over 'CNOSPFcnos()=#[]@+-123456',
-seed 2) against itself, rows = columns, via getMultipleRows(0, 32768).  The
-packed set is resident in HBM and the 4 GiB result stays in HBM; no host
-copies are timed.
-
-Prints ONE JSON line.  Besides the contract fields it carries
-  roofline      the dominant kernel's algorithmic integer ops per launch
-                (4 * (La + Lb) per pair) over its mean event-timed duration,
-                against the VALU peak 256 CU * 128 lanes * 2.4 GHz
+Default run (one JSON line):
+  value       the all-pairs workload: the full 32768 x 32768 float32
+              Tanimoto matrix of seeded strings (U[20,120] over
+              'CNOSPFcnos()=#[]@+-123456', seed 2) against themselves via
+              getMultipleRows(0, 32768); packed set and 4 GiB result resident
+              in HBM, no host copies timed.  One step = one matrix.
+  roofline    the all-pairs kernel against HBM (algorithmic bytes per
+              launch over its mean event-timed duration, peak 8 TB/s) and,
+              beside it, against its limiter, the LDS table probes (bytes
+              the probes read over the same duration, against the LDS rate
+              siml_selftest_lds measures on this device)
+  batched     the batched-uneven-sets workload: 1M-molecule resident
+              database (lognormal lengths, median 45, seed 3), 2000 query
+              sets of geometric(1/128) sizes clipped to [1, 4096], top-100 per
+              query molecule; "plain" = one call per set, "batched" = all sets
+              in one call; pairs/s of each, their ratio, and the similarity
+              kernel's roofline
+  serving     the serving loop on a 10M-molecule resident database: Poisson
+              arrivals of query batches (geometric(1/32) sizes clipped to
+              [1, 512]), half top-50, half threshold 0.8, continuous batching
+              over 2 slots; p50/p99 latency and pairs/s per arrival rate
   cpu_baseline  the CPU oracle (C restatement of _CPULingo.c, OpenMP over
-                rows) timed on a bounded sample of the same workload
+              rows) on a bounded sample of the all-pairs workload, on the
+              host threads allotted to this process and on one core
 
-Other workloads (--workload), each one JSON line:
-  zinc       real SMILES: the reference's examples/demo4096.smi (4096 ZINC
-             molecules) cycled to --n molecules, all-pairs float32 matrix and
-             top-100, checked bit-exact on sampled rows.
-  cpu4096    the reference's CPU case through the CPULingo interface: 4096
-             strings U[20,80] (seed 1), the 4096 x 4096 matrix in 256-row
-             blocks with the reference's default magnitudes, checked
-             bit-exact against the CPU oracle over the full matrix.
-  batched    1M-molecule resident database (lognormal lengths, median 45,
-             sigma 0.35, clipped [8, 256], seed 3); 2000 query sets with
-             geometric(1/128) sizes clipped [1, 4096]; top-100 per query
-             molecule.  "plain" runs one getTopK per set, "batched" runs all
-             sets as one launch sequence; both end with results on the host.
-  threshold  10M-molecule database (same law, seed 4) x 1024 queries that
-             are near duplicates of database molecules; complete CSR
-             neighbour lists for T >= 0.7 and the 1000-bin int64 histogram of
-             all 1.024e10 similarities.
-  serve      10M-molecule database; Poisson arrivals of query batches (sizes
-             geometric(1/32) clipped [1, 512], lognormal lengths), half top-50
-             and half threshold 0.8, continuous batching over 4 slots;
-             latency p50/p99 and throughput per arrival rate; value is the
-             best throughput among the rates whose p99 stays below 50 ms.
+--quick leaves out the batched and serving parts.  Other workloads
+(--workload, each one JSON line): zinc (the reference's demo4096.smi cycled
+to 32768), cpu4096 (the reference's CPU case through CPULingo), batched,
+threshold (10M database x 1024 near-duplicate queries: CSR for T >= 0.7 and
+the 1000-bin int64 histogram), serve.
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -47,28 +44,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X vector peak: 256 CUs x 128 lanes per clock (4 SIMDs, a wave64
-# instruction every 2 cycles; the 157.3 TFLOP/s FP32 vector rate is these
-# lanes doing FMA) x 2.4 GHz = 7.86e13 int32 lane-ops/s.  siml_selftest_valu
-# measures 6.7e13 with independent v_add_u32 chains.
-P_VALU = 256 * 128 * 2.4e9
-B_HBM = 8.0e12                     # bytes/s
+B_HBM = 8.0e12                     # bytes/s, MI355X HBM3E
 N_ALLPAIRS = 32768
-# HBM bytes per launch of the default all-pairs kernel, from rocprofv3 counter
-# runs of this workload: FETCH_SIZE 648,849 KiB x 2.0 + WRITE_SIZE
-# 4,194,304 KiB x 1.0 (the matrix exactly), the factors being what the same
-# counters read for a device copy of known size
-# (profiles/k_bits_f32block_allpairs32768.md).
-TRAFFIC_ALLPAIRS = (2.0 * 648849 + 1.0 * 4194304) * 1024
-# VALU issue utilisation of the same kernel from SQ_INSTS_VALU: 7.964e8
-# wave-instructions x 2 cycles / (1024 SIMDs x 2.4 GHz x its duration).
-VALU_INSTS_ALLPAIRS = 7.964e8
-# LDS busy fraction of the same kernel from its counter run: SQ_LDS_IDX_ACTIVE
-# 7.269e8 CU-cycles / (256 CUs x 3.915e6 cycles of GRBM_GUI_ACTIVE per XCC);
-# SQ_LDS_BANK_CONFLICT accounts for 4.605e8 of those cycles.  The table reads
-# (their bank conflicts and latency) bound this kernel, not the VALU.
-LDS_BUSY_ALLPAIRS = 7.269e8 / (256 * (31318790 / 8))
-LDS_CONFLICT_ALLPAIRS = 4.605e8 / (256 * (31318790 / 8))
+MAX_LEVEL = 15                     # levels per Lingo in the bit-sliced kernels
 
 
 def log(msg):
@@ -84,6 +62,73 @@ def cpu_model():
     except OSError:
         pass
     return None
+
+
+def host_threads():
+    """Host threads allotted to this process: OMP_NUM_THREADS when set (the
+    per-GPU allotment of a shared machine), else every visible core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def sources_sha():
+    """Hash of the library's sources: recorded counter figures apply to the
+    build whose sources hash the same."""
+    h = hashlib.sha256()
+    for p in sorted(glob.glob(os.path.join(ROOT, "siml_amd", "csrc", "*"))):
+        with open(p, "rb") as f:
+            h.update(os.path.basename(p).encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def recorded_counters(name):
+    """The `counters:` JSON line of profiles/<name>.md (figures from
+    rocprofv3 counter runs), or None."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("counters: "):
+                    rec = json.loads(line[len("counters: "):])
+                    rec["file"] = os.path.relpath(path, ROOT)
+                    return rec
+    except OSError:
+        pass
+    return None
+
+
+def levels_padded(lingos, counts, lengths):
+    """Levels per molecule of a padded set: sum over its ASCII Lingos of
+    min(count, 15), the column levels the bit-sliced kernels probe."""
+    w = lingos.shape[1]
+    live = np.arange(w)[None, :] < lengths.reshape(-1, 1)
+    ascii_ = (lingos.astype(np.int64) & 0x80808080) == 0
+    c = np.where(live & ascii_ & (counts > 0), np.minimum(counts, MAX_LEVEL), 0)
+    return c.sum(axis=1).astype(np.int64)
+
+
+def levels_ragged(ids, counts, offsets):
+    n = len(offsets) - 1
+    ids, counts = ids[:offsets[-1]], counts[:offsets[-1]]
+    ascii_ = (ids.astype(np.int64) & 0x80808080) == 0
+    c = np.where(ascii_ & (counts > 0), np.minimum(counts, MAX_LEVEL), 0).astype(np.int64)
+    cs = np.zeros(len(c) + 1, np.int64)
+    np.cumsum(c, out=cs[1:])
+    return cs[offsets[1:]] - cs[offsets[:-1]] if n else np.zeros(0, np.int64)
+
+
+def device_peaks():
+    """Measured LDS read rates (bytes/s) of this device: 8-byte reads from a
+    32 KiB table without bank conflicts and at random slots."""
+    import ctypes
+    from siml_amd import _lib
+    lib = _lib.load()
+    out = {}
+    for op, name in ((0, "lds_conflict_free"), (1, "lds_random_slots")):
+        v = ctypes.c_double()
+        _lib.check(lib.siml_selftest_lds(0, op, ctypes.byref(v)))
+        out[name] = v.value
+    return out
 
 
 def _oracle_rate(s, threads, budget_s):
@@ -102,16 +147,19 @@ def _oracle_rate(s, threads, budget_s):
 
 
 def cpu_baseline(s, budget_s=10.0):
-    """The oracle's C restatement of _CPULingo.c timed on this host: all
-    cores up to 16 (the reported value), and one core."""
-    threads = min(os.cpu_count() or 1, 16)
+    """The oracle's C restatement of _CPULingo.c timed on this host: on the
+    host threads allotted to this process (the reported value), and on one
+    core."""
+    threads = host_threads()
     rows, dt = _oracle_rate(s, threads, budget_s)
     rows1, dt1 = _oracle_rate(s, 1, budget_s / 4)
     n = len(s[0])
     return {"value": rows * n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"{rows} rows x {n} columns of the same workload, {dt:.1f} s",
             "seconds": round(dt, 3), "value_1core": rows1 * n / dt1,
-            "sample_1core": f"{rows1} rows x {n} columns, {dt1:.1f} s", "cpu_model": cpu_model()}
+            "sample_1core": f"{rows1} rows x {n} columns, {dt1:.1f} s", "cpu_model": cpu_model(),
+            "cores_visible": os.cpu_count(),
+            "thread_policy": "OMP_NUM_THREADS when set (the host-thread allotment of this GPU), else all cores"}
 
 
 def copy_calibration(nbytes):
@@ -152,23 +200,53 @@ def base_record(args, metric, value, unit, wall, config, dtype="int32 intersecti
             "data": "synthetic (seeded SMILES-like strings, packed with the native packer)", "config": config}
 
 
+def roofline(alg_bytes, probe_bytes, kernel_s, peaks, counters=None):
+    """Roofline object of one kernel: HBM (algorithmic bytes per launch over
+    its duration, peak 8 TB/s; `traffic` = measured HBM bytes per launch
+    from counter runs of the same sources) and the LDS table probes (bytes
+    read per launch over the same duration, against the measured rates)."""
+    sha = sources_sha()
+    traffic, note = None, None
+    if counters is not None:
+        if counters.get("sources_sha") == sha:
+            traffic = counters["fetch_kib"] * 1024 * counters["fetch_factor"] + \
+                counters["write_kib"] * 1024 * counters["write_factor"]
+        else:
+            note = f"{counters['file']} was measured on sources {counters.get('sources_sha')}, not {sha}"
+    achieved = alg_bytes / kernel_s
+    lds = probe_bytes / kernel_s
+    rec = {"bound": "hbm", "achieved": achieved / 1e9, "peak": B_HBM / 1e9, "unit": "GB/s",
+           "frac": achieved / B_HBM, "traffic": traffic,
+           "alg_bytes_per_launch": alg_bytes, "kernel_ms": kernel_s * 1e3,
+           "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
+           "lds": {"achieved": lds / 1e9, "unit": "GB/s", "probe_bytes_per_launch": probe_bytes,
+                   "peak_conflict_free": peaks["lds_conflict_free"] / 1e9,
+                   "frac": lds / peaks["lds_conflict_free"],
+                   "peak_random_slots": peaks["lds_random_slots"] / 1e9,
+                   "frac_of_random_slot_rate": lds / peaks["lds_random_slots"]},
+           "sources_sha": sha}
+    if counters is not None and counters.get("sources_sha") == sha:
+        rec["counters"] = {k: v for k, v in counters.items() if k not in ("fetch_kib", "write_kib")}
+    if note:
+        rec["traffic_note"] = note
+    return rec
+
+
 # ---------------------------------------------------------------------------
-# all-pairs 32768 x 32768 (default)
+# all-pairs 32768 x 32768 (the value of the default line)
 # ---------------------------------------------------------------------------
 def workload_allpairs(args):
-    from siml_amd import synth
+    from siml_amd import _lib, synth
     from siml_amd.GPULingo import GPULingo
 
     n = args.n
     log(f"packing {n} synthetic strings (U[20,120], seed 2)")
     s = synth.packed_padded(n, 20, 120, seed=2)
-    lens = s[2].ravel().astype(np.int64)
     g = GPULingo(0)
     h = g._h
     configure(g, args)
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
-    from siml_amd import _lib
 
     for i in range(args.warmup):
         h.call("siml_multirow_async", 0, n)
@@ -182,46 +260,39 @@ def workload_allpairs(args):
         h.call("siml_multirow_async", 0, n)
     h.call("siml_synchronize")
     wall = time.perf_counter() - t0
-    kms, launches, kpairs = h.kernel_stats(reset=True)
+    kms, launches, _ = h.kernel_stats(reset=True)
     kernel = h.last_kernel()
     log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms wall, kernel {kms:.1f} ms over {launches} launches")
 
-    pairs_per_step = n * n
-    value = pairs_per_step * args.steps / wall
+    value = n * n * args.steps / wall
     kernel_s = kms / 1e3 / max(launches, 1)
-    ops = 4.0 * (2 * n * lens.sum())            # sum over pairs of 4*(La+Lb), rows == columns
-    bytes_alg = 4.0 * n * n + 8.0 * 2 * lens.sum() + 16.0 * n
-    achieved = ops / kernel_s
-    traffic = args.traffic
-    if traffic is None and n == N_ALLPAIRS and args.policy == "auto" and args.layout == "block":
-        traffic = TRAFFIC_ALLPAIRS
+    lv = levels_padded(s[0], s[1], s[2])
+    groups, merged = h.row_groups()
+    # algorithmic HBM bytes: the float32 matrix written once, the column
+    # level stream read once (8 B per level), the row tables (16 B per table
+    # entry: at most one per row level) and 8 B of per-column vectors
+    alg_bytes = 4.0 * n * n + 8.0 * lv.sum() + 16.0 * lv.sum() + 8.0 * n
+    probe_bytes = 16.0 * groups * lv.sum()
+    peaks = device_peaks()
+    counters = recorded_counters("k_bits_f32block_allpairs32768") if (
+        n == N_ALLPAIRS and args.policy == "auto" and args.layout == "block") else None
     rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact)", value, "pairs/s", wall,
                       {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
                        "rows": n, "columns": n, "seed": 2, "policy": args.policy, "layout": args.layout})
-    rec.update({
-        "kernel": kernel,
-        "kernel_ms_mean": kernel_s * 1e3,
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": P_VALU / 1e12,
-                     "unit": "Tops/s (int32 lane-ops)", "frac": achieved / P_VALU, "traffic": traffic,
-                     "alg_ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
-                     "hbm_frac": bytes_alg / kernel_s / B_HBM,
-                     "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
-                     "valu_issue_frac": (VALU_INSTS_ALLPAIRS * 2 / (1024 * 2.4e9 * kernel_s)) if traffic else None,
-                     "lds_busy_frac": LDS_BUSY_ALLPAIRS if traffic else None,
-                     "lds_bank_conflict_frac": LDS_CONFLICT_ALLPAIRS if traffic else None,
-                     "limiter": "LDS table reads (bank conflicts and latency)" if traffic else None},
-    })
+    rec.update({"kernel": kernel, "kernel_ms_mean": kernel_s * 1e3, "row_groups": groups, "merge_rows": merged,
+                "roofline": roofline(alg_bytes, probe_bytes, kernel_s, peaks, counters)})
     if not args.no_cpu:
         import oracle
         rows = [0, 1, n // 3, n // 2, n - 2, n - 1]
         peek = np.empty((len(rows), n), np.float32)
         for i, r in enumerate(rows):
             h.call("siml_peek_rows_f32", r, r + 1, _lib.f32p(peek[i:i + 1]), n)
-        want = oracle.tanimoto_block(tuple(x[rows] for x in s), s, nthreads=min(os.cpu_count() or 1, 16))
+        want = oracle.tanimoto_block(tuple(x[rows] for x in s), s, nthreads=host_threads())
         rec["parity"] = {"rows_checked": rows, "bit_exact": bool(np.array_equal(peek.view(np.int32),
                                                                                  want.view(np.int32)))}
         log("timing the CPU oracle on a bounded sample")
         rec["cpu_baseline"] = cpu_baseline(s, args.cpu_budget)
+    del g
     return rec
 
 
@@ -263,11 +334,11 @@ def workload_zinc(args):
     peek = np.empty((len(rows), n), np.float32)
     for i, r in enumerate(rows):
         h.call("siml_peek_rows_f32", r, r + 1, _lib.f32p(peek[i:i + 1]), n)
-    want = oracle.tanimoto_block(tuple(x[rows] for x in s), s, nthreads=min(os.cpu_count() or 1, 16))
+    want = oracle.tanimoto_block(tuple(x[rows] for x in s), s, nthreads=host_threads())
     t0 = time.perf_counter()
     ti, tt = g.getTopK(0, n, 100)
     t_topk = time.perf_counter() - t0
-    oi, ot = oracle.topk(tuple(x[rows] for x in s), s, 100, nthreads=min(os.cpu_count() or 1, 16))
+    oi, ot = oracle.topk(tuple(x[rows] for x in s), s, 100, nthreads=host_threads())
     rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), real SMILES", n * n * args.steps / wall,
                       "pairs/s", wall, {"workload": f"all-pairs {n} x {n} of the reference's demo4096.smi ZINC "
                                                     f"molecules cycled, full float32 matrix in HBM", "rows": n,
@@ -314,7 +385,7 @@ def workload_cpu4096(args):
                                           "columns": n, "seed": 1})
     if not args.no_cpu:
         q = (L, C, lens, np.int32(lens.sum(1)))
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_threads()
         t0 = time.perf_counter()
         want = oracle.tanimoto_block(q, q, nthreads=threads)
         dt = time.perf_counter() - t0
@@ -325,7 +396,7 @@ def workload_cpu4096(args):
         rec["cpu_baseline"] = {"value": n * n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
                                "sample": f"the full {n} x {n} matrix, {dt:.2f} s", "seconds": round(dt, 3),
                                "value_1core": rows1 * n / dt1, "sample_1core": f"{rows1} rows x {n} columns, {dt1:.2f} s",
-                               "cpu_model": cpu_model()}
+                               "cpu_model": cpu_model(), "cores_visible": os.cpu_count()}
         rec["bit_exact"] = bool(np.array_equal(got.view(np.int32), want.view(np.int32)))
     return rec
 
@@ -351,7 +422,7 @@ def oracle_topk_ragged(q, db, rows, k, chunk=1 << 20):
     for c0 in range(0, n, chunk):
         c1 = min(n, c0 + chunk)
         D = synth.padded_columns(*db, c0, c1)
-        oi, ot = oracle.topk(Q, D, min(k, c1 - c0), nthreads=min(os.cpu_count() or 1, 16))
+        oi, ot = oracle.topk(Q, D, min(k, c1 - c0), nthreads=host_threads())
         best_i = np.concatenate([best_i, oi + c0], axis=1)
         best_t = np.concatenate([best_t, ot], axis=1)
         order = np.lexsort((best_i, -best_t.astype(np.float64)), axis=1)[:, :k]
@@ -360,17 +431,18 @@ def oracle_topk_ragged(q, db, rows, k, chunk=1 << 20):
     return best_i, best_t
 
 
-def workload_batched(args):
+def batched_part(args, check_oracle=False, peaks=None):
+    """Plain (one call per set) and batched (one call) top-100 over the 1M
+    database; both end with the results on the host."""
     from siml_amd import synth
     from siml_amd.GPULingo import GPULingo, LingoDB
 
     ndb = args.db or 1_000_000
     nsets, k = args.sets, 100
-    log(f"packing the {ndb}-molecule database (lognormal, seed 3)")
+    log(f"batched: packing the {ndb}-molecule database (lognormal, seed 3)")
     db_set = lognormal_set(ndb, 3)
     sizes = synth.geometric_sizes(nsets, 1.0 / 128, 1, 4096, seed=3)
     nq = int(sizes.sum())
-    log(f"packing {nq} query molecules in {nsets} sets")
     q = lognormal_set(nq, 33)
     bounds = np.zeros(nsets + 1, np.int64)
     np.cumsum(sizes, out=bounds[1:])
@@ -384,7 +456,7 @@ def workload_batched(args):
         g.set_autotune(True)
     # warm-up (and, with the autotuner, shape tuning): every set once and
     # the batched call once
-    log("warm-up pass over the sets and the batched call")
+    log("batched: warm-up pass over the sets and the batched call")
     for s in sets:
         g.set_refsmiles_ragged(*s)
         g.getTopK(0, len(s[2]) - 1, k)
@@ -394,60 +466,74 @@ def workload_batched(args):
 
     h.call("siml_set_timing", 1)
     h.kernel_stats(reset=True)
-    log("plain: one getTopK per query set")
+    log("batched: plain, one getTopK per query set")
     t0 = time.perf_counter()
     plain = []
     for s in sets:
         g.set_refsmiles_ragged(*s)
         plain.append(g.getTopK(0, len(s[2]) - 1, k))
     t_plain = time.perf_counter() - t0
+    sim_plain, sel_plain = h.kernel_split()
     kms_plain, n_plain, _ = h.kernel_stats(reset=True)
-    log(f"plain: {t_plain:.2f} s wall, kernels {kms_plain:.0f} ms over {n_plain} launches")
 
     log("batched: all sets as one launch sequence")
     t0 = time.perf_counter()
     g.set_refsmiles_ragged(*q)
     bi, bt = g.getTopK(0, nq, k)
     t_batched = time.perf_counter() - t0
+    sim_b, sel_b = h.kernel_split()
     kms_b, n_b, _ = h.kernel_stats(reset=True)
     kernel = h.last_kernel()
     shape = g.launch_shape()
-    log(f"batched: {t_batched:.2f} s wall, kernels {kms_b:.0f} ms over {n_b} launches ({kernel})")
+    groups, merged = h.row_groups()
+    log(f"batched: plain {t_plain:.2f} s ({kms_plain:.0f} ms kernels), batched {t_batched:.2f} s "
+        f"({kms_b:.0f} ms kernels: {sim_b:.0f} similarity + {sel_b:.0f} selection)")
 
     pi = np.concatenate([p[0] for p in plain])
     pt = np.concatenate([p[1] for p in plain])
     same = bool(np.array_equal(pi, bi) and np.array_equal(pt.view(np.int32), bt.view(np.int32)))
-    parity = None
-    if not args.no_cpu:
+    out = {"workload": f"{nsets} query sets (geometric 1/128 sizes clipped [1, 4096], {nq} molecules) x "
+                       f"{ndb}-molecule resident database (lognormal lengths, median 45), top-{k} per query "
+                       f"molecule", "sets": nsets, "queries": nq, "database": ndb, "k": k, "seed": 3,
+           "pairs": pairs,
+           "plain": {"wall_s": t_plain, "pairs_per_s": pairs / t_plain, "kernel_ms": kms_plain,
+                     "similarity_ms": sim_plain, "selection_ms": sel_plain, "calls": nsets},
+           "batched": {"wall_s": t_batched, "pairs_per_s": pairs / t_batched, "kernel_ms": kms_b,
+                       "similarity_ms": sim_b, "selection_ms": sel_b, "calls": 1},
+           "speedup_wall": t_plain / t_batched, "speedup_kernel": kms_plain / max(kms_b, 1e-9),
+           "speedup_similarity_kernel": sim_plain / max(sim_b, 1e-9),
+           "plain_equals_batched": same, "kernel": kernel, "autotune": bool(args.autotune),
+           "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1],
+                                    "topk_seed": shape[2]}}
+    if peaks is not None:
+        lv_db = levels_ragged(*db_set[:3])
+        lv_q = levels_ragged(*q[:3])
+        # algorithmic HBM bytes of the batched similarity kernel: the database
+        # level stream once (8 B per level), the query tables (16 B per
+        # entry) and 8 B per candidate key for k keys per query row
+        alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8.0 * k * nq
+        out["row_groups"] = groups
+        out["merge_rows"] = merged
+        out["roofline"] = roofline(alg, 16.0 * groups * lv_db.sum(), sim_b / 1e3, peaks,
+                                   recorded_counters("k_bits_topk_batched1M"))
+    if check_oracle:
         rng = np.random.default_rng(7)
         rows = np.sort(rng.choice(nq, size=min(6, nq), replace=False))
-        log(f"oracle top-{k} of query rows {rows.tolist()} over the whole database")
+        log(f"batched: oracle top-{k} of query rows {rows.tolist()} over the whole database")
         oi, ot = oracle_topk_ragged(q, db_set, rows, k)
-        parity = bool(np.array_equal(oi, bi[rows]) and np.array_equal(ot.view(np.int32), bt[rows].view(np.int32)))
-    lens_db = np.diff(db_set[2]).astype(np.int64)
-    lens_q = np.diff(q[2]).astype(np.int64)
-    ops = 4.0 * (lens_q.sum() * ndb + lens_db.sum() * nq)
-    kernel_s = kms_b / 1e3
-    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), batched top-k", pairs / t_batched, "pairs/s",
-                      t_batched, {"workload": f"batched uneven query sets: {nsets} sets (geometric 1/128, clipped "
-                                              f"[1, 4096], {nq} molecules) x {ndb}-molecule resident database, "
-                                              f"top-{k}", "sets": nsets, "queries": nq, "database": ndb, "k": k,
-                                  "seed": 3}, steps=1, warmup=1)
-    rec.update({
-        "plain": {"wall_s": t_plain, "pairs_per_s": pairs / t_plain, "kernel_ms": kms_plain, "launches": n_plain},
-        "batched": {"wall_s": t_batched, "pairs_per_s": pairs / t_batched, "kernel_ms": kms_b, "launches": n_b},
-        "speedup_wall": t_plain / t_batched,
-        "speedup_kernel": kms_plain / max(kms_b, 1e-9),
-        "plain_equals_batched": same,
-        "oracle_rows_match": parity,
-        "kernel": kernel,
-        "autotune": bool(args.autotune),
-        "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1],
-                                 "topk_seed": shape[2]},
-        "roofline": {"bound": "valu", "achieved": ops / kernel_s / 1e12, "peak": P_VALU / 1e12,
-                     "unit": "Tops/s (int32 lane-ops)", "frac": ops / kernel_s / P_VALU, "traffic": None},
-    })
+        out["oracle_rows_match"] = bool(np.array_equal(oi, bi[rows]) and
+                                        np.array_equal(ot.view(np.int32), bt[rows].view(np.int32)))
+    del g
     db.close()
+    return out
+
+
+def workload_batched(args):
+    part = batched_part(args, check_oracle=not args.no_cpu, peaks=device_peaks())
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), batched top-k", part["batched"]["pairs_per_s"],
+                      "pairs/s", part["batched"]["wall_s"],
+                      {"workload": "batched uneven query sets: " + part["workload"]}, steps=1, warmup=1)
+    rec.update(part)
     return rec
 
 
@@ -455,10 +541,9 @@ def workload_batched(args):
 # threshold neighbours + global histogram
 # ---------------------------------------------------------------------------
 def near_duplicate_queries(nq, db_buf, db_offs, seed):
-    from siml_amd import compiler
+    from siml_amd import compiler, synth
     rng = np.random.default_rng(seed)
     picks = rng.choice(len(db_offs) - 1, size=nq, replace=False)
-    from siml_amd import synth
     buf, offs = synth.mutated_strings(db_buf, db_offs, picks, 2, seed)
     return compiler.pack_ragged_buffer(buf, offs)
 
@@ -483,6 +568,7 @@ def workload_threshold(args):
     g.getNeighborsCSR(0, 32, lo)
     g.getGlobalHistogram(0, 32)
     pairs = nq * ndb
+    peaks = device_peaks()
     h.call("siml_set_timing", 1)
     h.kernel_stats(reset=True)
     h.work_stats(reset=True)
@@ -492,9 +578,8 @@ def workload_threshold(args):
     t_csr = (time.perf_counter() - t0) / args.steps
     kms_csr, n_csr, _ = h.kernel_stats(reset=True)
     k_csr = h.last_kernel()
-    # merge-model work of the pairs the magnitude band did not prune, counted
-    # on the device (summed over the count pass and, when one runs, the fill
-    # pass: both evaluate the same pairs)
+    # pairs the magnitude band did not prune, counted on the device (summed
+    # over the count pass and, when one runs, the fill pass)
     ev_pairs, ev_len = h.work_stats(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -502,13 +587,11 @@ def workload_threshold(args):
     t_hist = (time.perf_counter() - t0) / args.steps
     kms_hist, n_hist, _ = h.kernel_stats(reset=True)
     k_hist = h.last_kernel()
+    groups, merged = h.row_groups()
     log(f"CSR: {t_csr * 1e3:.1f} ms ({int(roff[-1])} neighbours); hist: {t_hist * 1e3:.1f} ms")
-    q_len = int(np.diff(q[2]).sum())
-    db_len = int(np.diff(db_set[2]).sum())
-    hist_ops = 4.0 * (nq * db_len + ndb * q_len)   # every pair: sum of 4 (La + Lb)
-    csr_s = kms_csr / 1e3 / args.steps
+    lv_db = levels_ragged(*db_set[:3])
+    lv_q = levels_ragged(*q[:3])
     hist_s = kms_hist / 1e3 / args.steps
-    csr_ops = 4.0 * ev_len / args.steps
     checks = {"hist_total": int(hist.sum()), "hist_total_ok": int(hist.sum()) == pairs, "nnz": int(roff[-1])}
     if not args.no_cpu:
         rows = [0, 1, nq // 2, nq - 1]
@@ -519,7 +602,7 @@ def workload_threshold(args):
         for c0 in range(0, ndb, 1 << 20):
             c1 = min(ndb, c0 + (1 << 20))
             D = synth.padded_columns(*db_set, c0, c1)
-            lst, _ = oracle.neighbors(Q, D, lo, nthreads=min(os.cpu_count() or 1, 16))
+            lst, _ = oracle.neighbors(Q, D, lo, nthreads=host_threads())
             for i, l in enumerate(lst):
                 nb[i].append(l + c0)
             oh += oracle.hist1000(Q, D)
@@ -532,19 +615,18 @@ def workload_threshold(args):
                       {"workload": f"threshold neighbours T >= {lo} + 1000-bin histogram: {nq} near-duplicate "
                                    f"queries x {ndb}-molecule resident database", "queries": nq, "database": ndb,
                        "seed": 4})
+    # the histogram evaluates every pair: its probes are every group against
+    # every database level; the neighbour search prunes tiles by magnitude
+    hist_alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8000.0
     rec.update({"csr": {"wall_ms": t_csr * 1e3, "pairs_per_s": pairs / t_csr, "kernel_ms": kms_csr / args.steps,
                         "kernel": k_csr, "evaluated_pairs_per_call": ev_pairs / args.steps,
-                        "evaluated_frac": ev_pairs / args.steps / pairs,
-                        "roofline": {"bound": "valu", "achieved": csr_ops / csr_s / 1e12, "peak": P_VALU / 1e12,
-                                     "unit": "Tops/s (int32 lane-ops, evaluated pairs only)",
-                                     "frac": csr_ops / csr_s / P_VALU, "traffic": None}},
+                        "evaluated_frac": ev_pairs / args.steps / pairs},
                 "hist1000": {"wall_ms": t_hist * 1e3, "pairs_per_s": pairs / t_hist,
                              "kernel_ms": kms_hist / args.steps, "kernel": k_hist,
-                             "roofline": {"bound": "valu", "achieved": hist_ops / hist_s / 1e12,
-                                          "peak": P_VALU / 1e12, "unit": "Tops/s (int32 lane-ops)",
-                                          "frac": hist_ops / hist_s / P_VALU, "traffic": None}},
+                             "roofline": roofline(hist_alg, 16.0 * groups * lv_db.sum(), hist_s, peaks,
+                                                  recorded_counters("k_bits_hist1000_db10M"))},
                 "checks": checks})
-    rec["roofline"] = rec["csr"]["roofline"]
+    rec["roofline"] = rec["hist1000"]["roofline"]
     db.close()
     return rec
 
@@ -552,28 +634,28 @@ def workload_threshold(args):
 # ---------------------------------------------------------------------------
 # serving loop
 # ---------------------------------------------------------------------------
-def workload_serve(args):
+def serving_part(args, rates, requests_for):
+    """The serving loop at each arrival rate: p50/p99 latency and pairs/s."""
     from siml_amd import synth
     from siml_amd.GPULingo import LingoDB
     from siml_amd.serving import Request, Server, run_poisson
 
-    ndb = args.db or 10_000_000
-    log(f"packing the {ndb}-molecule database (lognormal, seed 5)")
+    ndb = args.db_serve or 10_000_000
+    log(f"serving: packing the {ndb}-molecule database (lognormal, seed 5)")
     db_set = lognormal_set(ndb, 5)
     db = LingoDB(*db_set)
-    nreq = args.requests
-    sizes = synth.geometric_sizes(nreq, 1.0 / 32, 1, 512, seed=5)
+    nmax = max(requests_for(r) for r in rates)
+    sizes = synth.geometric_sizes(nmax, 1.0 / 32, 1, 512, seed=5)
     pool = lognormal_set(int(sizes.sum()), 55)
-    bounds = np.zeros(nreq + 1, np.int64)
+    bounds = np.zeros(nmax + 1, np.int64)
     np.cumsum(sizes, out=bounds[1:])
-    rates = [float(r) for r in args.rates.split(",")]
     results = []
     with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30,
                 autotune=args.autotune) as srv:
         # warm-up: one request of each kind per power-of-two size up to the
         # batch limit (with the autotuner this also fixes every shape bucket
         # before the timed arrivals)
-        log("warm-up requests" + (" (tuning launch shapes)" if args.autotune else ""))
+        log("serving: warm-up requests" + (" (tuning launch shapes)" if args.autotune else ""))
         n = 1
         while n <= args.max_rows:
             for op, prm in (("topk", 50), ("threshold", 0.8)):
@@ -584,6 +666,7 @@ def workload_serve(args):
                     raise r.error
             n *= 2
         for rate in rates:
+            nreq = requests_for(rate)
             reqs = []
             for i in range(nreq):
                 s = synth.ragged_slice(*pool, int(bounds[i]), int(bounds[i + 1]))
@@ -594,24 +677,48 @@ def workload_serve(args):
             errs = [r.error for r in reqs if r.error is not None]
             if errs:
                 raise errs[0]
-            pairs = int(sizes.sum()) * ndb
+            pairs = int(sizes[:nreq].sum()) * ndb
             results.append({"rate_per_s": rate, "requests": nreq, "wall_s": wall, "batches": srv.batches - b0,
                             "p50_ms": float(np.percentile(lat, 50) * 1e3),
                             "p99_ms": float(np.percentile(lat, 99) * 1e3),
                             "max_ms": float(lat.max() * 1e3), "pairs_per_s": pairs / wall,
                             "requests_per_s": nreq / wall})
-            log(f"rate {rate}/s: p50 {results[-1]['p50_ms']:.1f} ms, p99 {results[-1]['p99_ms']:.1f} ms, "
+            log(f"serving: rate {rate}/s: p50 {results[-1]['p50_ms']:.1f} ms, p99 {results[-1]['p99_ms']:.1f} ms, "
                 f"{results[-1]['pairs_per_s']:.3g} pairs/s")
     ok = [r for r in results if r["p99_ms"] < 50.0]
     best = max(ok or results, key=lambda r: r["pairs_per_s"])
-    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), serving loop", best["pairs_per_s"], "pairs/s",
-                      best["wall_s"], {"workload": f"serving: {ndb}-molecule resident database, Poisson arrivals "
-                                                   f"of query batches (geometric 1/32 sizes clipped [1, 512]), "
-                                                   f"half top-50, half threshold 0.8", "database": ndb,
-                                       "requests": nreq, "slots": args.slots, "seed": 5}, steps=1, warmup=1)
-    rec.update({"rates": results, "best_rate_with_p99_below_50ms": best["rate_per_s"]
-                if ok else None})
     db.close()
+    return {"workload": f"{ndb}-molecule resident database, Poisson arrivals of query batches (geometric 1/32 "
+                        f"sizes clipped [1, 512], lognormal lengths), half top-50, half threshold 0.8",
+            "database": ndb, "slots": args.slots, "max_rows": args.max_rows, "seed": 5, "rates": results,
+            "best_rate_with_p99_below_50ms": best["rate_per_s"] if ok else None,
+            "best_pairs_per_s_with_p99_below_50ms": best["pairs_per_s"] if ok else None}
+
+
+def workload_serve(args):
+    rates = [float(r) for r in args.rates.split(",")]
+    part = serving_part(args, rates, lambda r: args.requests)
+    best = max(part["rates"], key=lambda r: r["pairs_per_s"] if r["p99_ms"] < 50 else -1)
+    rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact), serving loop", best["pairs_per_s"], "pairs/s",
+                      best["wall_s"], {"workload": "serving: " + part["workload"]}, steps=1, warmup=1)
+    rec.update(part)
+    return rec
+
+
+def workload_default(args):
+    """The all-pairs line, with the batched and serving parts unless
+    --quick."""
+    rec = workload_allpairs(args)
+    if args.quick:
+        return rec
+    peaks = device_peaks()
+    t0 = time.perf_counter()
+    rec["batched"] = batched_part(args, peaks=peaks)
+    log(f"batched part: {time.perf_counter() - t0:.0f} s")
+    t0 = time.perf_counter()
+    rec["serving"] = serving_part(args, [30.0, 1000.0, 2000.0, 2500.0],
+                                  lambda r: int(min(600, max(120, 3 * r))))
+    log(f"serving part: {time.perf_counter() - t0:.0f} s")
     return rec
 
 
@@ -619,13 +726,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
-    ap.add_argument("--workload", choices=["allpairs", "zinc", "cpu4096", "batched", "threshold", "serve"],
-                    default="allpairs")
+    ap.add_argument("--workload", choices=["default", "allpairs", "zinc", "cpu4096", "batched", "threshold", "serve"],
+                    default="default")
+    ap.add_argument("--quick", action="store_true", help="default workload: the all-pairs line only")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_ALLPAIRS, help="all-pairs: molecules (rows = columns)")
-    ap.add_argument("--db", type=int, default=0, help="database molecules (batched/threshold/serve)")
+    ap.add_argument("--db", type=int, default=0, help="database molecules (batched/threshold)")
+    ap.add_argument("--db-serve", type=int, default=0, help="database molecules of the serving loop")
     ap.add_argument("--sets", type=int, default=2000, help="batched: query sets")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per arrival rate")
     ap.add_argument("--rates", default="30,300,1000,2000,2500", help="serve: Poisson arrival rates, batches/s")
@@ -636,8 +745,6 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="batched/serve: launch shapes chosen by the autotuner (1) or the defaults (0)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--traffic", type=float, default=None,
-                    help="all-pairs: HBM bytes per launch from counter runs, reported as roofline.traffic")
     ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
                     help="auto: bit-sliced 32-row kernels; merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural", "sorted"], default="block",
@@ -647,11 +754,12 @@ def main():
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (the box has one); --gpus must be 1. "
+                                   "Sharding across GPUs is described in DESIGN.md."}))
         return 2
-    rec = {"allpairs": workload_allpairs, "zinc": workload_zinc, "cpu4096": workload_cpu4096,
-           "batched": workload_batched,
-           "threshold": workload_threshold, "serve": workload_serve}[args.workload](args)
+    rec = {"default": workload_default, "allpairs": workload_allpairs, "zinc": workload_zinc,
+           "cpu4096": workload_cpu4096, "batched": workload_batched, "threshold": workload_threshold,
+           "serve": workload_serve}[args.workload](args)
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -189,6 +189,9 @@ int siml_peek_rows_f32(siml_ctx* ctx, int64_t r0, int64_t r1, float* host, int64
  * since the last reset. */
 int siml_set_timing(siml_ctx* ctx, int on);
 int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_t* pairs, int reset);
+/* The time of siml_kernel_stats split into the similarity kernels and the
+ * top-k selection kernels that follow them (zero for other operations). */
+int siml_kernel_split(siml_ctx* ctx, double* similarity_ms, double* selection_ms);
 /* Work the neighbour kernels actually did while timing was on, counted on
  * the device: the pairs they evaluated (those the magnitude bound did not
  * prune) and the sum of La + Lb (unique lingo counts) over those pairs, the
@@ -228,6 +231,10 @@ const char* siml_last_kernel(siml_ctx* ctx);
 int siml_set_autotune(siml_ctx* ctx, int on);
 int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap, int32_t* topk_seed);
 int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed);
+/* Bit-sliced row groups of the current reference set (no counterpart in
+ * the reference): the number of groups (each probes every level of every
+ * query molecule once per launch) and of rows left to the merge kernel. */
+int siml_row_groups(siml_ctx* ctx, int64_t* groups, int64_t* merge_rows);
 /* Self-test of a short float32 division method of the kernels (no
  * counterpart in the reference): compares it on the device with IEEE
  * division for every integer numerator in [0, 65536) and denominator in
@@ -239,6 +246,11 @@ int siml_selftest_division(int device, int method, int64_t* mismatches, int64_t*
  * v_min_i32 (op 1) chains on every SIMD, the measured counterpart of the
  * 256 CU x 128 lanes x 2.4 GHz vector peak. */
 int siml_selftest_valu(int device, int op, double* lane_ops_per_s);
+/* Achievable LDS read rate of the device (no counterpart in the reference):
+ * bytes per second of 8-byte reads from a 32 KiB table per workgroup on
+ * every CU, without bank conflicts (op 0) or at pseudo-random slots (op 1,
+ * the access pattern of the bit-sliced kernels' table probes). */
+int siml_selftest_lds(int device, int op, double* bytes_per_s);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,9 @@ SIGNATURES = {
     "siml_selftest_division": (_int, [_int, _int, _i64p, _i64p, _i64p]),
     "siml_set_autotune": (_int, [_vp, _int]),
     "siml_selftest_valu": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
+    "siml_selftest_lds": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
+    "siml_row_groups": (_int, [_vp, _i64p, _i64p]),
+    "siml_kernel_split": (_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "siml_launch_shape": (_int, [_vp, _i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "siml_set_launch_shape": (_int, [_vp, _i64, ctypes.c_int32, ctypes.c_int32]),
 }
@@ -187,6 +190,20 @@ class Handle:
         ms, n, pairs = ctypes.c_double(0), _i64(0), _i64(0)
         self.call("siml_kernel_stats", ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs), int(reset))
         return ms.value, n.value, pairs.value
+
+    def kernel_split(self):
+        """(similarity ms, top-k selection ms) of kernel_stats' time since
+        its last reset."""
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        self.call("siml_kernel_split", ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def row_groups(self):
+        """(bit-sliced row groups, rows on the merge kernel) of the
+        reference set."""
+        g, m = _i64(0), _i64(0)
+        self.call("siml_row_groups", ctypes.byref(g), ctypes.byref(m))
+        return g.value, m.value
 
     def work_stats(self, reset=False):
         """(pairs evaluated, sum of La + Lb over them) by the neighbour

@@ -185,6 +185,19 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
     return v;
 }
 
+// OR of v over the wave, returned wave-uniform: DPP row shifts 1, 2, 4, 8
+// OR each row of 16 lanes into its lane 15, row broadcasts 15 and 31 carry
+// the rows into lane 63.  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -871,6 +884,11 @@ struct EpiF32 {
                           int64_t, int) const {
         if (valid) out[(row - rb) * opitch + col] = t;
     }
+    // a row none of the tile's columns intersects: T = 0 for every pair
+    __device__ bool zero_skips() const { return true; }
+    __device__ void skip_row(Lds&, Reg&, int, int32_t row, bool valid, int64_t col) const {
+        if (valid) out[(row - rb) * opitch + col] = 0.0f;
+    }
     __device__ void end(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
     __device__ void finish(Lds&, const BitRowInfo&) const {}
 };
@@ -920,6 +938,8 @@ struct EpiHist101 {
         const bool nz = valid && !z;
         if (__ballot(nz) != 0ull && nz) atomicAdd(&L.bins[r][bin101_f32(t)], 1);
     }
+    __device__ bool zero_skips() const { return true; }
+    __device__ void skip_row(Lds&, Reg& R, int r, int32_t, bool valid, int64_t) const { R.zero[r] += valid ? 1u : 0u; }
     __device__ void end(Lds& L, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
 #pragma unroll
         for (int r = 0; r < siml::kBitRows; ++r)
@@ -974,6 +994,8 @@ struct EpiHist1000 {
         else
             atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
     }
+    __device__ bool zero_skips() const { return true; }
+    __device__ void skip_row(Lds&, Reg& R, int, int32_t, bool valid, int64_t) const { R.zero += valid ? 1u : 0u; }
     __device__ void end(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
         if (R.zero) atomicAdd(&L.bins[0], R.zero);
     }
@@ -1068,6 +1090,9 @@ struct EpiNbr {
         if (lane_id() == 0) L.pos[w][r] = pos + __popcll(m);
         wave_lds_sync();
     }
+    // T = 0 is a neighbour only when lo <= 0
+    __device__ bool zero_skips() const { return lo > 0.0f; }
+    __device__ void skip_row(Lds&, Reg&, int, int32_t, bool, int64_t) const {}
     __device__ void end(Lds& L, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
         if (!FILL && lane < 32 && ri.row[lane] >= 0) counts[((int64_t)ri.row[lane] - rb) * nw + gw] = (int32_t)L.pos[w][lane];
@@ -1134,6 +1159,9 @@ struct EpiTopk {
             R.cnt = cnt;
         }
     }
+    // with pos_only a pair with T = 0 is never a candidate
+    __device__ bool zero_skips() const { return pos_only; }
+    __device__ void skip_row(Lds&, Reg&, int, int32_t, bool, int64_t) const {}
     __device__ void end(Lds&, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
         const int lane = lane_id();
         if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = R.cnt;
@@ -1312,6 +1340,10 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         t1 = (C.ntiles * (gw + 1)) / nw;
     }
     epi.begin(el, R, ri, gw, nw);
+    // With consistent magnitudes (FAST) a pair whose intersection is 0 has
+    // T = +0 exactly; an epilogue that can take such pairs in bulk skips the
+    // rows none of a tile's columns intersects (one wave-wide OR per tile).
+    const bool skip = FAST && epi.zero_skips();
     for (int64_t ti = t0; ti < t1; ++ti) {
         const int64_t t = ti * C.tstride;
         if constexpr (Epi::kPrune) {
@@ -1320,9 +1352,17 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         const int64_t c = t * kWave + lane;
         uint32_t P[NB];
         bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
+        const bool valid = c < C.n;
+        // bit r: row r's count is non-zero in some lane of the tile
+        uint32_t rows_any = 0xFFFFFFFFu;
+        if (skip) {
+            uint32_t nz = P[0];
+#pragma unroll
+            for (int k = 1; k < NB; ++k) nz |= P[k];
+            rows_any = wave_or_u32(valid ? nz : 0u);
+        }
         transpose8(P);
         if constexpr (NB == 16) transpose8(P + 8);
-        const bool valid = c < C.n;
         if constexpr (Epi::kPrune) epi.count_tile(R, C, valid, c);
         const int mb = valid ? C.mag[c] : 0;
         const float mbf = (float)mb;
@@ -1331,6 +1371,10 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         for (int r = 0; r < siml::kBitRows; ++r) {
             const int32_t row = ri.row[r];
             if (row < 0) continue;
+            if (!((rows_any >> r) & 1u)) {
+                epi.skip_row(el, R, r, row, valid, col);
+                continue;
+            }
             const int ma = ri.ma[r];
             const float fi = count_of<NB>(P, r);
             const float tv = pair_tanimoto<FAST>(fi, ri.maf[r], mbf, ma == 0 || mb == 0);
@@ -1494,6 +1538,36 @@ __global__ __launch_bounds__(kBlock) void k_valu_peak(uint32_t* __restrict__ out
         }
     }
     out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
+}
+
+// Achievable LDS rate of 8-byte reads (ds_read_b64, the table read of the
+// bit-sliced kernels) from a 32 KiB table per workgroup of 4 waves, as in
+// those kernels.  OP 0: a wave's 64 reads cover 512 contiguous bytes (no
+// bank conflicts); OP 1: every lane reads a pseudo-random slot, the access
+// pattern of the table probes.  Each thread issues iters x 8 independent
+// reads; bytes read = threads * iters * 64.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_lds_peak(uint32_t* __restrict__ out, int iters, uint32_t step) {
+    __shared__ uint2 tab[siml::kBitSlots];
+    for (int s = threadIdx.x; s < siml::kBitSlots; s += blockDim.x) tab[s] = make_uint2((uint32_t)s, (uint32_t)s * 3u);
+    __syncthreads();
+    uint32_t acc[8], idx[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        acc[q] = 0u;
+        idx[q] = OP == 0 ? threadIdx.x + 256u * q : (threadIdx.x + 1u) * 2654435761u + 40503u * q;
+    }
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t slot = OP == 0 ? (idx[q] & (siml::kBitSlots - 1)) : (idx[q] >> (32 - siml::kBitSlotBits));
+            const uint2 v = tab[slot];
+            acc[q] += v.x ^ v.y;
+            idx[q] = OP == 0 ? idx[q] + step : idx[q] * 1664525u + 1013904223u;
+        }
+    }
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] =
+        acc[0] ^ acc[1] ^ acc[2] ^ acc[3] ^ acc[4] ^ acc[5] ^ acc[6] ^ acc[7];
 }
 
 // Exhaustive comparison of short_div(method) with IEEE division over
@@ -1808,9 +1882,14 @@ struct siml_ctx {
     std::vector<int64_t> csr_rows;  // host row offsets of a CSR result
     // timing
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_ev;
+    // (start, selection start or null, end) of each timed operation
+    struct Timed {
+        hipEvent_t a, m, b;
+    };
+    std::vector<Timed> pending_ev;
     std::vector<hipEvent_t> ev_pool;
     double kernel_ms = 0.0;
+    double select_ms = 0.0;  // the part of kernel_ms spent after selection marks
     int64_t launches = 0, pairs = 0, pending_pairs = 0;
     // evaluated-work counter of the neighbour kernels (while timing is on):
     // device [pairs, sum of La + Lb], and the values at the last reset
@@ -1854,10 +1933,11 @@ hipEvent_t take_event(siml_ctx* ctx) {
     return e;
 }
 
-// Brackets the compute launches of one operation with events when timing.
+// Brackets the compute launches of one operation with events when timing;
+// select() marks where a top-k operation's selection kernels begin.
 struct TimedRegion {
     siml_ctx* ctx;
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t a = nullptr, m = nullptr, b = nullptr;
     int64_t pairs;
     TimedRegion(siml_ctx* c, int64_t p) : ctx(c), pairs(p) {
         if (ctx->timing) {
@@ -1866,16 +1946,23 @@ struct TimedRegion {
             if (a) (void)hipEventRecord(a, ctx->stream);
         }
     }
+    void select() {
+        if (ctx->timing && a && b && !m) {
+            m = take_event(ctx);
+            if (m) (void)hipEventRecord(m, ctx->stream);
+        }
+    }
     void done() {
         if (ctx->timing && a && b) {
             (void)hipEventRecord(b, ctx->stream);
-            ctx->pending_ev.emplace_back(a, b);
+            ctx->pending_ev.push_back({a, m, b});
             ctx->pending_pairs += pairs;
-            a = b = nullptr;
+            a = m = b = nullptr;
         }
     }
     ~TimedRegion() {
         if (a) ctx->ev_pool.push_back(a);
+        if (m) ctx->ev_pool.push_back(m);
         if (b) ctx->ev_pool.push_back(b);
     }
 };
@@ -1883,13 +1970,18 @@ struct TimedRegion {
 int collect_timing(siml_ctx* ctx) {
     if (ctx->pending_ev.empty()) return SIML_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (auto& pr : ctx->pending_ev) {
+    for (auto& t : ctx->pending_ev) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+        HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
         ctx->kernel_ms += ms;
+        if (t.m) {
+            HIP_TRY(hipEventElapsedTime(&ms, t.m, t.b));
+            ctx->select_ms += ms;
+            ctx->ev_pool.push_back(t.m);
+        }
         ctx->launches += 1;
-        ctx->ev_pool.push_back(pr.first);
-        ctx->ev_pool.push_back(pr.second);
+        ctx->ev_pool.push_back(t.a);
+        ctx->ev_pool.push_back(t.b);
     }
     ctx->pending_ev.clear();
     ctx->pairs += ctx->pending_pairs;
@@ -2249,6 +2341,11 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
     auto a = std::lower_bound(B.merge_rows.begin(), B.merge_rows.end(), (int32_t)rb);
     auto b = std::lower_bound(B.merge_rows.begin(), B.merge_rows.end(), rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
     p.merge.assign(a, b);
+    // the launches' group list, uploaded before any timed region begins
+    if (!p.groups.empty()) {
+        ctx->h_glist = p.groups;
+        TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
+    }
     return SIML_OK;
 }
 
@@ -2265,8 +2362,6 @@ template <class Epi>
 int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_t rl, const Epi& epi,
                 const char* name, const ColArgs* cols = nullptr) {
     if (p.groups.empty()) return SIML_OK;
-    ctx->h_glist = p.groups;
-    TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
     const dim3 grid((unsigned)p.groups.size(), gy);
     const RowSet::BitGrouping& B = ctx->ref.bits[p.which];
     const BitGroupArgs G{B.hdr.as<siml::BitGroupHdr>(), B.rows.as<int32_t>(), B.ent.as<uint4>(),
@@ -2594,9 +2689,10 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->work.release();
     ctx->pinned[0].release();
     ctx->pinned[1].release();
-    for (auto& pr : ctx->pending_ev) {
-        (void)hipEventDestroy(pr.first);
-        (void)hipEventDestroy(pr.second);
+    for (auto& t : ctx->pending_ev) {
+        (void)hipEventDestroy(t.a);
+        if (t.m) (void)hipEventDestroy(t.m);
+        (void)hipEventDestroy(t.b);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -3206,6 +3302,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
                                                          d_buf, d_cnt);
         TRY(launch_check("k_topk"));
     }
+    tr.select();
     TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>(), nullptr));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : "topk_merge";
@@ -3682,9 +3779,27 @@ int siml_kernel_stats(siml_ctx* ctx, double* total_ms, int64_t* launches, int64_
     if (pairs) *pairs = ctx->pairs;
     if (reset) {
         ctx->kernel_ms = 0.0;
+        ctx->select_ms = 0.0;
         ctx->launches = 0;
         ctx->pairs = 0;
     }
+    return SIML_OK;
+}
+
+int siml_kernel_split(siml_ctx* ctx, double* similarity_ms, double* selection_ms) {
+    if (!ctx || !similarity_ms || !selection_ms) return fail(SIML_EINVAL, "null pointer");
+    TRY(set_device(ctx));
+    TRY(collect_timing(ctx));
+    *similarity_ms = ctx->kernel_ms - ctx->select_ms;
+    *selection_ms = ctx->select_ms;
+    return SIML_OK;
+}
+
+int siml_row_groups(siml_ctx* ctx, int64_t* groups, int64_t* merge_rows) {
+    if (!ctx || !groups || !merge_rows) return fail(SIML_EINVAL, "null pointer");
+    const RowSet::BitGrouping& B = ctx->ref.bits[0];
+    *groups = B.built ? (int64_t)B.h_hdr.size() : 0;
+    *merge_rows = B.built ? (int64_t)B.merge_rows.size() : std::max<int64_t>(ctx->ref.n, 0);
     return SIML_OK;
 }
 
@@ -3844,6 +3959,42 @@ int siml_selftest_valu(int device, int op, double* lane_ops_per_s) {
     (void)hipEventDestroy(b);
     TRY(rc);
     *lane_ops_per_s = (double)blocks * kBlock * iters * 16.0 * 8.0 / (ms * 1e-3);
+    return SIML_OK;
+}
+
+int siml_selftest_lds(int device, int op, double* bytes_per_s) {
+    if (!bytes_per_s) return fail(SIML_EINVAL, "null pointer");
+    if (op != 0 && op != 1) return fail(SIML_EINVAL, "unknown access pattern %d", op);
+    siml_ctx* ctx = nullptr;
+    std::unique_lock<std::mutex> lk;
+    TRY(shared_ctx(device, lk, &ctx));
+    TRY(set_device(ctx));
+    const int blocks = 256 * 4 * 8;  // as many workgroups as 8 rounds of 4 per CU
+    const int iters = 4096;
+    TRY(ctx->scratch.reserve(sizeof(uint32_t) * (size_t)blocks * kBlock));
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(SIML_EDEVICE, "hipEventCreate failed");
+    }
+    float ms = 0.f;
+    int rc = SIML_OK;
+    for (int rep = 0; rep < 2 && rc == SIML_OK; ++rep) {  // the first launch warms up
+        (void)hipEventRecord(a, ctx->stream);
+        if (op == 0)
+            k_lds_peak<0><<<dim3(blocks), dim3(kBlock), 0, ctx->stream>>>(ctx->scratch.as<uint32_t>(), iters, 1024u);
+        else
+            k_lds_peak<1><<<dim3(blocks), dim3(kBlock), 0, ctx->stream>>>(ctx->scratch.as<uint32_t>(), iters, 1024u);
+        rc = launch_check("k_lds_peak");
+        (void)hipEventRecord(b, ctx->stream);
+        if (rc == SIML_OK && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = fail(SIML_EDEVICE, "timing k_lds_peak failed");
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    TRY(rc);
+    *bytes_per_s = (double)blocks * kBlock * iters * 8.0 * 8.0 / (ms * 1e-3);
     return SIML_OK;
 }
 
