@@ -186,6 +186,7 @@ def configure(g, args):
     h.call("siml_set_layout", {"natural": _lib.LAYOUT_NATURAL, "block": _lib.LAYOUT_BLOCK,
                                "sorted": _lib.LAYOUT_SORTED}[args.layout])
     h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "merge": _lib.POLICY_MERGE}[args.policy])
+    g.set_group_tables(args.group_tables)
 
 
 def base_record(args, metric, value, unit, wall, config, dtype="int32 intersections, float32 division", steps=None,
@@ -452,29 +453,34 @@ def batched_part(args, check_oracle=False, peaks=None):
     configure(g, args)
     g.attach_db(db)
     h = g._h
-    if args.autotune:
+    if args.launch_shape:
+        g.set_launch_shape(*[int(x) for x in args.launch_shape.split(",")])
+    elif args.autotune:
         g.set_autotune(True)
     # warm-up (and, with the autotuner, shape tuning): every set once and
     # the batched call once
     log("batched: warm-up pass over the sets and the batched call")
-    for s in sets:
-        g.set_refsmiles_ragged(*s)
-        g.getTopK(0, len(s[2]) - 1, k)
+    if not args.no_plain:
+        for s in sets:
+            g.set_refsmiles_ragged(*s)
+            g.getTopK(0, len(s[2]) - 1, k)
     g.set_refsmiles_ragged(*q)
     g.getTopK(0, nq, k)
     pairs = nq * ndb
 
     h.call("siml_set_timing", 1)
     h.kernel_stats(reset=True)
-    log("batched: plain, one getTopK per query set")
-    t0 = time.perf_counter()
     plain = []
-    for s in sets:
-        g.set_refsmiles_ragged(*s)
-        plain.append(g.getTopK(0, len(s[2]) - 1, k))
-    t_plain = time.perf_counter() - t0
-    sim_plain, sel_plain = h.kernel_split()
-    kms_plain, n_plain, _ = h.kernel_stats(reset=True)
+    t_plain = sim_plain = sel_plain = kms_plain = float("nan")
+    if not args.no_plain:
+        log("batched: plain, one getTopK per query set")
+        t0 = time.perf_counter()
+        for s in sets:
+            g.set_refsmiles_ragged(*s)
+            plain.append(g.getTopK(0, len(s[2]) - 1, k))
+        t_plain = time.perf_counter() - t0
+        sim_plain, sel_plain = h.kernel_split()
+        kms_plain, n_plain, _ = h.kernel_stats(reset=True)
 
     log("batched: all sets as one launch sequence")
     t0 = time.perf_counter()
@@ -489,9 +495,11 @@ def batched_part(args, check_oracle=False, peaks=None):
     log(f"batched: plain {t_plain:.2f} s ({kms_plain:.0f} ms kernels), batched {t_batched:.2f} s "
         f"({kms_b:.0f} ms kernels: {sim_b:.0f} similarity + {sel_b:.0f} selection)")
 
-    pi = np.concatenate([p[0] for p in plain])
-    pt = np.concatenate([p[1] for p in plain])
-    same = bool(np.array_equal(pi, bi) and np.array_equal(pt.view(np.int32), bt.view(np.int32)))
+    same = None
+    if plain:
+        pi = np.concatenate([p[0] for p in plain])
+        pt = np.concatenate([p[1] for p in plain])
+        same = bool(np.array_equal(pi, bi) and np.array_equal(pt.view(np.int32), bt.view(np.int32)))
     out = {"workload": f"{nsets} query sets (geometric 1/128 sizes clipped [1, 4096], {nq} molecules) x "
                        f"{ndb}-molecule resident database (lognormal lengths, median 45), top-{k} per query "
                        f"molecule", "sets": nsets, "queries": nq, "database": ndb, "k": k, "seed": 3,
@@ -651,7 +659,7 @@ def serving_part(args, rates, requests_for):
     np.cumsum(sizes, out=bounds[1:])
     results = []
     with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30,
-                autotune=args.autotune) as srv:
+                autotune=args.autotune, group_tables=args.group_tables) as srv:
         # warm-up: one request of each kind per power-of-two size up to the
         # batch limit (with the autotuner this also fixes every shape bucket
         # before the timed arrivals)
@@ -745,6 +753,11 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="batched/serve: launch shapes chosen by the autotuner (1) or the defaults (0)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-plain", action="store_true", help="batched: only the batched call (profiling runs)")
+    ap.add_argument("--group-tables", type=int, default=4, choices=[1, 2, 4],
+                    help="most row groups one workgroup of the bit-sliced kernels serves")
+    ap.add_argument("--launch-shape", default="",
+                    help="batched: fixed launch shape 'workgroups,topk_wave_cap,topk_seed' instead of the autotuner")
     ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
                     help="auto: bit-sliced 32-row kernels; merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural", "sorted"], default="block",

@@ -143,6 +143,11 @@ class GPULingo:
         self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c), ctypes.byref(sd))
         return t.value, c.value, sd.value
 
+    def set_group_tables(self, max_tables=4):
+        """Most row groups (1, 2 or 4) one workgroup of the bit-sliced
+        kernels serves; results do not depend on it."""
+        self._h.call("siml_set_group_tables", int(max_tables))
+
     def set_launch_shape(self, target_wg=8192, topk_wave_cap=256, topk_seed=0):
         """Fixes the launch shape (turns the autotuner off).  topk_seed: the
         top-k seed stride (0 or False: none; True: every 16th query tile;

@@ -88,6 +88,7 @@ SIGNATURES = {
     "siml_kernel_split": (_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "siml_launch_shape": (_int, [_vp, _i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "siml_set_launch_shape": (_int, [_vp, _i64, ctypes.c_int32, ctypes.c_int32]),
+    "siml_set_group_tables": (_int, [_vp, _int]),
 }
 
 _lib = None

@@ -29,6 +29,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "bitgroups.h"
@@ -860,18 +861,26 @@ __device__ __forceinline__ float count_of(const uint32_t (&P)[NB], int r) {
     }
 }
 
-struct BitRowInfo {
-    int32_t row[32];  // caller row index of group row r, -1 when not live
-    int32_t ma[32];
-    float maf[32];
+// Rows of the G groups one workgroup serves (32 per group): row q * 32 + r
+// is row r of the workgroup's q-th group.
+template <int NR>
+struct BitRows {
+    int32_t row[NR];  // caller row index, -1 when not live
+    int32_t ma[NR];
+    float maf[NR];
     int32_t amin, amax;  // magnitude range of the live rows
 };
+using BitRowInfo = BitRows<siml::kBitRows>;
 
 // Epilogue policies: init (LDS), begin (per wave), apply (per row, tile),
-// end (per wave), finish (after the workgroup barrier).
+// end (per wave), finish (after the workgroup barrier).  G is the number of
+// row groups (tables) a workgroup serves: 32 G rows, 4 G waves.
+template <int G>
 struct EpiF32 {
+    static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    using Rows = BitRows<32 * G>;
     float* out;
     int64_t opitch, rb;
     struct Lds {
@@ -879,7 +888,7 @@ struct EpiF32 {
     };
     struct Reg {};
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
+    __device__ void begin(Lds&, Reg&, const Rows&, int64_t, int64_t) const {}
     __device__ void apply(Lds&, Reg&, int, int32_t row, bool valid, int64_t col, int, int, float t, int64_t,
                           int64_t, int) const {
         if (valid) out[(row - rb) * opitch + col] = t;
@@ -889,8 +898,8 @@ struct EpiF32 {
     __device__ void skip_row(Lds&, Reg&, int, int32_t row, bool valid, int64_t col) const {
         if (valid) out[(row - rb) * opitch + col] = 0.0f;
     }
-    __device__ void end(Lds&, Reg&, const BitRowInfo&, int64_t, int64_t) const {}
-    __device__ void finish(Lds&, const BitRowInfo&) const {}
+    __device__ void end(Lds&, Reg&, const Rows&, int64_t, int64_t) const {}
+    __device__ void finish(Lds&, const Rows&) const {}
 };
 
 // Float32 matrix written a 256-column block at a time: each wave computes
@@ -899,6 +908,7 @@ struct EpiF32 {
 // Needs a layout that keeps columns inside their block, out and opitch
 // 16-byte aligned.
 struct EpiF32Block {
+    static constexpr int kG = 1;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = true;
     float* out;
@@ -914,6 +924,7 @@ struct EpiF32Block {
 // 101-bin rows: pairs in bin 0 (T * 100 < 1, the bulk of unrelated pairs)
 // are counted in per-lane registers, the rest with LDS atomics.
 struct EpiHist101 {
+    static constexpr int kG = 1;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
     int32_t* hist;
@@ -955,16 +966,20 @@ struct EpiHist101 {
 
 // 1000-bin histogram of the whole block: bin-0 pairs (T < fl32(1/1000)) in
 // per-lane registers, the rest with LDS atomics.
+template <int G>
 struct EpiHist1000 {
+    static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    using Rows = BitRows<32 * G>;
     unsigned long long* hist;
     // Pairs with I in [1, kIMax] and U = ma + mb - I below kIU (most
     // non-zero pairs of unrelated molecules) are counted by (I, U), T being a
     // function of those two: the counters spread over many more LDS
     // addresses than the few bins such T fall in, so fewer atomics of one
     // instruction collide.  finish() adds them to their bins.
-    static constexpr int kIU = 512;
+    // (kIU = 448 keeps two-table workgroups within half of a CU's LDS)
+    static constexpr int kIU = 448;
     static constexpr int kIMax = 4;
     struct Lds {
         uint32_t bins[1000];
@@ -979,7 +994,7 @@ struct EpiHist1000 {
         for (int k = threadIdx.x; k < kIMax * kIU; k += blockDim.x) L.iu[k] = 0u;
         stage_edges(L.edges);
     }
-    __device__ void begin(Lds&, Reg& R, const BitRowInfo&, int64_t, int64_t) const { R.zero = 0u; }
+    __device__ void begin(Lds&, Reg& R, const Rows&, int64_t, int64_t) const { R.zero = 0u; }
     __device__ void apply(Lds& L, Reg& R, int, int32_t, bool valid, int64_t, int ma, int mb, float t, int64_t,
                           int64_t, int isct) const {
         const bool z = !(t >= 1.0f / 1000.0f);  // edge 1 of stage_edges, fl32(1/1000)
@@ -996,10 +1011,10 @@ struct EpiHist1000 {
     }
     __device__ bool zero_skips() const { return true; }
     __device__ void skip_row(Lds&, Reg& R, int, int32_t, bool valid, int64_t) const { R.zero += valid ? 1u : 0u; }
-    __device__ void end(Lds& L, Reg& R, const BitRowInfo&, int64_t, int64_t) const {
+    __device__ void end(Lds& L, Reg& R, const Rows&, int64_t, int64_t) const {
         if (R.zero) atomicAdd(&L.bins[0], R.zero);
     }
-    __device__ void finish(Lds& L, const BitRowInfo&) const {
+    __device__ void finish(Lds& L, const Rows&) const {
         // the float expression of pair_tanimoto: (float)I / (float)U, both
         // exact integers (short_div equals IEEE division there)
         for (int k = threadIdx.x; k < kIMax * kIU; k += blockDim.x)
@@ -1015,10 +1030,13 @@ struct EpiHist1000 {
 // bound prefilter of gpuGetNeighbors_multirow (kernels.py:451-455) is
 // evaluated only where lo <= T < hi already holds; the result is the same
 // conjunction.
-template <bool FILL>
+template <bool FILL, int G>
 struct EpiNbr {
+    static constexpr int kG = G;
     static constexpr bool kPrune = true;
     static constexpr bool kBlockMode = false;
+    static constexpr int kRows = 32 * G;
+    using Rows = BitRows<kRows>;
     float lo, hi;
     int32_t* counts;
     const int64_t* offs;
@@ -1035,7 +1053,7 @@ struct EpiNbr {
     unsigned long long* work;
     const int32_t* rlen;
     struct Lds {
-        int64_t pos[kBlock / kWave][32];
+        int64_t pos[4 * G][kRows];
     };
     struct Reg {
         unsigned long long pairs = 0, lsum = 0;
@@ -1052,13 +1070,20 @@ struct EpiNbr {
         return false;
     }
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds& L, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void begin(Lds& L, Reg& R, const Rows& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
-        if (lane < 32) L.pos[w][lane] = (FILL && ri.row[lane] >= 0) ? offs[((int64_t)ri.row[lane] - rb) * nw + gw] : 0;
+        for (int i = lane; i < kRows; i += kWave)
+            L.pos[w][i] = (FILL && ri.row[i] >= 0) ? offs[((int64_t)ri.row[i] - rb) * nw + gw] : 0;
         if (work) {
-            const bool live = lane < siml::kBitRows && ri.row[lane < siml::kBitRows ? lane : 0] >= 0;
-            R.nrows = __popcll(__ballot(live));
-            R.sum_la = wave_sum_i32(live ? rlen[ri.row[lane]] : 0);
+            int nrows = 0, sum_la = 0;
+            for (int i0 = 0; i0 < kRows; i0 += kWave) {
+                const int i = i0 + lane;
+                const bool live = i < kRows && ri.row[i < kRows ? i : 0] >= 0;
+                nrows += __popcll(__ballot(live));
+                sum_la += wave_sum_i32(live ? rlen[ri.row[i]] : 0);
+            }
+            R.nrows = nrows;
+            R.sum_la = sum_la;
         }
         wave_lds_sync();
     }
@@ -1093,23 +1118,30 @@ struct EpiNbr {
     // T = 0 is a neighbour only when lo <= 0
     __device__ bool zero_skips() const { return lo > 0.0f; }
     __device__ void skip_row(Lds&, Reg&, int, int32_t, bool, int64_t) const {}
-    __device__ void end(Lds& L, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void end(Lds& L, Reg& R, const Rows& ri, int64_t gw, int64_t nw) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
-        if (!FILL && lane < 32 && ri.row[lane] >= 0) counts[((int64_t)ri.row[lane] - rb) * nw + gw] = (int32_t)L.pos[w][lane];
+        if (!FILL)
+            for (int i = lane; i < kRows; i += kWave)
+                if (ri.row[i] >= 0) counts[((int64_t)ri.row[i] - rb) * nw + gw] = (int32_t)L.pos[w][i];
         if (work) add_work(work, R.pairs, R.lsum);
     }
-    __device__ void finish(Lds&, const BitRowInfo&) const {}
+    __device__ void finish(Lds&, const Rows&) const {}
 };
 
 // Top-k candidates per (row, wave).  The per-row state lives in registers:
-// lane r of Reg::thr holds row r's threshold key (0 until armed) and lane r
-// of Reg::cnt the number of keys in its buffer.  A tile in which no key of a
+// row i's threshold key (0 until armed) is lane i % 64 of Reg::thr[i / 64]
+// and the number of keys in its buffer lane i % 64 of Reg::cnt[i / 64].  A tile in which no key of a
 // row passes costs one 64-bit compare and a ballot; otherwise the passing
 // keys are appended, after the buffer has been cut to its k largest if the
 // append could overflow it.
+template <int G>
 struct EpiTopk {
+    static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    static constexpr int kRows = 32 * G;
+    static constexpr int kRegs = (kRows + kWave - 1) / kWave;
+    using Rows = BitRows<kRows>;
     int k, cap;
     uint64_t* buf;
     int32_t* bufcnt;
@@ -1117,34 +1149,39 @@ struct EpiTopk {
     bool pos_only;           // only pairs with T > 0 are candidates
     const uint64_t* theta0;  // per row: a key at most the row's k-th largest (0: none), or null
     struct Lds {
-        uint32_t hist[kBlock / kWave][256];
+        uint32_t hist[4 * G][256];
     };
     struct Reg {
-        uint64_t thr;
-        int32_t cnt;
+        uint64_t thr[kRegs];
+        int32_t cnt[kRegs];
     };
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds&, Reg& R, const BitRowInfo& ri, int64_t, int64_t) const {
+    __device__ void begin(Lds&, Reg& R, const Rows& ri, int64_t, int64_t) const {
         const int lane = lane_id();
-        // a seeded row admits only keys >= its seed: threshold = seed - 1
-        uint64_t th = 0ull;
-        if (theta0 && lane < 32 && ri.row[lane] >= 0) {
-            const uint64_t s0 = theta0[(int64_t)ri.row[lane] - rb];
-            th = s0 ? s0 - 1 : 0ull;
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) {
+            // a seeded row admits only keys >= its seed: threshold = seed - 1
+            const int i = j * kWave + lane;
+            uint64_t th = 0ull;
+            if (theta0 && i < kRows && ri.row[i < kRows ? i : 0] >= 0) {
+                const uint64_t s0 = theta0[(int64_t)ri.row[i] - rb];
+                th = s0 ? s0 - 1 : 0ull;
+            }
+            R.thr[j] = th;
+            R.cnt[j] = 0;
         }
-        R.thr = th;
-        R.cnt = 0;
     }
     __device__ void apply(Lds& L, Reg& R, int r, int32_t row, bool valid, int64_t col, int, int, float t,
                           int64_t gw, int64_t nw, int) const {
         const int w = threadIdx.x / kWave, lane = lane_id();
+        const int j = r / kWave, rl = r % kWave;
         const uint64_t key = topk_key(t, col);
-        uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr >> 32), r) << 32) |
-                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)R.thr, r);
+        uint64_t thr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(R.thr[j] >> 32), rl) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)R.thr[j], rl);
         valid = valid && (!pos_only || t > 0.0f);
         uint64_t m = __ballot(valid && key > thr);
         if (m == 0ull) return;
-        int cnt = __builtin_amdgcn_readlane(R.cnt, r);
+        int cnt = __builtin_amdgcn_readlane(R.cnt[j], rl);
         uint64_t* b = buf + (((int64_t)row - rb) * nw + gw) * (int64_t)cap;
         if (cnt + kWave > cap) {
             // cnt > cap - 64 >= k distinct keys: keep the k largest
@@ -1154,19 +1191,23 @@ struct EpiTopk {
         }
         if (valid && key > thr) b[cnt + rank_below(m)] = key;
         cnt += __popcll(m);
-        if (lane == r) {
-            R.thr = thr;
-            R.cnt = cnt;
+        if (lane == rl) {
+            R.thr[j] = thr;
+            R.cnt[j] = cnt;
         }
     }
     // with pos_only a pair with T = 0 is never a candidate
     __device__ bool zero_skips() const { return pos_only; }
     __device__ void skip_row(Lds&, Reg&, int, int32_t, bool, int64_t) const {}
-    __device__ void end(Lds&, Reg& R, const BitRowInfo& ri, int64_t gw, int64_t nw) const {
+    __device__ void end(Lds&, Reg& R, const Rows& ri, int64_t gw, int64_t nw) const {
         const int lane = lane_id();
-        if (lane < 32 && ri.row[lane] >= 0) bufcnt[((int64_t)ri.row[lane] - rb) * nw + gw] = R.cnt;
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) {
+            const int i = j * kWave + lane;
+            if (i < kRows && ri.row[i < kRows ? i : 0] >= 0) bufcnt[((int64_t)ri.row[i] - rb) * nw + gw] = R.cnt[j];
+        }
     }
-    __device__ void finish(Lds&, const BitRowInfo&) const {}
+    __device__ void finish(Lds&, const Rows&) const {}
 };
 
 // Tanimoto of one pair from its intersection count fi, the row magnitude
@@ -1206,29 +1247,32 @@ __device__ __forceinline__ void bits_wg(int& gslot, int& y, int& ny) {
     }
 }
 
-template <class Epi>
+// The workgroup's GT groups are glist[gslot * GT + q], q < GT; a negative
+// entry (padding of the last slot) is an empty group: no live rows, an empty
+// table that matches no level.
+template <int GT, class Epi>
 __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
-                                              int64_t rl, const Epi& epi, uint2* tab, BitRowInfo& ri,
-                                              typename Epi::Lds& el) {
+                                              int64_t rl, const Epi& epi, uint2 (*tab)[siml::kBitSlots],
+                                              BitRows<32 * GT>& ri, typename Epi::Lds& el) {
     int gslot, y, ny;
     bits_wg(gslot, y, ny);
-    const int g = G.glist[gslot];
-    const siml::BitGroupHdr hd = G.hdr[g];
-    for (int s = threadIdx.x; s < siml::kBitSlots; s += blockDim.x) tab[s] = make_uint2(0u, 0u);
-    if (threadIdx.x < 32) {
-        const int r = threadIdx.x;
-        const int32_t row = r < hd.nrows ? G.rows[(int64_t)g * siml::kBitRowStride + r] : -1;
+    for (int s = threadIdx.x; s < GT * siml::kBitSlots; s += blockDim.x) (&tab[0][0])[s] = make_uint2(0u, 0u);
+    if (threadIdx.x < 32 * GT) {
+        const int q = threadIdx.x / 32, r = threadIdx.x % 32;
+        const int g = G.glist[(int64_t)gslot * GT + q];
+        const int nrows = g >= 0 ? G.hdr[g].nrows : 0;
+        const int32_t row = r < nrows ? G.rows[(int64_t)g * siml::kBitRowStride + r] : -1;
         const bool live = row >= 0 && row >= rb && row < rl;
-        ri.row[r] = live ? row : -1;
+        ri.row[threadIdx.x] = live ? row : -1;
         const int32_t ma = live ? rmag[row] : 0;
-        ri.ma[r] = ma;
-        ri.maf[r] = (float)ma;
+        ri.ma[threadIdx.x] = ma;
+        ri.maf[threadIdx.x] = (float)ma;
     }
     epi.init(el);
     __syncthreads();
     if (Epi::kPrune && threadIdx.x == 0) {
         int lo = INT32_MAX, hi = INT32_MIN;
-        for (int r = 0; r < 32; ++r)
+        for (int r = 0; r < 32 * GT; ++r)
             if (ri.row[r] >= 0) {
                 lo = min(lo, ri.ma[r]);
                 hi = max(hi, ri.ma[r]);
@@ -1236,11 +1280,77 @@ __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32
         ri.amin = lo;
         ri.amax = hi;
     }
-    for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
-        const uint4 e = G.ent[hd.entry_off + k];
-        tab[e.z] = make_uint2(e.x, e.y);
+#pragma unroll
+    for (int q = 0; q < GT; ++q) {
+        const int g = G.glist[(int64_t)gslot * GT + q];
+        if (g < 0) continue;
+        const siml::BitGroupHdr hd = G.hdr[g];
+        for (int k = threadIdx.x; k < hd.nkeys; k += blockDim.x) {
+            const uint4 e = G.ent[hd.entry_off + k];
+            tab[q][e.z] = make_uint2(e.x, e.y);
+        }
     }
     __syncthreads();
+}
+
+// Streams the Lt (a multiple of 8) levels of one query molecule against the
+// GT tables of the workgroup: each block of 8 levels is loaded once and
+// probed in every table, so the level stream is read once per GT groups.
+template <int NB, int GT>
+__device__ __forceinline__ void bits_accumulate_g(const uint2 (*tab)[siml::kBitSlots], const int4* __restrict__ p,
+                                                  int Lt, uint32_t (&P)[GT][NB]) {
+#pragma unroll
+    for (int q = 0; q < GT; ++q)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) P[q][k] = 0u;
+    const int nblk = Lt >> 3;
+    if (nblk <= 0) return;
+    // one table's 16 table reads are in flight at a time: the scheduling
+    // barriers keep the compiler from hoisting every table's reads (and
+    // their result registers) ahead of the first table's adder tree
+    int4 a[4], b[4];
+    load8(a, p);
+    int k = 0;
+    for (; k + 2 <= nblk; k += 2) {
+        load8(b, p + (size_t)(k + 1) * 4 * kWave);
+#pragma unroll
+        for (int q = 0; q < GT; ++q) {
+            probe8<NB>(tab[q], a, P[q]);
+            if constexpr (GT > 1) __builtin_amdgcn_sched_barrier(0);
+        }
+        // may read the 8 levels after this molecule's last: the stream
+        // carries one spare block at its end, so this stays in bounds
+        load8(a, p + (size_t)(k + 2) * 4 * kWave);
+#pragma unroll
+        for (int q = 0; q < GT; ++q) {
+            probe8<NB>(tab[q], b, P[q]);
+            if constexpr (GT > 1) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (k < nblk)
+#pragma unroll
+        for (int q = 0; q < GT; ++q) probe8<NB>(tab[q], a, P[q]);
+}
+
+// First tile of wave gw out of nw: the waves take contiguous, ascending tile
+// ranges holding equal shares of the level stream (binary search of the
+// level offsets), so waves that share a workgroup's tables finish together
+// whatever the storage order.  A sampled launch (tstride > 1) splits by
+// tile count.
+__device__ __forceinline__ int64_t wave_tile_start(const ColArgs& C, int64_t gw, int64_t nw) {
+    if (gw <= 0) return 0;
+    if (gw >= nw) return C.ntiles;
+    if (C.tstride != 1) return (C.ntiles * gw) / nw;
+    const int64_t want = C.btoff[C.ntiles] * gw / nw;
+    int64_t lo = 0, hi = C.ntiles;  // smallest t with btoff[t] >= want
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (C.btoff[mid] >= want)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
 }
 
 // Float32 matrix, a 256-column block per wave step (EpiF32Block).  At most
@@ -1248,10 +1358,11 @@ __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32
 template <int NB, bool FAST>
 __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
                                                           int64_t rb, int64_t rl, EpiF32Block epi) {
-    __shared__ uint2 tab[siml::kBitSlots];
+    __shared__ uint2 tabs[1][siml::kBitSlots];
     __shared__ BitRowInfo ri;
     __shared__ EpiF32Block::Lds el;
-    bits_prologue(G, rmag, rb, rl, epi, tab, ri, el);
+    bits_prologue<1>(G, rmag, rb, rl, epi, tabs, ri, el);
+    const uint2* tab = tabs[0];
     const int lane = lane_id();
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
     const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
@@ -1318,14 +1429,20 @@ __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const 
     }
 }
 
-// Every other epilogue: each wave walks a contiguous range of tiles.
+// Every other epilogue: each wave walks a contiguous range of tiles and
+// probes the workgroup's Epi::kG tables (32 rows each) with every level it
+// loads.
+// Workgroups serving several tables take the LDS of 2 or 4 single-table
+// ones, so they are held to 128 VGPRs: 4 waves per SIMD (the second bound
+// is the minimum number of waves per SIMD).
 template <int NB, bool FAST, class Epi>
-__global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
-                                                 int64_t rb, int64_t rl, Epi epi) {
-    __shared__ uint2 tab[siml::kBitSlots];
-    __shared__ BitRowInfo ri;
+__global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
+    BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C, int64_t rb, int64_t rl, Epi epi) {
+    constexpr int GT = Epi::kG;
+    __shared__ uint2 tab[GT][siml::kBitSlots];
+    __shared__ BitRows<32 * GT> ri;
     __shared__ typename Epi::Lds el;
-    bits_prologue(G, rmag, rb, rl, epi, tab, ri, el);
+    bits_prologue<GT>(G, rmag, rb, rl, epi, tab, ri, el);
     const int lane = lane_id();
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
     typename Epi::Reg R;
@@ -1336,13 +1453,14 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
         const int wpb = blockDim.x / kWave;
         gw = (int64_t)y * wpb + (threadIdx.x / kWave);
         nw = (int64_t)ny * wpb;
-        t0 = (C.ntiles * gw) / nw;
-        t1 = (C.ntiles * (gw + 1)) / nw;
+        t0 = wave_tile_start(C, gw, nw);
+        t1 = wave_tile_start(C, gw + 1, nw);
     }
     epi.begin(el, R, ri, gw, nw);
     // With consistent magnitudes (FAST) a pair whose intersection is 0 has
     // T = +0 exactly; an epilogue that can take such pairs in bulk skips the
-    // rows none of a tile's columns intersects (one wave-wide OR per tile).
+    // rows none of a tile's columns intersects (one wave-wide OR per table
+    // and tile).
     const bool skip = FAST && epi.zero_skips();
     for (int64_t ti = t0; ti < t1; ++ti) {
         const int64_t t = ti * C.tstride;
@@ -1350,35 +1468,44 @@ __global__ __launch_bounds__(kBlock) void k_bits(BitGroupArgs G, const int32_t* 
             if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
         }
         const int64_t c = t * kWave + lane;
-        uint32_t P[NB];
-        bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
+        uint32_t P[GT][NB];
+        bits_accumulate_g<NB, GT>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
         const bool valid = c < C.n;
-        // bit r: row r's count is non-zero in some lane of the tile
-        uint32_t rows_any = 0xFFFFFFFFu;
-        if (skip) {
-            uint32_t nz = P[0];
+        // bit r of rows_any[q]: row r of table q has a non-zero count in some
+        // lane of the tile
+        uint32_t rows_any[GT];
 #pragma unroll
-            for (int k = 1; k < NB; ++k) nz |= P[k];
-            rows_any = wave_or_u32(valid ? nz : 0u);
+        for (int q = 0; q < GT; ++q) {
+            rows_any[q] = 0xFFFFFFFFu;
+            if (skip) {
+                uint32_t nz = P[q][0];
+#pragma unroll
+                for (int k = 1; k < NB; ++k) nz |= P[q][k];
+                rows_any[q] = wave_or_u32(valid ? nz : 0u);
+            }
+            transpose8(P[q]);
+            if constexpr (NB == 16) transpose8(P[q] + 8);
         }
-        transpose8(P);
-        if constexpr (NB == 16) transpose8(P + 8);
         if constexpr (Epi::kPrune) epi.count_tile(R, C, valid, c);
         const int mb = valid ? C.mag[c] : 0;
         const float mbf = (float)mb;
         const int64_t col = valid ? col_of(C, c) : 0;
 #pragma unroll
-        for (int r = 0; r < siml::kBitRows; ++r) {
-            const int32_t row = ri.row[r];
-            if (row < 0) continue;
-            if (!((rows_any >> r) & 1u)) {
-                epi.skip_row(el, R, r, row, valid, col);
-                continue;
+        for (int q = 0; q < GT; ++q) {
+#pragma unroll
+            for (int r = 0; r < siml::kBitRows; ++r) {
+                const int rr = q * siml::kBitRows + r;
+                const int32_t row = ri.row[rr];
+                if (row < 0) continue;
+                if (!((rows_any[q] >> r) & 1u)) {
+                    epi.skip_row(el, R, rr, row, valid, col);
+                    continue;
+                }
+                const int ma = ri.ma[rr];
+                const float fi = count_of<NB>(P[q], r);
+                const float tv = pair_tanimoto<FAST>(fi, ri.maf[rr], mbf, ma == 0 || mb == 0);
+                epi.apply(el, R, rr, row, valid, col, ma, mb, tv, gw, nw, (int)fi);
             }
-            const int ma = ri.ma[r];
-            const float fi = count_of<NB>(P, r);
-            const float tv = pair_tanimoto<FAST>(fi, ri.maf[r], mbf, ma == 0 || mb == 0);
-            epi.apply(el, R, r, row, valid, col, ma, mb, tv, gw, nw, (int)fi);
         }
     }
     epi.end(el, R, ri, gw, nw);
@@ -1905,6 +2032,8 @@ struct siml_ctx {
     // top-k: seed row thresholds from every topk_seed-th query tile first
     // (0: no seeding)
     int topk_seed = 0;
+    // most row groups (tables) one bit-sliced workgroup serves: 1, 2 or 4
+    int max_tables = 4;
     bool autotune = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
@@ -2307,6 +2436,9 @@ int upload_rowlist(siml_ctx* ctx, const std::vector<int32_t>& rows) {
     return upload(ctx->rowlist, ctx->h_rowlist.data(), ctx->h_rowlist.size(), ctx->stream);
 }
 
+// Most row groups one bit-sliced workgroup serves (bits_tables).
+constexpr int kMaxTables = 4;
+
 // Bit-sliced plan of rows [rb, rl): the groups holding any of those rows,
 // the planes they need, and the rows left to the merge kernel.
 struct BitPlan {
@@ -2342,11 +2474,33 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
     auto b = std::lower_bound(B.merge_rows.begin(), B.merge_rows.end(), rl > INT32_MAX ? INT32_MAX : (int32_t)rl);
     p.merge.assign(a, b);
     // the launches' group list, uploaded before any timed region begins
+    // padded with empty groups (-1) to a multiple of kMaxTables, the most
+    // groups one workgroup serves
     if (!p.groups.empty()) {
         ctx->h_glist = p.groups;
+        while (ctx->h_glist.size() % kMaxTables) ctx->h_glist.push_back(-1);
         TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
     }
     return SIML_OK;
+}
+
+// Tables (row groups) per workgroup of a bit-sliced launch: each column
+// level a workgroup loads is probed in all of its tables, so the level
+// stream is read from memory once per that many groups.  Fewer tables when
+// the launch has few groups (an empty table costs its probes) or needs 16
+// bit planes (the planes of several tables exceed the registers).
+int bits_tables(const siml_ctx* ctx, const BitPlan& p) {
+    const size_t n = p.groups.size();
+    if (p.nb != 8 || n < 2 || ctx->max_tables < 2) return 1;
+    return (n < 8 || ctx->max_tables < 4) ? 2 : 4;
+}
+
+// Calls f with std::integral_constant<int, G> for G = gt (1, 2 or 4).
+template <class F>
+int with_tables(int gt, F&& f) {
+    if (gt == 4) return f(std::integral_constant<int, 4>{});
+    if (gt == 2) return f(std::integral_constant<int, 2>{});
+    return f(std::integral_constant<int, 1>{});
 }
 
 int launch_check(const char* name);
@@ -2357,12 +2511,15 @@ constexpr int64_t kSeedStride = 16;
 // Neighbour indices kept per (row, wave) slot by the count pass.
 constexpr int kNbrCap = 16;
 
-// Launches k_bits<NB, Epi> over the plan's groups with grid.y = gy.
+// Launches k_bits<NB, Epi> over the plan's groups, Epi::kG groups per
+// workgroup, with grid.y = gy column ranges.
 template <class Epi>
 int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_t rl, const Epi& epi,
                 const char* name, const ColArgs* cols = nullptr) {
     if (p.groups.empty()) return SIML_OK;
-    const dim3 grid((unsigned)p.groups.size(), gy);
+    constexpr int GT = Epi::kG;
+    if (GT > 1 && p.nb != 8) return fail(SIML_EINVAL, "%s: %d tables per workgroup need 8 bit planes", name, GT);
+    const dim3 grid((unsigned)((p.groups.size() + GT - 1) / GT), gy);
     const RowSet::BitGrouping& B = ctx->ref.bits[p.which];
     const BitGroupArgs G{B.hdr.as<siml::BitGroupHdr>(), B.rows.as<int32_t>(), B.ent.as<uint4>(),
                          ctx->glist.as<int32_t>()};
@@ -2379,14 +2536,17 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
         else
             k_bits_block<16, false><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
     } else {
+        const dim3 block(kBlock * GT);
         if (p.nb == 8 && fast)
-            k_bits<8, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits<8, true, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
         else if (p.nb == 8)
-            k_bits<8, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
-        else if (fast)
-            k_bits<16, true, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
-        else
-            k_bits<16, false, Epi><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits<8, false, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        else if constexpr (GT == 1) {
+            if (fast)
+                k_bits<16, true, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            else
+                k_bits<16, false, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+        }
     }
     return launch_check(name);
 }
@@ -2981,7 +3141,8 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
             TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32Block{ctx->res.as<float>(), dp, rowbase},
                             "k_bits<f32 block>"));
         else
-            TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32{ctx->res.as<float>(), dp, rowbase}, "k_bits<f32>"));
+            TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32<1>{ctx->res.as<float>(), dp, rowbase},
+                            "k_bits<f32>"));
         if (!bp.merge.empty()) {
             k_rows_f32<false><<<row_grid((int64_t)bp.merge.size(), ctx->qry->ntiles), dim3(kBlock), lds,
                                 ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase,
@@ -3081,28 +3242,36 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
+    const int gt = bp.use ? bits_tables(ctx, bp) : 1;
     if (bp.use) {
-        // bit-sliced groups hold up to 32 rows: size grid.y by groups
-        const unsigned gyb = row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y;
-        const int64_t cap = std::min<int64_t>(65535, (int64_t(1) << 24) / (nr * (kBlock / kWave)));
+        // bit-sliced workgroups serve gt groups of up to 32 rows: size
+        // grid.y by workgroups, at most 2^24 (row, wave) slots
+        const int64_t slots = std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1);
+        const unsigned gyb = row_grid(slots, ctx->qry->ntiles, ctx->target_wg).y;
+        const int64_t cap = std::min<int64_t>(65535 / gt, (int64_t(1) << 24) / (nr * (kBlock / kWave) * gt));
         grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
     }
-    const int64_t nw = (int64_t)grid.y * (kBlock / kWave);
+    // waves per row: the merge kernel (4 waves per workgroup) runs gt times
+    // as many column ranges so that both kernels index the same slots
+    const int64_t nw = (int64_t)grid.y * (kBlock / kWave) * gt;
     int32_t* d_nbuf = nullptr;
-    const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y);
+    const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y * gt);
     unsigned long long* work = ctx->timing ? ctx->work.as<unsigned long long>() : nullptr;
     const int32_t* rlen = ctx->ref.args().len;
     auto launch = [&](bool fill, const int64_t* offs, int32_t* idx) -> int {
         int32_t* counts = fill ? nullptr : d_counts_of(ctx);
         if (bp.use) {
-            if (fill)
-                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
-                                EpiNbr<true>{lo, hi, nullptr, offs, idx, rowbase, nullptr, 0, work, rlen},
-                                "k_bits<nbr fill>"));
-            else
-                TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
-                                EpiNbr<false>{lo, hi, counts, nullptr, nullptr, rowbase, d_nbuf, kNbrCap, work, rlen},
-                                "k_bits<nbr count>"));
+            TRY(with_tables(gt, [&](auto gc) {
+                constexpr int GC = decltype(gc)::value;
+                if (fill)
+                    return launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
+                                       EpiNbr<true, GC>{lo, hi, nullptr, offs, idx, rowbase, nullptr, 0, work, rlen},
+                                       "k_bits<nbr fill>");
+                return launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
+                                   EpiNbr<false, GC>{lo, hi, counts, nullptr, nullptr, rowbase, d_nbuf, kNbrCap, work,
+                                                     rlen},
+                                   "k_bits<nbr count>");
+            }));
         } else {
             if (fill)
                 k_neighbors<true><<<grid, dim3(kBlock), lds, ctx->stream>>>(ctx->ref.args(), ctx->qry->args(), nullptr,
@@ -3212,11 +3381,15 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
     static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
                   "top-k buffer exceeds wave registers");
-    dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
-                         bp.use ? ctx->target_wg : 2048);
-    grid.y = std::min<unsigned>(grid.y, ctx->topk_wave_cap);
-    if (bp.use) grid.y = xcd_gy(grid.y, std::min<unsigned>(ctx->topk_wave_cap, 65535));
-    int64_t nw = (int64_t)grid.y * (kBlock / kWave);
+    // bit-sliced workgroups serve gt groups each (4 gt waves); the merge
+    // kernel runs gt times as many column ranges so that both index the
+    // same (row, wave) candidate buffers
+    const int gt = bp.use ? bits_tables(ctx, bp) : 1;
+    dim3 grid = row_grid(bp.use ? std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1) : nr,
+                         ctx->qry->ntiles, bp.use ? ctx->target_wg : 2048);
+    grid.y = std::min<unsigned>(grid.y, std::max(1u, ctx->topk_wave_cap / (unsigned)gt));
+    if (bp.use) grid.y = xcd_gy(grid.y, std::min<unsigned>(std::max(1u, ctx->topk_wave_cap / (unsigned)gt), 65535));
+    int64_t nw = (int64_t)grid.y * (kBlock / kWave) * gt;
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
     const size_t cnt_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
@@ -3278,10 +3451,13 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
         ColArgs S = ctx->qry->args();
         S.tstride = stride;
         S.ntiles = (ctx->qry->ntiles + stride - 1) / stride;
-        const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid.y, (S.ntiles + 3) / 4));
-        const int64_t nws = (int64_t)gys * (kBlock / kWave);
-        TRY(launch_bits(ctx, bp, gys, rowbase, rowlimit,
-                        EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only, nullptr}, "k_bits<topk sample>", &S));
+        const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid.y, (S.ntiles + 4 * gt - 1) / (4 * gt)));
+        const int64_t nws = (int64_t)gys * (kBlock / kWave) * gt;
+        TRY(with_tables(gt, [&](auto gc) {
+            return launch_bits(ctx, bp, gys, rowbase, rowlimit,
+                               EpiTopk<decltype(gc)::value>{k, cap, d_buf, d_cnt, rowbase, pos_only, nullptr},
+                               "k_bits<topk sample>", &S);
+        }));
         const int64_t nw_full = nw;
         nw = nws;
         const int rc = select(nullptr, nullptr, d_seed);
@@ -3290,10 +3466,13 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
         seed = d_seed;
     }
     if (bp.use) {
-        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiTopk{k, cap, d_buf, d_cnt, rowbase, pos_only, seed},
-                        "k_bits<topk>"));
+        TRY(with_tables(gt, [&](auto gc) {
+            return launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
+                               EpiTopk<decltype(gc)::value>{k, cap, d_buf, d_cnt, rowbase, pos_only, seed},
+                               "k_bits<topk>");
+        }));
         if (!mrows.empty()) {
-            k_topk<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
+            k_topk<<<dim3((unsigned)mrows.size(), grid.y * gt), dim3(kBlock), lds, ctx->stream>>>(
                 ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
             TRY(launch_check("k_topk<rowlist>"));
         }
@@ -3327,15 +3506,19 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    dim3 grid = row_grid(bp.use ? std::max<int64_t>((int64_t)bp.groups.size(), 1) : nr, ctx->qry->ntiles,
-                         bp.use ? ctx->target_wg : 2048);
-    // one workgroup counts at most 32 rows x its columns in uint32 LDS bins
-    const int64_t min_gy = (siml::kBitRows * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
+    const int gt = bp.use ? bits_tables(ctx, bp) : 1;
+    dim3 grid = row_grid(bp.use ? std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1) : nr,
+                         ctx->qry->ntiles, bp.use ? ctx->target_wg : 2048);
+    // one workgroup counts at most 32 gt rows x its columns in uint32 LDS bins
+    const int64_t min_gy = (siml::kBitRows * gt * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
     if (bp.use) grid.y = std::max<unsigned>(xcd_gy(grid.y, 65535), (unsigned)std::min<int64_t>(min_gy, 65535));
     TimedRegion tr(ctx, nr * nq);
     if (bp.use) {
-        TRY(launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiHist1000{hist}, "k_bits<hist1000>"));
+        TRY(with_tables(gt, [&](auto gc) {
+            return launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiHist1000<decltype(gc)::value>{hist},
+                               "k_bits<hist1000>");
+        }));
         if (!mrows.empty()) {
             k_hist1000<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
                 ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, hist);
@@ -3489,6 +3672,14 @@ int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap,
     *target_wg = ctx->target_wg;
     *topk_wave_cap = (int32_t)ctx->topk_wave_cap;
     *topk_seed = ctx->topk_seed;
+    return SIML_OK;
+}
+
+int siml_set_group_tables(siml_ctx* ctx, int max_tables) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (max_tables != 1 && max_tables != 2 && max_tables != 4)
+        return fail(SIML_EINVAL, "tables per workgroup must be 1, 2 or 4, not %d", max_tables);
+    ctx->max_tables = max_tables;
     return SIML_OK;
 }
 

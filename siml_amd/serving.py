@@ -80,7 +80,7 @@ class Server:
     near saturation (2500 batches/s).
     """
 
-    def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False):
+    def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False, group_tables=4):
         self.db = db
         self.max_rows = int(max_rows)
         self.mem_budget = int(mem_budget)
@@ -94,6 +94,7 @@ class Server:
         for _ in range(slots):
             g = GPULingo(device)
             g.attach_db(db)
+            g.set_group_tables(group_tables)
             if autotune:
                 g.set_autotune(True)
             self.handles.append(g)

@@ -232,3 +232,39 @@ def test_topk_launch_shapes_and_seeding(db_and_queries, shape):
         assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
     assert g.launch_shape() == (shape[0], shape[1], shape[2])
     db.close()
+
+
+@pytest.mark.parametrize("tables", [1, 2, 4])
+def test_group_tables_bit_identical(db_and_queries, tables):
+    """Workgroups serving 1, 2 or 4 row groups give the oracle's results
+    for top-k (with and without seeding), neighbour CSR (single pass and
+    fill pass) and the 1000-bin histogram, also when some rows are served by
+    the merge kernel in the same call (counts above 15: the merge kernel
+    then indexes the same (row, wave) slots as the bit-sliced kernels)."""
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set)
+    D = synth.padded_columns(*db_set, 0, len(db_set[2]) - 1)
+    chains = [b"C" * n + b"\n" for n in (22, 30, 45)]     # counts 19, 27, 42: merge-kernel rows
+    Qp = synth.padded_columns(*q, 0, 300)
+    Cp = compiler.cSMILEStoMatrices(chains)[:4]
+    w = max(Qp[0].shape[1], Cp[0].shape[1])
+    pad = lambda m: np.pad(m, ((0, 0), (0, w - m.shape[1])))
+    Q = (np.vstack([pad(Qp[0]), pad(Cp[0])]), np.vstack([pad(Qp[1]), pad(Cp[1])]),
+         np.vstack([Qp[2], Cp[2]]), np.vstack([Qp[3], Cp[3]]))
+    n = Q[0].shape[0]
+    for seed in (0, 4):
+        g = GPULingo(0)
+        g.attach_db(db)
+        g.set_group_tables(tables)
+        g.set_launch_shape(2048, 256, seed)
+        g.set_refsmiles(*Q)
+        ti, tt = g.getTopK(0, n, 40)
+        oi, ot = oracle.topk(Q, D, 40)
+        assert np.array_equal(ti, oi) and np.array_equal(bits(tt), bits(ot))
+    for lo in (0.5, 0.0):   # 0.0: every pair, more than 16 per slot (fill pass)
+        offs, idx = g.getNeighborsCSR(0, n, lo)
+        lists, counts = oracle.neighbors(Q, D, lo)
+        assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
+    assert np.array_equal(g.getGlobalHistogram(0, n), oracle.hist1000(Q, D))
+    assert np.array_equal(g.getGlobalHistogram(17, 290), oracle.hist1000(tuple(x[17:290] for x in Q), D))
+    db.close()
