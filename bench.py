@@ -754,7 +754,7 @@ def main():
                     help="batched/serve: launch shapes chosen by the autotuner (1) or the defaults (0)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-plain", action="store_true", help="batched: only the batched call (profiling runs)")
-    ap.add_argument("--group-tables", type=int, default=4, choices=[1, 2, 4],
+    ap.add_argument("--group-tables", type=int, default=2, choices=[1, 2, 4],
                     help="most row groups one workgroup of the bit-sliced kernels serves")
     ap.add_argument("--launch-shape", default="",
                     help="batched: fixed launch shape 'workgroups,topk_wave_cap,topk_seed' instead of the autotuner")

@@ -236,11 +236,12 @@ int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_ca
  * query molecule once per launch) and of rows left to the merge kernel. */
 int siml_row_groups(siml_ctx* ctx, int64_t* groups, int64_t* merge_rows);
 /* Most row groups one bit-sliced workgroup serves (no counterpart in the
- * reference): 1, 2 or 4 (default 4).  A workgroup serving G groups holds G
+ * reference): 1, 2 or 4 (default 2).  A workgroup serving G groups holds G
  * tables in LDS and probes each column level it loads in all of them, so
- * the query stream is read from memory once per G groups; launches with few
- * groups, or with rows of more than 255 levels, use fewer.  Results do not
- * depend on it. */
+ * the query stream is read from memory once per G groups.  Top-k and the
+ * 1000-bin histogram use it; launches with few groups, rows of more than
+ * 255 levels, and the magnitude-pruned neighbour search use one table.
+ * Results do not depend on it. */
 int siml_set_group_tables(siml_ctx* ctx, int max_tables);
 /* Self-test of a short float32 division method of the kernels (no
  * counterpart in the reference): compares it on the device with IEEE

@@ -143,7 +143,7 @@ class GPULingo:
         self._h.call("siml_launch_shape", ctypes.byref(t), ctypes.byref(c), ctypes.byref(sd))
         return t.value, c.value, sd.value
 
-    def set_group_tables(self, max_tables=4):
+    def set_group_tables(self, max_tables=2):
         """Most row groups (1, 2 or 4) one workgroup of the bit-sliced
         kernels serves; results do not depend on it."""
         self._h.call("siml_set_group_tables", int(max_tables))

@@ -174,6 +174,18 @@ __device__ __forceinline__ void wave_tiles(int64_t ntiles, int64_t& t0, int64_t&
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// Element i of an array no kernel writes while it runs, read through the
+// constant address space: with a wave-uniform index the load is a scalar
+// load (scalar cache, scalar registers) instead of a vector load.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p, int64_t i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+__device__ __forceinline__ int2 load_uniform(const int2* p, int64_t i) {
+    const int32_t* q = reinterpret_cast<const int32_t*>(p);
+    return make_int2(load_uniform(q, 2 * i), load_uniform(q, 2 * i + 1));
+}
+
 // Number of set bits of `m` below this lane.
 __device__ __forceinline__ int rank_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -907,6 +919,14 @@ struct EpiF32 {
 // offsets inside the block and stores them as one contiguous 1 KiB run.
 // Needs a layout that keeps columns inside their block, out and opitch
 // 16-byte aligned.
+// Waves per workgroup of the float32 block kernel: with 8 bit planes, 8
+// waves share the workgroup's table (3 workgroups per CU by LDS, 6 waves per
+// SIMD within 80 VGPRs); with 16 planes, 4.
+template <int NB>
+constexpr int block_waves() {
+    return NB == 8 ? 8 : 4;
+}
+
 struct EpiF32Block {
     static constexpr int kG = 1;
     static constexpr bool kPrune = false;
@@ -914,7 +934,7 @@ struct EpiF32Block {
     float* out;
     int64_t opitch, rb;
     struct Lds {
-        float stage[kBlock / kWave][5 * kWave];
+        int unused;  // the kernel sizes its staging rows by its wave count
     };
     struct Reg {};
     __device__ void init(Lds&) const {}
@@ -1341,11 +1361,11 @@ __device__ __forceinline__ int64_t wave_tile_start(const ColArgs& C, int64_t gw,
     if (gw <= 0) return 0;
     if (gw >= nw) return C.ntiles;
     if (C.tstride != 1) return (C.ntiles * gw) / nw;
-    const int64_t want = C.btoff[C.ntiles] * gw / nw;
+    const int64_t want = load_uniform(C.btoff, C.ntiles) * gw / nw;
     int64_t lo = 0, hi = C.ntiles;  // smallest t with btoff[t] >= want
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if (C.btoff[mid] >= want)
+        if (load_uniform(C.btoff, mid) >= want)
             hi = mid;
         else
             lo = mid + 1;
@@ -1353,14 +1373,16 @@ __device__ __forceinline__ int64_t wave_tile_start(const ColArgs& C, int64_t gw,
     return lo;
 }
 
-// Float32 matrix, a 256-column block per wave step (EpiF32Block).  At most
-// 128 VGPRs, so 4 waves share each SIMD.
+// Float32 matrix, a 256-column block per wave step (EpiF32Block), with
+// block_waves<NB>() waves per workgroup held to the registers that let 6
+// (8 planes) or 4 (16 planes) waves share each SIMD.
 template <int NB, bool FAST>
-__global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C,
-                                                          int64_t rb, int64_t rl, EpiF32Block epi) {
+__global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_bits_block(
+    BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C, int64_t rb, int64_t rl, EpiF32Block epi) {
     __shared__ uint2 tabs[1][siml::kBitSlots];
     __shared__ BitRowInfo ri;
     __shared__ EpiF32Block::Lds el;
+    __shared__ float stage_rows[block_waves<NB>()][5 * kWave];
     bits_prologue<1>(G, rmag, rb, rl, epi, tabs, ri, el);
     const uint2* tab = tabs[0];
     const int lane = lane_id();
@@ -1368,21 +1390,35 @@ __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const 
     const int wpb = blockDim.x / kWave, w = threadIdx.x / kWave;
     int gslot, y, ny;
     bits_wg(gslot, y, ny);
-    const int64_t gw = (int64_t)y * wpb + w, nw = (int64_t)ny * wpb;
+    // wave-uniform block loop: tile offsets and lengths are scalar loads
+    const int64_t gw = __builtin_amdgcn_readfirstlane(y * wpb + w), nw = (int64_t)ny * wpb;
     const int64_t nblocks = (C.ntiles + 3) / 4;
     const int64_t b0 = (nblocks * gw) / nw, b1 = (nblocks * (gw + 1)) / nw;
-    float* stage = el.stage[w];  // 256 column slots + 64 for lanes past the end
+    float* stage = stage_rows[w];  // 256 column slots + 64 for lanes past the end
+    // the rows' caller indices and magnitudes in lane registers (lane r holds
+    // row r), and which rows are live, read once
+    const int32_t row_lane = ri.row[lane & (siml::kBitRows - 1)];
+    const int32_t ma_lane = ri.ma[lane & (siml::kBitRows - 1)];
+    const uint32_t live = (uint32_t)__ballot(lane < siml::kBitRows && row_lane >= 0);
     for (int64_t blk = b0; blk < b1; ++blk) {
         uint32_t Q[4][NB];
         float mbf[4];
         bool zb[4];
         int loc[4];
+        int64_t toff[4];
+        int32_t tlen[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t t = blk * 4 + q;
+            toff[q] = t < C.ntiles ? load_uniform(C.btoff, t) : 0;
+            tlen[q] = t < C.ntiles ? load_uniform(C.btlen, t) : 0;
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int64_t t = blk * 4 + q;
             const int64_t c = t * kWave + lane;
             if (t < C.ntiles) {
-                bits_accumulate<NB>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], Q[q]);
+                bits_accumulate<NB>(tab, bt4 + (toff[q] >> 1) + lane, tlen[q], Q[q]);
             } else {
 #pragma unroll
                 for (int k = 0; k < NB; ++k) Q[q][k] = 0u;
@@ -1406,10 +1442,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_bits_block(BitGroupArgs G, const 
         const bool full = cbase + 4 * kWave <= C.n;
 #pragma unroll
         for (int r = 0; r < siml::kBitRows; ++r) {
-            const int32_t row = ri.row[r];
-            if (row < 0) continue;
-            const bool za = ri.ma[r] == 0;
-            const float maf = ri.maf[r];
+            if (!((live >> r) & 1u)) continue;
+            const int32_t row = __builtin_amdgcn_readlane(row_lane, r);
+            const int ma = __builtin_amdgcn_readlane(ma_lane, r);
+            const bool za = ma == 0;
+            const float maf = (float)ma;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 stage[loc[q]] = pair_tanimoto<FAST>(count_of<NB>(Q[q], r), maf, mbf[q], za || zb[q]);
@@ -1451,26 +1488,52 @@ __global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
         int gslot, y, ny;
         bits_wg(gslot, y, ny);
         const int wpb = blockDim.x / kWave;
-        gw = (int64_t)y * wpb + (threadIdx.x / kWave);
+        // wave-uniform: the tile loop, its offsets and lengths then live in
+        // scalar registers and are read through the scalar cache
+        gw = __builtin_amdgcn_readfirstlane(y * wpb + (int)(threadIdx.x / kWave));
         nw = (int64_t)ny * wpb;
         t0 = wave_tile_start(C, gw, nw);
         t1 = wave_tile_start(C, gw + 1, nw);
     }
     epi.begin(el, R, ri, gw, nw);
+    // the rows' caller indices and magnitudes in lane registers (lane r of
+    // table q: row q * 32 + r), and which rows are live, read once
+    int32_t row_lane[GT], ma_lane[GT];
+    uint32_t live[GT];
+#pragma unroll
+    for (int q = 0; q < GT; ++q) {
+        const int i = q * siml::kBitRows + (lane & (siml::kBitRows - 1));
+        row_lane[q] = ri.row[i];
+        ma_lane[q] = ri.ma[i];
+        live[q] = (uint32_t)__ballot(lane < siml::kBitRows && row_lane[q] >= 0);
+    }
     // With consistent magnitudes (FAST) a pair whose intersection is 0 has
     // T = +0 exactly; an epilogue that can take such pairs in bulk skips the
     // rows none of a tile's columns intersects (one wave-wide OR per table
     // and tile).
     const bool skip = FAST && epi.zero_skips();
+    // scalar loads of the next tile's level offset and length are issued
+    // while the current tile is probed
+    int64_t next_off = t0 < t1 ? load_uniform(C.btoff, t0 * C.tstride) : 0;
+    int32_t next_len = t0 < t1 ? load_uniform(C.btlen, t0 * C.tstride) : 0;
     for (int64_t ti = t0; ti < t1; ++ti) {
         const int64_t t = ti * C.tstride;
+        const int64_t off = next_off;
+        const int32_t len = next_len;
+        if (ti + 1 < t1) {
+            next_off = load_uniform(C.btoff, t + C.tstride);
+            next_len = load_uniform(C.btlen, t + C.tstride);
+        }
         if constexpr (Epi::kPrune) {
-            if (epi.skip_tile(C.tmag[t], ri.amin, ri.amax)) continue;
+            if (epi.skip_tile(load_uniform(C.tmag, t), ri.amin, ri.amax)) continue;
         }
         const int64_t c = t * kWave + lane;
-        uint32_t P[GT][NB];
-        bits_accumulate_g<NB, GT>(tab, bt4 + (C.btoff[t] >> 1) + lane, C.btlen[t], P);
         const bool valid = c < C.n;
+        // the columns' magnitudes and positions, in flight during the probes
+        const int mb = valid ? C.mag[c] : 0;
+        const int64_t col = valid ? col_of(C, c) : 0;
+        uint32_t P[GT][NB];
+        bits_accumulate_g<NB, GT>(tab, bt4 + (off >> 1) + lane, len, P);
         // bit r of rows_any[q]: row r of table q has a non-zero count in some
         // lane of the tile
         uint32_t rows_any[GT];
@@ -1487,23 +1550,21 @@ __global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
             if constexpr (NB == 16) transpose8(P[q] + 8);
         }
         if constexpr (Epi::kPrune) epi.count_tile(R, C, valid, c);
-        const int mb = valid ? C.mag[c] : 0;
         const float mbf = (float)mb;
-        const int64_t col = valid ? col_of(C, c) : 0;
 #pragma unroll
         for (int q = 0; q < GT; ++q) {
 #pragma unroll
             for (int r = 0; r < siml::kBitRows; ++r) {
                 const int rr = q * siml::kBitRows + r;
-                const int32_t row = ri.row[rr];
-                if (row < 0) continue;
+                if (!((live[q] >> r) & 1u)) continue;
+                const int32_t row = __builtin_amdgcn_readlane(row_lane[q], r);
                 if (!((rows_any[q] >> r) & 1u)) {
                     epi.skip_row(el, R, rr, row, valid, col);
                     continue;
                 }
-                const int ma = ri.ma[rr];
+                const int ma = __builtin_amdgcn_readlane(ma_lane[q], r);
                 const float fi = count_of<NB>(P[q], r);
-                const float tv = pair_tanimoto<FAST>(fi, ri.maf[rr], mbf, ma == 0 || mb == 0);
+                const float tv = pair_tanimoto<FAST>(fi, (float)ma, mbf, ma == 0 || mb == 0);
                 epi.apply(el, R, rr, row, valid, col, ma, mb, tv, gw, nw, (int)fi);
             }
         }
@@ -2033,7 +2094,7 @@ struct siml_ctx {
     // (0: no seeding)
     int topk_seed = 0;
     // most row groups (tables) one bit-sliced workgroup serves: 1, 2 or 4
-    int max_tables = 4;
+    int max_tables = 2;
     bool autotune = false;
     DevBuf rowlist, sort_tmp, csr_dev, glist;
     std::vector<int32_t> h_glist;
@@ -2489,9 +2550,13 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
 // stream is read from memory once per that many groups.  Fewer tables when
 // the launch has few groups (an empty table costs its probes) or needs 16
 // bit planes (the planes of several tables exceed the registers).
-int bits_tables(const siml_ctx* ctx, const BitPlan& p) {
+// Measured on the MI355X: two tables per workgroup made the 1M-molecule
+// batched top-k 23% and the 10M-molecule histogram 13% faster than one,
+// four tables neither; the neighbour search prunes tiles by the
+// workgroup's magnitude band, which several groups widen, so it keeps one.
+int bits_tables(const siml_ctx* ctx, const BitPlan& p, bool pruned = false) {
     const size_t n = p.groups.size();
-    if (p.nb != 8 || n < 2 || ctx->max_tables < 2) return 1;
+    if (pruned || p.nb != 8 || n < 2 || ctx->max_tables < 2) return 1;
     return (n < 8 || ctx->max_tables < 4) ? 2 : 4;
 }
 
@@ -2507,6 +2572,10 @@ int launch_check(const char* name);
 
 // Top-k seeding samples every kSeedStride-th tile unless told otherwise.
 constexpr int64_t kSeedStride = 16;
+
+// Top-k candidate buffers of one call are held to this many bytes by
+// running fewer column ranges (at least one: k + 128 keys per row and wave).
+constexpr int64_t kTopkBufferBudget = int64_t(16) << 30;
 
 // Neighbour indices kept per (row, wave) slot by the count pass.
 constexpr int kNbrCap = 16;
@@ -2527,14 +2596,15 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
     const int32_t* rmag = ctx->ref.mag.as<int32_t>();
     const ColArgs C = cols ? *cols : ctx->qry->args();
     if constexpr (Epi::kBlockMode) {
+        const dim3 b8(block_waves<8>() * kWave), b16(block_waves<16>() * kWave);
         if (p.nb == 8 && fast)
-            k_bits_block<8, true><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits_block<8, true><<<grid, b8, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
         else if (p.nb == 8)
-            k_bits_block<8, false><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits_block<8, false><<<grid, b8, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
         else if (fast)
-            k_bits_block<16, true><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits_block<16, true><<<grid, b16, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
         else
-            k_bits_block<16, false><<<grid, dim3(kBlock), 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            k_bits_block<16, false><<<grid, b16, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
     } else {
         const dim3 block(kBlock * GT);
         if (p.nb == 8 && fast)
@@ -3242,7 +3312,7 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
-    const int gt = bp.use ? bits_tables(ctx, bp) : 1;
+    const int gt = bp.use ? bits_tables(ctx, bp, true) : 1;
     if (bp.use) {
         // bit-sliced workgroups serve gt groups of up to 32 rows: size
         // grid.y by workgroups, at most 2^24 (row, wave) slots
@@ -3387,8 +3457,13 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const int gt = bp.use ? bits_tables(ctx, bp) : 1;
     dim3 grid = row_grid(bp.use ? std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1) : nr,
                          ctx->qry->ntiles, bp.use ? ctx->target_wg : 2048);
-    grid.y = std::min<unsigned>(grid.y, std::max(1u, ctx->topk_wave_cap / (unsigned)gt));
-    if (bp.use) grid.y = xcd_gy(grid.y, std::min<unsigned>(std::max(1u, ctx->topk_wave_cap / (unsigned)gt), 65535));
+    // column ranges: at most the wave cap, and candidate buffers of at most
+    // kTopkBufferBudget bytes (one range always)
+    const int64_t range_bytes = nr * (kBlock / kWave) * gt * (int64_t)cap * (int64_t)sizeof(uint64_t);
+    const unsigned ycap = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)(ctx->topk_wave_cap / (unsigned)gt), kTopkBufferBudget / range_bytes, 65535}));
+    grid.y = std::min<unsigned>(grid.y, ycap);
+    if (bp.use) grid.y = xcd_gy(grid.y, ycap);
     int64_t nw = (int64_t)grid.y * (kBlock / kWave) * gt;
     const int64_t nslots = nr * nw;
     const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;

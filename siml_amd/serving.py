@@ -80,7 +80,7 @@ class Server:
     near saturation (2500 batches/s).
     """
 
-    def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False, group_tables=4):
+    def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False, group_tables=2):
         self.db = db
         self.max_rows = int(max_rows)
         self.mem_budget = int(mem_budget)
