@@ -160,6 +160,20 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
  * to 999; exact int64 counts. */
 int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
 
+/* Device memory of one call (no counterpart in the reference): an upper
+ * bound of the bytes a call of operation op over `rows` reference rows
+ * against the current query set allocates on the device beyond the
+ * reference set itself -- candidate buffers, neighbour slots, selection
+ * levels and the result -- from the launch shapes the call may take (every
+ * candidate shape while the autotuner is on).  The indices of neighbour
+ * lists depend on the data and are not included (4 bytes per neighbour).
+ * A handle keeps its largest buffers between calls.  k is the top-k k
+ * (ignored otherwise). */
+#define SIML_OP_TOPK 1
+#define SIML_OP_NEIGHBORS 2
+#define SIML_OP_HIST1000 3
+int siml_call_bytes(siml_ctx* ctx, int op, int64_t rows, int32_t k, int64_t* bytes);
+
 /* Pending-result protocol (GPULingo.py:537-648). */
 int siml_query_done(siml_ctx* ctx, int* done);   /* asyncOperationsDone */
 int siml_synchronize(siml_ctx* ctx);

@@ -24,6 +24,7 @@ MAX_ROW_LINGOS = 8192
 MAX_TOPK = 256
 POLICY_AUTO, POLICY_MERGE = 0, 1
 LAYOUT_NATURAL, LAYOUT_BLOCK, LAYOUT_SORTED = 0, 1, 2
+OP_TOPK, OP_NEIGHBORS, OP_HIST1000 = 1, 2, 3
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -89,6 +90,7 @@ SIGNATURES = {
     "siml_launch_shape": (_int, [_vp, _i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "siml_set_launch_shape": (_int, [_vp, _i64, ctypes.c_int32, ctypes.c_int32]),
     "siml_set_group_tables": (_int, [_vp, _int]),
+    "siml_call_bytes": (_int, [_vp, _int, _i64, ctypes.c_int32, _i64p]),
 }
 
 _lib = None

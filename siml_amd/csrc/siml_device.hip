@@ -3184,6 +3184,96 @@ int siml_set_sizes(siml_ctx* ctx, int64_t* nref, int64_t* nquery) {
     return SIML_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Launch shape and scratch layout of a top-k call over nr rows.  `bits`:
+// the rows are served by the bit-sliced kernels in `groups` row groups, gt
+// of them per workgroup (the merge kernel for rows the groups do not take
+// runs gt times as many column ranges, so both index the same (row, wave)
+// candidate buffers); otherwise every row is on the merge kernel.
+struct TopkShape {
+    int gt = 1, cap = 0;
+    dim3 grid;
+    int64_t nw = 0;
+    size_t buf = 0, cnt = 0, s1 = 0, s2 = 0, c1 = 0, c2 = 0, seed = 0;
+    size_t scratch() const { return buf + cnt + s1 + s2 + c1 + c2 + seed; }
+};
+
+TopkShape topk_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, int gt, int32_t k, int64_t target_wg,
+                     unsigned wave_cap) {
+    TopkShape t;
+    t.gt = bits ? gt : 1;
+    t.cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
+    t.grid = row_grid(bits ? std::max<int64_t>((groups + t.gt - 1) / t.gt, 1) : nr, ctx->qry->ntiles,
+                      bits ? target_wg : 2048);
+    // column ranges: at most the wave cap, and candidate buffers of at most
+    // kTopkBufferBudget bytes (one range always)
+    const int64_t range_bytes = std::max<int64_t>(nr, 1) * (kBlock / kWave) * t.gt * (int64_t)t.cap *
+                                (int64_t)sizeof(uint64_t);
+    const unsigned ycap = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)(wave_cap / (unsigned)t.gt), kTopkBufferBudget / range_bytes, 65535}));
+    t.grid.y = std::min<unsigned>(t.grid.y, ycap);
+    if (bits) t.grid.y = xcd_gy(t.grid.y, ycap);
+    t.nw = (int64_t)t.grid.y * (kBlock / kWave) * t.gt;
+    const int64_t nslots = nr * t.nw;
+    auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    t.buf = sizeof(uint64_t) * (size_t)nslots * (size_t)t.cap;
+    t.cnt = r256(sizeof(int32_t) * (size_t)nslots);
+    // selection levels: segments of the first level and (ping-pong) of the
+    // later, smaller ones
+    const int64_t ns1 = t.nw > kTopkFan ? nr * ((t.nw + kTopkFan - 1) / kTopkFan) : 0;
+    const int64_t ns2 =
+        t.nw > kTopkFan * kTopkFan ? nr * ((t.nw + kTopkFan * kTopkFan - 1) / (kTopkFan * kTopkFan)) : 0;
+    t.s1 = r256(sizeof(uint64_t) * (size_t)ns1 * (size_t)k);
+    t.s2 = r256(sizeof(uint64_t) * (size_t)ns2 * (size_t)k);
+    t.c1 = r256(sizeof(int32_t) * (size_t)ns1);
+    t.c2 = r256(sizeof(int32_t) * (size_t)ns2);
+    t.seed = sizeof(uint64_t) * (size_t)nr;
+    return t;
+}
+
+// Launch shape and scratch layout of a neighbour search over nr rows (see
+// topk_shape; the bit-sliced neighbour kernels use one table per workgroup
+// unless gt says otherwise).  `one_pass`: the count pass also buffers
+// kNbrCap indices per (row, wave) slot.
+struct NbrShape {
+    int gt = 1;
+    dim3 grid;
+    int64_t nw = 0;
+    size_t counts = 0, offs = 0, nbuf = 0;
+    size_t scratch() const { return counts + offs + nbuf; }
+};
+
+NbrShape nbr_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, int gt, bool one_pass,
+                   int64_t target_wg) {
+    NbrShape t;
+    t.gt = bits ? gt : 1;
+    t.grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
+    if (bits) {
+        // bit-sliced workgroups serve gt groups of up to 32 rows: size
+        // grid.y by workgroups, at most 2^24 (row, wave) slots
+        const int64_t slots = std::max<int64_t>((groups + t.gt - 1) / t.gt, 1);
+        const unsigned gyb = row_grid(slots, ctx->qry->ntiles, target_wg).y;
+        const int64_t cap =
+            std::min<int64_t>(65535 / t.gt, (int64_t(1) << 24) / (std::max<int64_t>(nr, 1) * (kBlock / kWave) * t.gt));
+        t.grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
+    }
+    // waves per row: the merge kernel (4 waves per workgroup) runs gt times
+    // as many column ranges so that both kernels index the same slots
+    t.nw = (int64_t)t.grid.y * (kBlock / kWave) * t.gt;
+    const int64_t nslots = nr * t.nw;
+    t.counts = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
+    t.offs = (sizeof(int64_t) * (size_t)nslots + 255) & ~size_t(255);
+    t.nbuf = one_pass ? sizeof(int32_t) * (size_t)nslots * kNbrCap : 0;
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
 static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleRows"));
     TRY(set_device(ctx));
@@ -3311,19 +3401,14 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    dim3 grid = row_grid(nr, ctx->qry->ntiles, 8192, int64_t(1) << 24);
-    const int gt = bp.use ? bits_tables(ctx, bp, true) : 1;
-    if (bp.use) {
-        // bit-sliced workgroups serve gt groups of up to 32 rows: size
-        // grid.y by workgroups, at most 2^24 (row, wave) slots
-        const int64_t slots = std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1);
-        const unsigned gyb = row_grid(slots, ctx->qry->ntiles, ctx->target_wg).y;
-        const int64_t cap = std::min<int64_t>(65535 / gt, (int64_t(1) << 24) / (nr * (kBlock / kWave) * gt));
-        grid.y = xcd_gy((unsigned)std::max<int64_t>(1, std::min<int64_t>(gyb, cap)), (unsigned)std::max<int64_t>(cap, 1));
-    }
-    // waves per row: the merge kernel (4 waves per workgroup) runs gt times
-    // as many column ranges so that both kernels index the same slots
-    const int64_t nw = (int64_t)grid.y * (kBlock / kWave) * gt;
+    // single pass when every row is on the bit-sliced kernel: the count pass
+    // keeps the first kNbrCap indices of each slot
+    const bool one_pass = bp.use && mrows.empty();
+    const NbrShape shp = nbr_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), bp.use ? bits_tables(ctx, bp, true) : 1,
+                                   one_pass, ctx->target_wg);
+    const dim3 grid = shp.grid;
+    const int gt = shp.gt;
+    const int64_t nw = shp.nw;
     int32_t* d_nbuf = nullptr;
     const dim3 mgrid((unsigned)std::max<size_t>(mrows.size(), 1), grid.y * gt);
     unsigned long long* work = ctx->timing ? ctx->work.as<unsigned long long>() : nullptr;
@@ -3367,13 +3452,8 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
         return SIML_OK;
     };
     const int64_t nslots = nr * nw;
-    const size_t counts_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
-    const size_t offs_bytes = (sizeof(int64_t) * (size_t)nslots + 255) & ~size_t(255);
-    // single pass when every row is on the bit-sliced kernel: the count pass
-    // keeps the first kNbrCap indices of each slot
-    const bool one_pass = bp.use && mrows.empty();
-    const size_t nbuf_bytes = one_pass ? sizeof(int32_t) * (size_t)nslots * kNbrCap : 0;
-    TRY(ctx->scratch.reserve(counts_bytes + offs_bytes + nbuf_bytes));
+    const size_t counts_bytes = shp.counts, offs_bytes = shp.offs;
+    TRY(ctx->scratch.reserve(shp.scratch()));
     int32_t* d_counts = ctx->scratch.as<int32_t>();
     int64_t* d_offs = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + counts_bytes);
     d_nbuf = one_pass ? reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + counts_bytes + offs_bytes) : nullptr;
@@ -3444,40 +3524,20 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    const int cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
     // every row on the bit-sliced kernel with consistent magnitudes: T is in
     // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
     // final selection fills up with T == 0 columns in index order)
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
     static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
                   "top-k buffer exceeds wave registers");
-    // bit-sliced workgroups serve gt groups each (4 gt waves); the merge
-    // kernel runs gt times as many column ranges so that both index the
-    // same (row, wave) candidate buffers
-    const int gt = bp.use ? bits_tables(ctx, bp) : 1;
-    dim3 grid = row_grid(bp.use ? std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1) : nr,
-                         ctx->qry->ntiles, bp.use ? ctx->target_wg : 2048);
-    // column ranges: at most the wave cap, and candidate buffers of at most
-    // kTopkBufferBudget bytes (one range always)
-    const int64_t range_bytes = nr * (kBlock / kWave) * gt * (int64_t)cap * (int64_t)sizeof(uint64_t);
-    const unsigned ycap = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>({(int64_t)(ctx->topk_wave_cap / (unsigned)gt), kTopkBufferBudget / range_bytes, 65535}));
-    grid.y = std::min<unsigned>(grid.y, ycap);
-    if (bp.use) grid.y = xcd_gy(grid.y, ycap);
-    int64_t nw = (int64_t)grid.y * (kBlock / kWave) * gt;
-    const int64_t nslots = nr * nw;
-    const size_t buf_bytes = sizeof(uint64_t) * (size_t)nslots * (size_t)cap;
-    const size_t cnt_bytes = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
-    // selection levels: segments of the first level and (ping-pong) of the
-    // later, smaller ones
-    const int64_t ns1 = nw > kTopkFan ? nr * ((nw + kTopkFan - 1) / kTopkFan) : 0;
-    const int64_t ns2 = nw > kTopkFan * kTopkFan ? nr * ((nw + kTopkFan * kTopkFan - 1) / (kTopkFan * kTopkFan)) : 0;
-    const size_t s1_bytes = (sizeof(uint64_t) * (size_t)ns1 * (size_t)k + 255) & ~size_t(255);
-    const size_t s2_bytes = (sizeof(uint64_t) * (size_t)ns2 * (size_t)k + 255) & ~size_t(255);
-    const size_t c1_bytes = (sizeof(int32_t) * (size_t)ns1 + 255) & ~size_t(255);
-    const size_t c2_bytes = (sizeof(int32_t) * (size_t)ns2 + 255) & ~size_t(255);
-    TRY(ctx->scratch.reserve(buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes + c2_bytes +
-                             sizeof(uint64_t) * (size_t)nr));
+    const TopkShape shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), bp.use ? bits_tables(ctx, bp) : 1, k,
+                                     ctx->target_wg, ctx->topk_wave_cap);
+    const int cap = shp.cap, gt = shp.gt;
+    const dim3 grid = shp.grid;
+    int64_t nw = shp.nw;
+    const size_t buf_bytes = shp.buf, cnt_bytes = shp.cnt, s1_bytes = shp.s1, s2_bytes = shp.s2,
+                 c1_bytes = shp.c1, c2_bytes = shp.c2;
+    TRY(ctx->scratch.reserve(shp.scratch()));
     char* base = ctx->scratch.as<char>();
     uint64_t* d_seed = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes +
                                                    c2_bytes);
@@ -3631,6 +3691,13 @@ struct TuneShape {
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, int>, TuneShape> g_tune;
 
+// Candidate launch shapes: the first 6 apply to every operation, the rest
+// to top-k only.
+const TuneShape kCands[] = {{8192, 256, 0},    {2048, 256, 0},  {4096, 256, 0},  {16384, 256, 0},
+                            {32768, 1024, 0},  {65536, 1024, 0}, {8192, 64, 0},   {8192, 256, 16},
+                            {2048, 256, 16},   {32768, 1024, 16}, {8192, 256, 4}, {2048, 256, 4}};
+int tune_candidates(int op) { return op == kTuneTopk ? 12 : 6; }
+
 int log2_bucket(int64_t x) {
     int b = 0;
     while (x > 1) {
@@ -3654,12 +3721,7 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
             return impl();
         }
     }
-    // the first 6 apply to every operation, the rest to top-k only
-    static const TuneShape kCands[] = {{8192, 256, 0},   {2048, 256, 0},    {4096, 256, 0},
-                                       {16384, 256, 0},  {32768, 1024, 0},  {65536, 1024, 0},
-                                       {8192, 64, 0},    {8192, 256, 16},   {2048, 256, 16},
-                                       {32768, 1024, 16}, {8192, 256, 4},   {2048, 256, 4}};
-    const int ncand = op == kTuneTopk ? 12 : 6;
+    const int ncand = tune_candidates(op);
     TRY(set_device(ctx));
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -3767,6 +3829,46 @@ int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_ca
     if (topk_seed < 0 || topk_seed > 64) return fail(SIML_EINVAL, "top-k seed stride %d outside [0, 64]", topk_seed);
     // 1 asks for seeding at the default stride
     ctx->topk_seed = topk_seed == 1 ? (int)kSeedStride : topk_seed;
+    return SIML_OK;
+}
+
+int siml_call_bytes(siml_ctx* ctx, int op, int64_t rows, int32_t k, int64_t* bytes) {
+    if (!ctx || !bytes) return fail(SIML_EINVAL, "null pointer");
+    if (ctx->qry->n < 0) return fail(SIML_ESTATE, "siml_call_bytes: no query set");
+    if (rows < 0) return fail(SIML_EINVAL, "siml_call_bytes: negative row count");
+    if (op == SIML_OP_TOPK && (k < 1 || k > SIML_MAX_TOPK))
+        return fail(SIML_EINVAL, "siml_call_bytes: k=%d outside [1, %d]", k, SIML_MAX_TOPK);
+    if (op != SIML_OP_TOPK && op != SIML_OP_NEIGHBORS && op != SIML_OP_HIST1000)
+        return fail(SIML_EINVAL, "siml_call_bytes: unknown operation %d", op);
+    // launch shapes the call may take: the handle's, or every candidate of
+    // the autotuner (a bucket's first call runs them all)
+    std::vector<std::pair<int64_t, unsigned>> shapes{{ctx->target_wg, ctx->topk_wave_cap}};
+    if (ctx->autotune) {
+        const int top = op == SIML_OP_TOPK ? kTuneTopk : kTuneNeighbors;
+        for (int i = 0; i < tune_candidates(top); ++i) shapes.emplace_back(kCands[i].target_wg, kCands[i].topk_wave_cap);
+    }
+    // plans: the fewest row groups the rows can form (the most column
+    // ranges per group), with each table count allowed, or every row on the
+    // merge kernel; scratch only shrinks as groups grow
+    const int64_t gmin = (rows + siml::kBitRows - 1) / siml::kBitRows;
+    std::vector<std::pair<bool, int>> plans{{false, 1}, {true, 1}};
+    if (ctx->max_tables >= 2 && gmin >= 2) plans.emplace_back(true, 2);
+    if (ctx->max_tables >= 4 && gmin >= 8) plans.emplace_back(true, 4);
+    size_t most = 0;
+    for (const auto& sh : shapes)
+        for (const auto& pl : plans) {
+            size_t b = 0;
+            if (op == SIML_OP_TOPK)
+                b = topk_shape(ctx, rows, pl.first, gmin, pl.second, k, sh.first, sh.second).scratch();
+            else if (op == SIML_OP_NEIGHBORS)
+                b = nbr_shape(ctx, rows, pl.first, gmin, 1, true, sh.first).scratch();
+            most = std::max(most, b);
+        }
+    // results on the device and the per-call lists (row list, group list)
+    size_t res = sizeof(uint64_t) * 1000;
+    if (op == SIML_OP_TOPK) res = (sizeof(int32_t) + sizeof(float)) * (size_t)rows * (size_t)k;
+    if (op == SIML_OP_NEIGHBORS) res = sizeof(int64_t) * ((size_t)rows + 1);
+    *bytes = (int64_t)(most + res + sizeof(int32_t) * ((size_t)rows + (size_t)gmin + kMaxTables) + 256);
     return SIML_OK;
 }
 

@@ -124,17 +124,67 @@ def test_serving_alone_equals_mix(db_and_queries):
 
 
 def test_serving_admission_budget(db_and_queries):
-    """A memory budget smaller than two requests still serves every request
-    (one at a time)."""
+    """A memory budget of 1.5 times the largest request's charge serves
+    every request, one at a time; a budget below a request's charge rejects
+    it with MemoryError."""
     db_set, q = db_and_queries
     db = LingoDB(*db_set, layout=_lib.LAYOUT_SORTED)
     reqs = _requests(q, [30, 30, 30, 30])
-    with Server(db, slots=2, max_rows=64, mem_budget=1) as srv:
+    with Server(db, slots=2, max_rows=64, mem_budget=1 << 40) as probe:
+        charges = [probe.memory_estimate(r) for r in reqs]
+    for r in reqs:
+        r.charge = None
+    with Server(db, slots=2, max_rows=64, mem_budget=int(max(charges) * 1.5)) as srv:
         for r in reqs:
             srv.submit(r)
         for r in reqs:
             r.event.wait()
             assert r.error is None and r.result is not None
+    small = _requests(q, [30])[0]
+    with Server(db, slots=1, mem_budget=min(charges) // 2) as srv:
+        srv.submit(small).event.wait()
+    assert isinstance(small.error, MemoryError)
+    db.close()
+
+
+def _free_device_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+@pytest.mark.parametrize("op", ["topk", "threshold"])
+def test_call_bytes_bounds_device_allocation(db_and_queries, op):
+    """The device memory one call allocates (free memory before and after,
+    on a fresh comparator whose reference set is already uploaded) stays
+    within siml_call_bytes' bound, for the default and the autotuned
+    launch shapes."""
+    import ctypes
+    db_set, q = db_and_queries
+    db = LingoDB(*db_set)
+    sub = synth.ragged_slice(*q, 0, 600)
+    for tune in (False, True):
+        g = GPULingo(0)
+        g.attach_db(db)
+        g.set_autotune(tune)
+        g.set_refsmiles_ragged(*sub)
+        bound = ctypes.c_int64()
+        code = _lib.OP_TOPK if op == "topk" else _lib.OP_NEIGHBORS
+        g._h.call("siml_call_bytes", code, 600, 100 if op == "topk" else 0, ctypes.byref(bound))
+        before = _free_device_bytes()
+        if op == "topk":
+            g.getTopK(0, 600, 100)
+        else:
+            offs, idx = g.getNeighborsCSR(0, 600, 0.9)
+        used = before - _free_device_bytes()
+        slack = 8 << 20   # allocation granularity of a handful of buffers
+        nbr = 4 * len(idx) if op == "threshold" else 0
+        print(f"{op} autotune={tune}: allocated {used} bytes, bound {bound.value}", flush=True)
+        assert bound.value > 0 and used <= bound.value + nbr + slack
+        del g
     db.close()
 
 
