@@ -148,7 +148,10 @@ int siml_hist_rows_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
 /* gpuGetNeighbors_multirow via getNeighbors_async (GPULingo.py:466-514,
  * kernels.py:393-475): for each row, every query column with
  * fl32(min(ma,mb)/(float)max(ma,mb)) >= lo and lo <= T < hi, ascending.
- * The result is complete CSR (the reference truncates at its pitch). */
+ * The result is complete CSR (the reference truncates at its pitch).  The
+ * call waits for the count pass (the host reads back three totals to size
+ * the index array); offsets are scanned on the device, and the fill,
+ * compaction and per-row sort are queued on the stream. */
 int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float lo, float hi);
 /* Top-k per row (no counterpart in the reference): larger T first, equal T
  * ordered by smaller column index; rows with fewer than k columns padded
@@ -166,7 +169,8 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
  * reference set itself -- candidate buffers, neighbour slots, selection
  * levels and the result -- from the launch shapes the call may take (every
  * candidate shape while the autotuner is on).  The indices of neighbour
- * lists depend on the data and are not included (4 bytes per neighbour).
+ * lists depend on the data and are not included (4 bytes per neighbour, 8
+ * when the query set is stored in a permuted order).
  * A handle keeps its largest buffers between calls.  k is the top-k k
  * (ignored otherwise). */
 #define SIML_OP_TOPK 1

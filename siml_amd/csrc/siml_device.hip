@@ -34,6 +34,7 @@
 
 #include "bitgroups.h"
 #include "host_threads.h"
+#include "csr.h"
 #include "siml_internal.h"
 
 namespace siml {
@@ -49,11 +50,6 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-// csr_sort.hip: sorts each segment [d_offsets[s], d_offsets[s+1]) of keys_in
-// ascending into keys_out; with d_temp == nullptr only *temp_bytes is set.
-int sort_segments_i32(const int32_t* keys_in, int32_t* keys_out, int64_t n, int64_t nseg, const int64_t* d_offsets,
-                      void* d_temp, size_t* temp_bytes, hipStream_t stream);
 
 }  // namespace siml
 
@@ -2067,7 +2063,7 @@ struct siml_ctx {
     int64_t rows = 0, cols = 0, nnz = 0;
     int64_t dpitch = 0;  // row pitch (elements) of a matrix result in res
     DevBuf res, res2, scratch;
-    std::vector<int64_t> csr_rows;  // host row offsets of a CSR result
+    bool csr_on_device = false;     // a CSR result's row offsets are in csr_dev
     // timing
     bool timing = false;
     // (start, selection start or null, end) of each timed operation
@@ -2096,7 +2092,7 @@ struct siml_ctx {
     // most row groups (tables) one bit-sliced workgroup serves: 1, 2 or 4
     int max_tables = 2;
     bool autotune = false;
-    DevBuf rowlist, sort_tmp, csr_dev, glist;
+    DevBuf rowlist, csr_dev, glist;
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
     std::vector<char> h_stage;       // staging of strided result copies
@@ -2654,7 +2650,7 @@ int launch_check(const char* name) {
 void clear_result(siml_ctx* ctx) {
     ctx->kind = SIML_RESULT_NONE;
     ctx->rows = ctx->cols = ctx->nnz = ctx->dpitch = 0;
-    ctx->csr_rows.clear();
+    ctx->csr_on_device = false;
 }
 
 int zero_fill_i32(siml_ctx* ctx, int32_t* p, int64_t n) {
@@ -2913,7 +2909,6 @@ int siml_destroy(siml_ctx* ctx) {
     ctx->res2.release();
     ctx->scratch.release();
     ctx->rowlist.release();
-    ctx->sort_tmp.release();
     ctx->glist.release();
     ctx->csr_dev.release();
     ctx->work.release();
@@ -3242,8 +3237,8 @@ struct NbrShape {
     int gt = 1;
     dim3 grid;
     int64_t nw = 0;
-    size_t counts = 0, offs = 0, nbuf = 0;
-    size_t scratch() const { return counts + offs + nbuf; }
+    size_t counts = 0, offs = 0, nbuf = 0, scan = 0;
+    size_t scratch() const { return counts + offs + nbuf + scan; }
 };
 
 NbrShape nbr_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, int gt, bool one_pass,
@@ -3267,6 +3262,7 @@ NbrShape nbr_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, i
     t.counts = (sizeof(int32_t) * (size_t)nslots + 255) & ~size_t(255);
     t.offs = (sizeof(int64_t) * (size_t)nslots + 255) & ~size_t(255);
     t.nbuf = one_pass ? sizeof(int32_t) * (size_t)nslots * kNbrCap : 0;
+    t.scan = siml::csr_scan_bytes(nslots);
     return t;
 }
 
@@ -3391,7 +3387,6 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     ctx->kind = SIML_RESULT_CSR;
     ctx->rows = nr;
     ctx->cols = nq;
-    ctx->csr_rows.assign((size_t)nr + 1, 0);
     if (nr == 0 || nq == 0) {
         TRY(ctx->res.reserve(256));
         return SIML_OK;
@@ -3452,30 +3447,25 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
         return SIML_OK;
     };
     const int64_t nslots = nr * nw;
-    const size_t counts_bytes = shp.counts, offs_bytes = shp.offs;
     TRY(ctx->scratch.reserve(shp.scratch()));
-    int32_t* d_counts = ctx->scratch.as<int32_t>();
-    int64_t* d_offs = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + counts_bytes);
-    d_nbuf = one_pass ? reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + counts_bytes + offs_bytes) : nullptr;
+    char* sbase = ctx->scratch.as<char>();
+    int32_t* d_counts = reinterpret_cast<int32_t*>(sbase);
+    int64_t* d_offs = reinterpret_cast<int64_t*>(sbase + shp.counts);
+    d_nbuf = one_pass ? reinterpret_cast<int32_t*>(sbase + shp.counts + shp.offs) : nullptr;
+    void* d_scan = sbase + shp.counts + shp.offs + shp.nbuf;
+    TRY(ctx->csr_dev.reserve(sizeof(int64_t) * (size_t)(nr + 1)));
+    int64_t* d_rows = ctx->csr_dev.as<int64_t>();
     TimedRegion tr(ctx, nr * nq);
     TRY(launch(false, nullptr, nullptr));
-    std::vector<int32_t> h_counts((size_t)nslots);
-    HIP_TRY(hipMemcpyAsync(h_counts.data(), d_counts, sizeof(int32_t) * nslots, hipMemcpyDeviceToHost, ctx->stream));
+    // slot and row offsets on the device; the host reads back only the
+    // totals it needs to size the index array
+    TRY(siml::csr_scan(d_counts, nr, nw, d_offs, d_rows, d_scan, ctx->stream));
+    int64_t stats[3];
+    HIP_TRY(hipMemcpyAsync(stats, d_scan, sizeof(stats), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::vector<int64_t> h_offs((size_t)nslots);
-    int64_t acc = 0;
-    int32_t most = 0;
-    for (int64_t r = 0; r < nr; ++r) {
-        ctx->csr_rows[(size_t)r] = acc;
-        for (int64_t w = 0; w < nw; ++w) {
-            h_offs[(size_t)(r * nw + w)] = acc;
-            acc += h_counts[(size_t)(r * nw + w)];
-            most = std::max(most, h_counts[(size_t)(r * nw + w)]);
-        }
-    }
-    ctx->csr_rows[(size_t)nr] = acc;
+    const int64_t acc = stats[0], most = stats[1], longest = stats[2];
     ctx->nnz = acc;
-    HIP_TRY(hipMemcpyAsync(d_offs, h_offs.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
+    ctx->csr_on_device = true;
     TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(acc, 1)));
     if (one_pass && most <= kNbrCap) {
         if (acc > 0) {
@@ -3487,21 +3477,15 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     } else {
         TRY(launch(true, d_offs, ctx->res.as<int32_t>()));
     }
-    if (!ctx->qry->identity && acc > 0) {
+    if (!ctx->qry->identity && acc > 1 && longest > 1) {
         // indices were written in storage order: sort each row ascending
-        TRY(upload(ctx->csr_dev, ctx->csr_rows.data(), (size_t)nr + 1, ctx->stream));
         TRY(ctx->res2.reserve(sizeof(int32_t) * (size_t)acc));
-        size_t tmp_bytes = 0;
-        TRY(siml::sort_segments_i32(ctx->res.as<int32_t>(), ctx->res2.as<int32_t>(), acc, nr,
-                                    ctx->csr_dev.as<int64_t>(), nullptr, &tmp_bytes, ctx->stream));
-        TRY(ctx->sort_tmp.reserve(std::max<size_t>(tmp_bytes, 16)));
-        TRY(siml::sort_segments_i32(ctx->res.as<int32_t>(), ctx->res2.as<int32_t>(), acc, nr,
-                                    ctx->csr_dev.as<int64_t>(), ctx->sort_tmp.p, &tmp_bytes, ctx->stream));
-        std::swap(ctx->res, ctx->res2);
+        int32_t* sorted = nullptr;
+        TRY(siml::csr_sort_rows(ctx->res.as<int32_t>(), ctx->res2.as<int32_t>(), acc, nr, d_rows, longest, &sorted,
+                                ctx->stream));
+        if (sorted != ctx->res.as<int32_t>()) std::swap(ctx->res, ctx->res2);
     }
     tr.done();
-    // h_offs and csr_rows must outlive the async copies above.
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->last_kernel = bp.use ? "neighbors_bits" : "neighbors_merge";
     return SIML_OK;
 }
@@ -4059,7 +4043,10 @@ int siml_retrieve_csr(siml_ctx* ctx, int64_t* offsets, int32_t* indices) {
     if (ctx->kind != SIML_RESULT_CSR) return fail(SIML_ESTATE, "no pending neighbour result");
     if (!offsets) return fail(SIML_EINVAL, "null offsets buffer");
     TRY(set_device(ctx));
-    std::memcpy(offsets, ctx->csr_rows.data(), sizeof(int64_t) * ctx->csr_rows.size());
+    if (ctx->csr_on_device)
+        TRY(copy_d2h(ctx, offsets, ctx->csr_dev.p, ctx->rows + 1, sizeof(int64_t)));
+    else
+        std::fill(offsets, offsets + ctx->rows + 1, int64_t(0));
     if (ctx->nnz > 0) {
         if (!indices) return fail(SIML_EINVAL, "null indices buffer");
         TRY(copy_d2h(ctx, indices, ctx->res.p, ctx->nnz, sizeof(int32_t)));

@@ -373,6 +373,27 @@ def test_hist_and_neighbors_both_policies(demo_lines, demo_set, policy):
     assert np.array_equal(np.diff(offs), counts) and np.array_equal(idx, np.concatenate(lists))
 
 
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_SORTED])
+def test_neighbor_lists_longer_than_sort_tiles(layout):
+    """lo = 0 makes all 20000 columns neighbours of every row: rows of five
+    4096-entry sort tiles and more, merged over three passes; the higher
+    thresholds give ragged rows that straddle tile boundaries, and lo > 1
+    gives no neighbour at all.  Slot and row offsets come from the device
+    scan; lists are complete and ascending in both storage orders."""
+    g = GPULingo(0)
+    g._h.call("siml_set_layout", layout)
+    qry = synth.packed_padded(20000, 10, 90, seed=17)
+    ref = synth.packed_padded(75, 10, 90, seed=18)
+    _set(g, ref, qry)
+    for lo, base, limit in ((0.0, 0, 75), (0.3, 0, 75), (0.45, 5, 70), (1.05, 0, 75)):
+        offs, idx = g.getNeighborsCSR(base, limit, lo)
+        lists, counts = oracle.neighbors(tuple(x[base:limit] for x in ref), qry, lo)
+        assert offs[0] == 0 and np.array_equal(np.diff(offs), counts), lo
+        want = np.concatenate(lists) if len(lists) else np.zeros(0, np.int32)
+        assert np.array_equal(idx, want), lo
+    assert offs[-1] == 0
+
+
 @pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK, _lib.LAYOUT_SORTED])
 def test_layouts_bit_identical(demo_lines, demo_set, layout):
     """Column storage order (natural or length-sorted blocks) changes no bit
