@@ -162,6 +162,23 @@ int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k)
  * the block, float32 edges fl32(k/1000), bin = max{k : T >= edge_k} clipped
  * to 999; exact int64 counts. */
 int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
+/* Top-k with no device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * result of siml_topk_async over the same rows, written on the handle's
+ * stream to caller-owned device arrays idx (int32, rows x k) and t
+ * (float32, rows x k); the call returns once the work is queued
+ * (siml_synchronize waits) and leaves the handle's pending result alone.
+ * Candidate buffers, selection levels and the row and group lists live in
+ * a static device array of SIML_NOALLOC_SCRATCH_BYTES, one per GPU, held
+ * until the process ends; the launch runs fewer column ranges until they
+ * fit, and a call that does not fit at one range fails with SIML_ELIMIT.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (from any handle or stream) race.  The autotuner and kernel timing
+ * are not used.  The sets must already be on the device (set_ref /
+ * set_query / attach_db allocate). */
+#define SIML_NOALLOC_SCRATCH_BYTES (128u << 20)
+int siml_topk_noalloc(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k, int32_t* idx, float* t);
 
 /* Device memory of one call (no counterpart in the reference): an upper
  * bound of the bytes a call of operation op over `rows` reference rows

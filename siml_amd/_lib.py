@@ -25,6 +25,7 @@ MAX_TOPK = 256
 POLICY_AUTO, POLICY_MERGE = 0, 1
 LAYOUT_NATURAL, LAYOUT_BLOCK, LAYOUT_SORTED = 0, 1, 2
 OP_TOPK, OP_NEIGHBORS, OP_HIST1000 = 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -66,6 +67,7 @@ SIGNATURES = {
     "siml_neighbors_async": (_int, [_vp, _i64, _i64, ctypes.c_float, ctypes.c_float]),
     "siml_topk_async": (_int, [_vp, _i64, _i64, ctypes.c_int32]),
     "siml_hist1000_async": (_int, [_vp, _i64, _i64]),
+    "siml_topk_noalloc": (_int, [_vp, _i64, _i64, ctypes.c_int32, _vp, _vp]),
     "siml_query_done": (_int, [_vp, ctypes.POINTER(_int)]),
     "siml_synchronize": (_int, [_vp]),
     "siml_result_info": (_int, [_vp, ctypes.POINTER(_int), _i64p, _i64p, _i64p]),

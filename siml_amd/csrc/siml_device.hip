@@ -2125,8 +2125,8 @@ struct TimedRegion {
     siml_ctx* ctx;
     hipEvent_t a = nullptr, m = nullptr, b = nullptr;
     int64_t pairs;
-    TimedRegion(siml_ctx* c, int64_t p) : ctx(c), pairs(p) {
-        if (ctx->timing) {
+    TimedRegion(siml_ctx* c, int64_t p, bool on = true) : ctx(c), pairs(p) {
+        if (on && ctx->timing) {
             a = take_event(ctx);
             b = take_event(ctx);
             if (a) (void)hipEventRecord(a, ctx->stream);
@@ -2503,13 +2503,16 @@ struct BitPlan {
     int nb = 8;
     int which = 0;  // grouping used
     std::vector<int32_t> groups, merge;
+    const int32_t* d_glist = nullptr;  // the group list in device memory
 };
 
 // `by_mag` asks for the magnitude-ordered grouping (built here on first
 // use).  Nothing is planned (use = false) when the policy is not AUTO, the
 // query set has no level stream, or the reference set was given under
-// another policy.
-int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false) {
+// another policy.  The group list goes to the handle's buffer, or to
+// glist_dst (room for every group of the grouping plus kMaxTables) when
+// given.
+int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false, int32_t* glist_dst = nullptr) {
     RowSet& R = ctx->ref;
     p = BitPlan();
     if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok || !R.bits[0].built) return SIML_OK;
@@ -2536,7 +2539,14 @@ int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = f
     if (!p.groups.empty()) {
         ctx->h_glist = p.groups;
         while (ctx->h_glist.size() % kMaxTables) ctx->h_glist.push_back(-1);
-        TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
+        if (glist_dst) {
+            HIP_TRY(hipMemcpyAsync(glist_dst, ctx->h_glist.data(), sizeof(int32_t) * ctx->h_glist.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            p.d_glist = glist_dst;
+        } else {
+            TRY(upload(ctx->glist, ctx->h_glist.data(), ctx->h_glist.size(), ctx->stream));
+            p.d_glist = ctx->glist.as<int32_t>();
+        }
     }
     return SIML_OK;
 }
@@ -2586,8 +2596,7 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
     if (GT > 1 && p.nb != 8) return fail(SIML_EINVAL, "%s: %d tables per workgroup need 8 bit planes", name, GT);
     const dim3 grid((unsigned)((p.groups.size() + GT - 1) / GT), gy);
     const RowSet::BitGrouping& B = ctx->ref.bits[p.which];
-    const BitGroupArgs G{B.hdr.as<siml::BitGroupHdr>(), B.rows.as<int32_t>(), B.ent.as<uint4>(),
-                         ctx->glist.as<int32_t>()};
+    const BitGroupArgs G{B.hdr.as<siml::BitGroupHdr>(), B.rows.as<int32_t>(), B.ent.as<uint4>(), p.d_glist};
     const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
     const int32_t* rmag = ctx->ref.mag.as<int32_t>();
     const ColArgs C = cols ? *cols : ctx->qry->args();
@@ -3197,18 +3206,18 @@ struct TopkShape {
 };
 
 TopkShape topk_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, int gt, int32_t k, int64_t target_wg,
-                     unsigned wave_cap) {
+                     unsigned wave_cap, int64_t budget = kTopkBufferBudget) {
     TopkShape t;
     t.gt = bits ? gt : 1;
     t.cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
     t.grid = row_grid(bits ? std::max<int64_t>((groups + t.gt - 1) / t.gt, 1) : nr, ctx->qry->ntiles,
                       bits ? target_wg : 2048);
     // column ranges: at most the wave cap, and candidate buffers of at most
-    // kTopkBufferBudget bytes (one range always)
+    // `budget` bytes (one range always)
     const int64_t range_bytes = std::max<int64_t>(nr, 1) * (kBlock / kWave) * t.gt * (int64_t)t.cap *
                                 (int64_t)sizeof(uint64_t);
     const unsigned ycap = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>({(int64_t)(wave_cap / (unsigned)t.gt), kTopkBufferBudget / range_bytes, 65535}));
+        1, std::min<int64_t>({(int64_t)(wave_cap / (unsigned)t.gt), budget / range_bytes, 65535}));
     t.grid.y = std::min<unsigned>(t.grid.y, ycap);
     if (bits) t.grid.y = xcd_gy(t.grid.y, ycap);
     t.nw = (int64_t)t.grid.y * (kBlock / kWave) * t.gt;
@@ -3490,23 +3499,89 @@ static int neighbors_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit
     return SIML_OK;
 }
 
-static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
+// Scratch of siml_topk_noalloc: a static device array, one copy per GPU,
+// held until the process ends.  In host code the array's name is only a
+// stand-in for it; its device address on each GPU comes from
+// hipGetSymbolAddress with that GPU current, taken on the entry point's
+// first call there and kept per device.
+static __device__ __align__(256) unsigned char noalloc_scratch[SIML_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_noalloc_mu;
+static std::map<int, char*> g_noalloc_addr;
+
+// Device address of noalloc_scratch on `device`, which must be current.
+static int noalloc_scratch_addr(int device, char** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_addr.find(device);
+    if (it == g_noalloc_addr.end()) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+        if (!p) return fail(SIML_EDEVICE, "hipGetSymbolAddress gave no address for the scratch array");
+        it = g_noalloc_addr.emplace(device, static_cast<char*>(p)).first;
+    }
+    *out = it->second;
+    return SIML_OK;
+}
+
+// Device memory an allocation-free top-k call brings: a scratch area for
+// the group list, the row list and every buffer of the kernels, and the
+// result arrays.
+struct CallerMem {
+    char* scratch;
+    size_t cap;
+    int32_t* idx;
+    float* t;
+};
+
+// Top-k of rows [rowbase, rowlimit).  Without `mem`, device buffers are the
+// handle's (grown as needed) and the result is the handle's pending result;
+// with `mem`, nothing is allocated: the launch is narrowed to fewer column
+// ranges until its buffers fit mem->cap bytes, and the result goes to
+// mem->idx / mem->t.
+static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k, const CallerMem* mem) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getTopK"));
     if (k < 1 || k > SIML_MAX_TOPK) return fail(SIML_EINVAL, "getTopK: k=%d outside [1, %d]", k, SIML_MAX_TOPK);
     if (ctx->qry->n > 0x7FFFFFFF) return fail(SIML_ELIMIT, "getTopK: more than 2^31-1 query molecules");
     TRY(set_device(ctx));
-    clear_result(ctx);
     const int64_t nr = rowlimit - rowbase, nq = ctx->qry->n;
-    TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(nr * k, 1)));
-    TRY(ctx->res2.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * k, 1)));
-    ctx->kind = SIML_RESULT_TOPK;
-    ctx->rows = nr;
-    ctx->cols = k;
+    if (!mem) {
+        clear_result(ctx);
+        TRY(ctx->res.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(nr * k, 1)));
+        TRY(ctx->res2.reserve(sizeof(float) * (size_t)std::max<int64_t>(nr * k, 1)));
+        ctx->kind = SIML_RESULT_TOPK;
+        ctx->rows = nr;
+        ctx->cols = k;
+    }
     if (nr == 0) return SIML_OK;
+    auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    // caller memory: the group list and the row list first
+    size_t used = 0;
+    int32_t* glist_dst = nullptr;
+    int32_t* rowlist_dst = nullptr;
+    if (mem) {
+        glist_dst = reinterpret_cast<int32_t*>(mem->scratch);
+        used = r256(sizeof(int32_t) * (ctx->ref.bits[0].h_hdr.size() + kMaxTables));
+        rowlist_dst = reinterpret_cast<int32_t*>(mem->scratch + used);
+        used += r256(sizeof(int32_t) * (size_t)nr);
+        if (used > mem->cap)
+            return fail(SIML_ELIMIT, "getTopK: the row and group lists of %lld rows need %zu bytes of scratch, more "
+                        "than the %zu available", (long long)nr, used, mem->cap);
+    }
     BitPlan bp;
-    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp, false, glist_dst));
     const std::vector<int32_t>& mrows = bp.merge;
-    if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
+    const int32_t* d_rowlist = nullptr;
+    if (!mrows.empty()) {
+        if (mem) {
+            ctx->h_rowlist = mrows;
+            HIP_TRY(hipMemcpyAsync(rowlist_dst, ctx->h_rowlist.data(), sizeof(int32_t) * mrows.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            d_rowlist = rowlist_dst;
+        } else {
+            TRY(upload_rowlist(ctx, mrows));
+            d_rowlist = ctx->rowlist.as<int32_t>();
+        }
+    }
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     // every row on the bit-sliced kernel with consistent magnitudes: T is in
     // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
@@ -3514,15 +3589,31 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
     static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
                   "top-k buffer exceeds wave registers");
-    const TopkShape shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), bp.use ? bits_tables(ctx, bp) : 1, k,
-                                     ctx->target_wg, ctx->topk_wave_cap);
+    const int gt_plan = bp.use ? bits_tables(ctx, bp) : 1;
+    TopkShape shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), gt_plan, k, ctx->target_wg,
+                               ctx->topk_wave_cap);
+    if (mem) {
+        const size_t room = mem->cap - used;
+        for (int64_t budget = (int64_t)room; shp.scratch() > room; budget /= 2) {
+            if (shp.grid.y <= 1)
+                return fail(SIML_ELIMIT, "getTopK: %lld rows with k=%d need %zu bytes of scratch at the narrowest "
+                            "launch, more than the %zu available", (long long)nr, k, used + shp.scratch(), mem->cap);
+            shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), gt_plan, k, ctx->target_wg,
+                             ctx->topk_wave_cap, budget);
+        }
+    }
     const int cap = shp.cap, gt = shp.gt;
     const dim3 grid = shp.grid;
     int64_t nw = shp.nw;
     const size_t buf_bytes = shp.buf, cnt_bytes = shp.cnt, s1_bytes = shp.s1, s2_bytes = shp.s2,
                  c1_bytes = shp.c1, c2_bytes = shp.c2;
-    TRY(ctx->scratch.reserve(shp.scratch()));
-    char* base = ctx->scratch.as<char>();
+    char* base = nullptr;
+    if (mem) {
+        base = mem->scratch + used;
+    } else {
+        TRY(ctx->scratch.reserve(shp.scratch()));
+        base = ctx->scratch.as<char>();
+    }
     uint64_t* d_seed = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes +
                                                    c2_bytes);
     uint64_t* d_buf = reinterpret_cast<uint64_t*>(base);
@@ -3559,7 +3650,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
                                                                          pos_only ? 1 : 0, nq, out_idx, out_t);
         return launch_check("k_topk_final");
     };
-    TimedRegion tr(ctx, nr * nq);
+    TimedRegion tr(ctx, nr * nq, mem == nullptr);
     const uint64_t* seed = nullptr;
     const int64_t stride = ctx->topk_seed;
     if (bp.use && pos_only && stride > 1 && ctx->qry->ntiles >= 4 * stride) {
@@ -3592,7 +3683,7 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
         }));
         if (!mrows.empty()) {
             k_topk<<<dim3((unsigned)mrows.size(), grid.y * gt), dim3(kBlock), lds, ctx->stream>>>(
-                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>(), rowbase, k, cap, d_buf, d_cnt);
+                ctx->ref.args(), ctx->qry->args(), d_rowlist, rowbase, k, cap, d_buf, d_cnt);
             TRY(launch_check("k_topk<rowlist>"));
         }
     } else {
@@ -3601,10 +3692,14 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
         TRY(launch_check("k_topk"));
     }
     tr.select();
-    TRY(select(ctx->res.as<int32_t>(), ctx->res2.as<float>(), nullptr));
+    TRY(select(mem ? mem->idx : ctx->res.as<int32_t>(), mem ? mem->t : ctx->res2.as<float>(), nullptr));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : "topk_merge";
     return SIML_OK;
+}
+
+static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
+    return topk_impl(ctx, rowbase, rowlimit, k, nullptr);
 }
 
 static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
@@ -3770,6 +3865,16 @@ int siml_neighbors_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, float
 int siml_topk_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
     return tuned_call(ctx, kTuneTopk, rowlimit - rowbase, k, [&] { return topk_async_impl(ctx, rowbase, rowlimit, k); });
+}
+
+int siml_topk_noalloc(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k, int32_t* idx, float* t) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    if (!idx || !t) return fail(SIML_EINVAL, "siml_topk_noalloc: null result array");
+    TRY(set_device(ctx));
+    char* scratch = nullptr;
+    TRY(noalloc_scratch_addr(ctx->device, &scratch));
+    const CallerMem mem{scratch, SIML_NOALLOC_SCRATCH_BYTES, idx, t};
+    return topk_impl(ctx, rowbase, rowlimit, k, &mem);
 }
 
 int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
