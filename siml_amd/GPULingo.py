@@ -148,7 +148,7 @@ class GPULingo:
         kernels serves; results do not depend on it."""
         self._h.call("siml_set_group_tables", int(max_tables))
 
-    def set_launch_shape(self, target_wg=8192, topk_wave_cap=256, topk_seed=0):
+    def set_launch_shape(self, target_wg=8192, topk_wave_cap=1024, topk_seed=16):
         """Fixes the launch shape (turns the autotuner off).  topk_seed: the
         top-k seed stride (0 or False: none; True: every 16th query tile;
         an int in [2, 64]: every topk_seed-th tile)."""
