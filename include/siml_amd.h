@@ -262,9 +262,10 @@ const char* siml_last_kernel(siml_ctx* ctx);
  * shape the handle used last; siml_set_launch_shape fixes one (and turns
  * the autotuner off): workgroups aimed at per launch, most waves per top-k
  * row, and the top-k seed stride: 0 = no seeding, s in [2, 64] = top-k first
- * seeds its row thresholds from every s-th query tile (1 means 16).
+ * seeds its row thresholds from every s-th query tile, 1 = a stride from
+ * the query set's size (its 64-molecule tiles / 512, clamped to [16, 64]).
  * Default (and after siml_set_autotune(ctx, 0)): 8192 workgroups, 1024
- * waves per top-k row, seed stride 16. */
+ * waves per top-k row, seed stride 1 (from the size). */
 int siml_set_autotune(siml_ctx* ctx, int on);
 int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap, int32_t* topk_seed);
 int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed);

@@ -148,10 +148,11 @@ class GPULingo:
         kernels serves; results do not depend on it."""
         self._h.call("siml_set_group_tables", int(max_tables))
 
-    def set_launch_shape(self, target_wg=8192, topk_wave_cap=1024, topk_seed=16):
+    def set_launch_shape(self, target_wg=8192, topk_wave_cap=1024, topk_seed=True):
         """Fixes the launch shape (turns the autotuner off).  topk_seed: the
-        top-k seed stride (0 or False: none; True: every 16th query tile;
-        an int in [2, 64]: every topk_seed-th tile)."""
+        top-k seed stride (0 or False: none; True or 1: a stride from the
+        query set's size, tiles / 512 clamped to [16, 64]; an int in
+        [2, 64]: every topk_seed-th tile)."""
         seed = 1 if topk_seed is True else int(topk_seed)
         self._h.call("siml_set_launch_shape", int(target_wg), int(topk_wave_cap), seed)
 
