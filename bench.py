@@ -45,6 +45,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 B_HBM = 8.0e12                     # bytes/s, MI355X HBM3E
+# int32 VALU lane-operations/s: 256 CUs x 4 SIMDs x 16 lanes, two VALU
+# instructions issued per SIMD per cycle (simple VOP2 forms), 2.4 GHz
+VALU_PEAK = 256 * 4 * 16 * 2 * 2.4e9
+N_CU, N_SIMD, N_SE = 256, 1024, 32   # shader engines: 8 XCDs x 4 (SQ_CYCLES is counted per engine)
 N_ALLPAIRS = 32768
 MAX_LEVEL = 15                     # levels per Lingo in the bit-sliced kernels
 
@@ -228,6 +232,24 @@ def roofline(alg_bytes, probe_bytes, kernel_s, peaks, counters=None):
            "sources_sha": sha}
     if counters is not None and counters.get("sources_sha") == sha:
         rec["counters"] = {k: v for k, v in counters.items() if k not in ("fetch_kib", "write_kib")}
+        c = counters
+        if "sq_cycles" in c:
+            # utilisation over the counter run's own launch: SQ_CYCLES per
+            # shader engine; VALU quad-cycles with one or two instructions
+            # issued (SQ_ACTIVE_INST_VALU counts each instruction once, one
+            # quad-cycle each; SQ_ACTIVE_INST_VALU2 the quad-cycles that
+            # issued two) over every SIMD's cycles
+            cyc = c["sq_cycles"] / N_SE
+            rec["valu"] = {
+                "busy_frac": (c["valu_active"] - c["valu_active2"]) * 4.0 / (N_SIMD * cyc),
+                "dual_issue_frac": 2.0 * c["valu_active2"] / c["valu_active"],
+                "insts_per_launch": c["valu_insts"],
+                "lane_ops_per_s": c["valu_insts"] * 64.0 / kernel_s,
+                "peak_lane_ops_per_s": VALU_PEAK,
+                "lane_ops_frac": c["valu_insts"] * 64.0 / kernel_s / VALU_PEAK}
+            rec["lds"]["busy_frac"] = c["lds_idx_active"] / (N_CU * cyc)
+            rec["lds"]["bank_conflict_frac"] = c["lds_bank_conflict"] / (N_CU * cyc)
+            rec["waves_per_simd"] = c["wave_cycles"] * 4.0 / (N_SIMD * cyc)
     if note:
         rec["traffic_note"] = note
     return rec
@@ -730,10 +752,56 @@ def workload_default(args):
     return rec
 
 
+# counter passes of a profile (rocprofv3 --pmc, one directory each) and the
+# name each counter takes in the `counters:` line
+COUNTER_PASSES = {
+    "fetch": {"FETCH_SIZE": "fetch_kib"},
+    "write": {"WRITE_SIZE": "write_kib"},
+    "sq1": {"SQ_INSTS_VALU": "valu_insts", "SQ_INSTS_LDS": "lds_insts", "SQ_LDS_BANK_CONFLICT": "lds_bank_conflict",
+            "SQ_WAVE_CYCLES": "wave_cycles", "SQ_WAVES": "waves", "SQ_INSTS_SALU": "salu_insts"},
+    "sq2": {"SQ_LDS_IDX_ACTIVE": "lds_idx_active", "SQ_LDS_ADDR_CONFLICT": "lds_addr_conflict",
+            "SQ_WAIT_ANY": "wait_any", "SQ_INSTS_VMEM_RD": "vmem_rd_insts", "SQ_INSTS_VMEM_WR": "vmem_wr_insts"},
+    "sq3": {"SQ_ACTIVE_INST_VALU": "valu_active", "SQ_ACTIVE_INST_VALU2": "valu_active2", "SQ_CYCLES": "sq_cycles"},
+}
+
+
+def counters_line(directory, kernel_regex):
+    """The `counters:` line of a profile from its rocprofv3 --pmc passes
+    (subdirectories named as COUNTER_PASSES): the last dispatch whose kernel
+    name matches kernel_regex, per pass, with the source hash of this tree."""
+    import csv
+    import re
+    pat = re.compile(kernel_regex)
+    out = {"sources_sha": sources_sha(), "fetch_factor": 2.0, "write_factor": 1.0, "kernel": None}
+    for sub, names in COUNTER_PASSES.items():
+        files = glob.glob(os.path.join(directory, sub, "*counter_collection.csv"))
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {os.path.join(directory, sub)}")
+        per, kname = {}, {}
+        with open(files[0]) as f:
+            for r in csv.DictReader(f):
+                if not pat.search(r["Kernel_Name"]):
+                    continue
+                d = int(r["Dispatch_Id"])
+                per.setdefault(d, {})
+                per[d][r["Counter_Name"]] = per[d].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+                kname[d] = r["Kernel_Name"]
+        if not per:
+            raise SystemExit(f"no dispatch of {kernel_regex!r} in {files[0]}")
+        last = max(per)
+        out["kernel"] = kname[last]
+        for counter, key in names.items():
+            out[key] = per[last][counter]
+    return "counters: " + json.dumps(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--counters-dir", default="",
+                    help="only print the `counters:` line of the rocprofv3 passes under this directory")
+    ap.add_argument("--counters-kernel", default="k_bits", help="with --counters-dir: kernel name regex")
     ap.add_argument("--workload", choices=["default", "allpairs", "zinc", "cpu4096", "batched", "threshold", "serve"],
                     default="default")
     ap.add_argument("--quick", action="store_true", help="default workload: the all-pairs line only")
@@ -766,6 +834,9 @@ def main():
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes)
+    if args.counters_dir:
+        print(counters_line(args.counters_dir, args.counters_kernel))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (the box has one); --gpus must be 1. "
                                    "Sharding across GPUs is described in DESIGN.md."}))

@@ -465,164 +465,215 @@ __global__ __launch_bounds__(kBlock) void k_topk(RowArgs R, ColArgs C, const int
 // Candidate buffers one selection workgroup takes (their keys are staged in
 // LDS).
 constexpr int kTopkFan = 16;
+// Buffers per selection workgroup when they are sparse (seeded passes).
+constexpr int kTopkStreamFan = 256;
+// Sampled tiles per wave of a top-k seeding pass, and the fewest
+// workgroups it runs on while its waves keep at least one tile each.
+constexpr int64_t kSampleTiles = 8;
+constexpr int64_t kSampleWorkgroups = 1024;
 
 // Dynamic LDS of a selection launch whose segments hold at most segcap
 // keys (at most 16 x 384 keys = 48 KiB, for k = 256): sized to the launch,
 // so more selection workgroups fit per CU for small k.
 inline size_t topk_lds(int segcap) { return sizeof(uint64_t) * (size_t)kTopkFan * (size_t)segcap; }
 
-// Workgroup radix select over the candidate segments [s0, s1) of one row
-// (segment w holds cnt[w] keys at buf + w * cap, s1 - s0 <= kTopkFan): the
-// keys are copied to LDS once, an MSB-first 8-bit radix select finds the
-// k-th largest, and `sorted` receives the min(k, total) largest keys
-// (unordered, rest 0).  Returns their number.
+// The k-th largest of the keys cand[0, total) (total > k; keys distinct and
+// non-zero), by an MSB-first 8-bit radix select in LDS.  Every thread of the
+// workgroup calls it; the result is uniform.
+__device__ uint64_t wg_kth_largest(const uint64_t* cand, uint32_t total, uint32_t k) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_prefix, s_pmask;
+    __shared__ unsigned long long s_min;
+    __shared__ uint32_t s_need, s_exact;
+    if (threadIdx.x == 0) {
+        s_prefix = 0ull;
+        s_pmask = 0ull;
+        s_need = k;
+    }
+    __syncthreads();
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint64_t prefix = s_prefix, pmask = s_pmask;
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const uint64_t key = cand[i];
+            const bool act = (key & pmask) == prefix;
+            const uint32_t dig = (uint32_t)(key >> shift) & 255u;
+            // the high digits of most keys are equal: when every active
+            // lane has the same digit, one atomic adds them all (equal-
+            // address atomics of one instruction serialise)
+            const uint64_t am = __ballot(act);
+            if (am == 0ull) continue;
+            const int leader = __builtin_ctzll(am);
+            const uint32_t d0 = (uint32_t)__shfl((int)dig, leader);
+            if (__ballot(act && dig == d0) == am) {
+                if (lane_id() == leader) atomicAdd(&hist[d0], (uint32_t)__popcll(am));
+            } else if (act) {
+                atomicAdd(&hist[dig], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kWave) {
+            // wave 0 finds the digit: suffix sums over 4 bins per lane
+            const int lane = threadIdx.x;
+            const uint32_t need = s_need;
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const uint32_t sl = h0 + h1 + h2 + h3;
+            uint32_t S = sl;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t t = __shfl_down(S, off);
+                if (lane + off < kWave) S += t;
+            }
+            const uint64_t ge = __ballot(S >= need);
+            const int L = 63 - __builtin_clzll(ge);
+            uint32_t above = __shfl(S - sl, L);
+            const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
+                                    (uint32_t)__shfl(h3, L)};
+            int d = 4 * L;
+            uint32_t cnt_d = hb[0];
+            for (int q = 3; q >= 0; --q) {
+                if (above + hb[q] >= need) {
+                    d = 4 * L + q;
+                    cnt_d = hb[q];
+                    break;
+                }
+                above += hb[q];
+            }
+            if (lane == 0) {
+                s_need = need - above;
+                s_prefix = prefix | ((uint64_t)d << shift);
+                s_pmask = pmask | ((uint64_t)255 << shift);
+                s_exact = cnt_d == need - above ? 1u : 0u;
+                s_min = ~0ull;
+            }
+        }
+        __syncthreads();
+        if (s_exact) {
+            // every key of the chosen bucket is needed: the k-th largest
+            // is the bucket's smallest key
+            const uint64_t bp = s_prefix, bm = s_pmask;
+            uint64_t mn = ~0ull;
+            for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+                const uint64_t key = cand[i];
+                if ((key & bm) == bp && key < mn) mn = key;
+            }
+            atomicMin(&s_min, (unsigned long long)mn);
+            __syncthreads();
+            break;
+        }
+    }
+    const uint64_t K = s_exact ? (uint64_t)s_min : s_prefix;
+    __syncthreads();
+    return K;
+}
+
+// Workgroup selection over the candidate segments [s0, s1) of one row
+// (segment w holds cnt[w] keys at buf + w * cap; keys distinct and
+// non-zero).  Segments are copied to LDS (`cand`, kTopkFan * cap keys) as
+// many at a time as fit; whenever more remain and the keys gathered exceed
+// k, they are cut to their k largest (radix select, then the keys >= the
+// k-th move to the front), so any number of segments streams through one
+// workgroup: one pass when they fit, as in a selection level of kTopkFan
+// segments.  `sorted` (kBlock keys) receives the min(k, total) largest keys
+// (unordered, rest 0); returns their number.  With kth, only the k-th
+// largest key is returned there (0 when there are fewer than k).
 __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32_t* __restrict__ cnt, int s0,
                                    int s1, int cap, int k, uint64_t* sorted, uint64_t* cand,
                                    uint64_t* kth = nullptr) {
-    __shared__ uint32_t hist[256];
-    __shared__ int32_t s_base[kTopkFan + 1];
-    __shared__ uint64_t s_prefix, s_pmask;
-    __shared__ unsigned long long s_min;
-    __shared__ uint32_t s_need, s_fill, s_exact;
-    const int nseg = s1 - s0;  // <= kTopkFan < kWave
-    if (threadIdx.x < kWave) {
-        // segment offsets: one wave prefix sum over the counts
-        const int lane = threadIdx.x;
-        const int n = lane < nseg ? cnt[s0 + lane] : 0;
-        int S = n;
-        for (int off = 1; off < kWave; off <<= 1) {
-            const int t = __shfl_up(S, off);
-            if (lane >= off) S += t;
-        }
-        if (lane <= nseg) s_base[lane] = S - n;
-        if (lane == 0) {
-            s_prefix = 0ull;
-            s_pmask = 0ull;
-            s_need = (uint32_t)k;
-            s_fill = 0u;
-        }
-    }
-    __syncthreads();
-    const uint32_t total = (uint32_t)s_base[nseg];
-    // keys to LDS: every element load is independent of the others, and a
-    // thread has four in flight
-    constexpr int kUnroll = 4;
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kUnroll * blockDim.x) {
-        uint64_t v[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const uint32_t i = i0 + u * blockDim.x;
-            if (i >= total) break;
-            int lo = 0, hi = nseg - 1;  // the segment w with s_base[w] <= i < s_base[w + 1]
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((uint32_t)s_base[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid - 1;
+    __shared__ int32_t s_base[kWave + 1];
+    __shared__ uint32_t s_take, s_fill;
+    const uint32_t room_all = (uint32_t)kTopkFan * (uint32_t)cap;
+    uint32_t carry = 0, total = 0;  // keys kept in cand[0, carry); uniform
+    for (int w0 = s0;;) {
+        const int nwin = min(kWave, s1 - w0);
+        if (threadIdx.x < kWave) {
+            // window of up to 64 segments: offsets by one wave prefix sum;
+            // the window's first segments whose keys fit the room left
+            // (at least one: room >= kTopkFan * cap - k >= cap)
+            const int lane = threadIdx.x;
+            const int n = lane < nwin ? cnt[w0 + lane] : 0;
+            int S = n;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int t = __shfl_up(S, off);
+                if (lane >= off) S += t;
             }
-            v[u] = buf[(int64_t)(s0 + lo) * cap + (i - (uint32_t)s_base[lo])];
+            if (lane < nwin) s_base[lane] = S - n;
+            if (lane == (nwin > 0 ? nwin - 1 : 0)) s_base[nwin] = nwin > 0 ? S : 0;
+            const uint64_t fits = __ballot(lane < nwin && (uint32_t)S <= room_all - carry);
+            if (lane == 0) {
+                s_take = (uint32_t)__popcll(fits);
+                s_fill = 0u;
+            }
         }
+        __syncthreads();
+        // m >= 1 whenever a segment is left (a segment holds at most cap keys);
+        // the loop ends after s1 - s0 segments whatever the counts say
+        const int m = nwin > 0 ? max((int)s_take, 1) : 0;
+        const uint32_t add = min((uint32_t)s_base[m], room_all - carry);
+        // keys to LDS: every element load is independent of the others, and
+        // a thread has four in flight
+        constexpr int kUnroll = 4;
+        for (uint32_t i0 = threadIdx.x; i0 < add; i0 += kUnroll * blockDim.x) {
+            uint64_t v[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const uint32_t i = i0 + u * blockDim.x;
-            if (i < total) cand[i] = v[u];
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t i = i0 + u * blockDim.x;
+                if (i >= add) break;
+                int lo = 0, hi = m - 1;  // the segment w with s_base[w] <= i < s_base[w + 1]
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((uint32_t)s_base[mid] <= i)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                v[u] = buf[(int64_t)(w0 + lo) * cap + (i - (uint32_t)s_base[lo])];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t i = i0 + u * blockDim.x;
+                if (i < add) cand[carry + i] = v[u];
+            }
         }
-    }
-    __syncthreads();
-    uint64_t K = 0ull;  // every key is > 0
-    if (total > (uint32_t)k) {
-        for (int shift = 56; shift >= 0; shift -= 8) {
-            hist[threadIdx.x] = 0u;
-            __syncthreads();
-            const uint64_t prefix = s_prefix, pmask = s_pmask;
+        __syncthreads();
+        total = carry + add;
+        w0 += m;
+        if (w0 >= s1) break;
+        if (total > (uint32_t)k) {
+            // more segments follow: keep the k largest at the front
+            const uint64_t K = wg_kth_largest(cand, total, (uint32_t)k);
             for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
                 const uint64_t key = cand[i];
-                const bool act = (key & pmask) == prefix;
-                const uint32_t dig = (uint32_t)(key >> shift) & 255u;
-                // the high digits of most keys are equal: when every active
-                // lane has the same digit, one atomic adds them all (equal-
-                // address atomics of one instruction serialise)
-                const uint64_t am = __ballot(act);
-                if (am == 0ull) continue;
-                const int leader = __builtin_ctzll(am);
-                const uint32_t d0 = (uint32_t)__shfl((int)dig, leader);
-                if (__ballot(act && dig == d0) == am) {
-                    if (lane_id() == leader) atomicAdd(&hist[d0], (uint32_t)__popcll(am));
-                } else if (act) {
-                    atomicAdd(&hist[dig], 1u);
-                }
+                if (key >= K) sorted[atomicAdd(&s_fill, 1u)] = key;
             }
             __syncthreads();
-            if (threadIdx.x < kWave) {
-                // wave 0 finds the digit: suffix sums over 4 bins per lane
-                const int lane = threadIdx.x;
-                const uint32_t need = s_need;
-                const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
-                               h3 = hist[4 * lane + 3];
-                const uint32_t sl = h0 + h1 + h2 + h3;
-                uint32_t S = sl;
-                for (int off = 1; off < kWave; off <<= 1) {
-                    const uint32_t t = __shfl_down(S, off);
-                    if (lane + off < kWave) S += t;
-                }
-                const uint64_t ge = __ballot(S >= need);
-                const int L = 63 - __builtin_clzll(ge);
-                uint32_t above = __shfl(S - sl, L);
-                const uint32_t hb[4] = {(uint32_t)__shfl(h0, L), (uint32_t)__shfl(h1, L), (uint32_t)__shfl(h2, L),
-                                        (uint32_t)__shfl(h3, L)};
-                int d = 4 * L;
-                uint32_t cnt_d = hb[0];
-                for (int q = 3; q >= 0; --q) {
-                    if (above + hb[q] >= need) {
-                        d = 4 * L + q;
-                        cnt_d = hb[q];
-                        break;
-                    }
-                    above += hb[q];
-                }
-                if (lane == 0) {
-                    s_need = need - above;
-                    s_prefix = prefix | ((uint64_t)d << shift);
-                    s_pmask = pmask | ((uint64_t)255 << shift);
-                    s_exact = cnt_d == need - above ? 1u : 0u;
-                    s_min = ~0ull;
-                }
-            }
+            for (int i = threadIdx.x; i < k; i += blockDim.x) cand[i] = sorted[i];
+            total = (uint32_t)k;
             __syncthreads();
-            if (s_exact) {
-                // every key of the chosen bucket is needed: the k-th largest
-                // is the bucket's smallest key
-                const uint64_t bp = s_prefix, bm = s_pmask;
-                uint64_t mn = ~0ull;
-                for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-                    const uint64_t key = cand[i];
-                    if ((key & bm) == bp && key < mn) mn = key;
-                }
-                atomicMin(&s_min, (unsigned long long)mn);
-                __syncthreads();
-                break;
-            }
         }
-        K = s_exact ? (uint64_t)s_min : s_prefix;
+        carry = total;
     }
+    uint64_t K = 0ull;  // every key is > 0
+    if (total > (uint32_t)k) K = wg_kth_largest(cand, total, (uint32_t)k);
     if (kth) {
         // only the k-th largest key is wanted (0 when there are fewer than k)
         if (total < (uint32_t)k) K = 0ull;
         if (total == (uint32_t)k) {
             // the smallest of exactly k keys
-            __shared__ uint64_t s_min;
-            if (threadIdx.x == 0) s_min = ~0ull;
+            __shared__ unsigned long long s_kmin;
+            if (threadIdx.x == 0) s_kmin = ~0ull;
             __syncthreads();
             uint64_t mn = ~0ull;
             for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) mn = cand[i] < mn ? cand[i] : mn;
-            atomicMin((unsigned long long*)&s_min, (unsigned long long)mn);
+            atomicMin(&s_kmin, (unsigned long long)mn);
             __syncthreads();
-            K = s_min;
+            K = s_kmin;
         }
         *kth = K;
         return total;
     }
+    if (threadIdx.x == 0) s_fill = 0u;
     sorted[threadIdx.x] = 0ull;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
@@ -633,17 +684,20 @@ __device__ uint32_t wg_select_topk(const uint64_t* __restrict__ buf, const int32
     return s_fill;
 }
 
-// One level of the per-row selection when a row has more than kTopkFan
-// candidate buffers: workgroup (row, s) reduces buffers
-// [s * kTopkFan, (s + 1) * kTopkFan) to their k largest keys.
+// One level of the per-row selection when a row has more candidate
+// buffers than one workgroup selects from: workgroup (row, s) reduces
+// buffers [s * fan, (s + 1) * fan) to their k largest keys (fan = kTopkFan
+// for full buffers; more when the buffers are sparse, which the workgroup
+// streams through its LDS).
 __global__ __launch_bounds__(kBlock) void k_topk_stage(const uint64_t* __restrict__ buf,
                                                        const int32_t* __restrict__ bufcnt, int nw, int cap, int k,
-                                                       uint64_t* __restrict__ out, int32_t* __restrict__ outcnt) {
+                                                       int fan, uint64_t* __restrict__ out,
+                                                       int32_t* __restrict__ outcnt) {
     extern __shared__ uint64_t topk_cand[];  // kTopkFan x cap keys (launch's dynamic LDS)
     __shared__ uint64_t sorted[kBlock];
     const int64_t r = blockIdx.x;
     const int s = blockIdx.y, ns = gridDim.y;
-    const int s0 = s * kTopkFan, s1 = min(nw, s0 + kTopkFan);
+    const int s0 = s * fan, s1 = min(nw, s0 + fan);
     const uint32_t have = wg_select_topk(buf + r * nw * (int64_t)cap, bufcnt + r * nw, s0, s1, cap, k, sorted,
                                          topk_cand);
     uint64_t* o = out + (r * ns + s) * (int64_t)k;
@@ -888,6 +942,7 @@ struct EpiF32 {
     static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    static constexpr bool kAdmit = false;
     using Rows = BitRows<32 * G>;
     float* out;
     int64_t opitch, rb;
@@ -927,6 +982,7 @@ struct EpiF32Block {
     static constexpr int kG = 1;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = true;
+    static constexpr bool kAdmit = false;
     float* out;
     int64_t opitch, rb;
     struct Lds {
@@ -943,6 +999,7 @@ struct EpiHist101 {
     static constexpr int kG = 1;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    static constexpr bool kAdmit = false;
     int32_t* hist;
     int64_t rb;
     struct Lds {
@@ -987,6 +1044,7 @@ struct EpiHist1000 {
     static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    static constexpr bool kAdmit = false;
     using Rows = BitRows<32 * G>;
     unsigned long long* hist;
     // Pairs with I in [1, kIMax] and U = ma + mb - I below kIU (most
@@ -1051,6 +1109,7 @@ struct EpiNbr {
     static constexpr int kG = G;
     static constexpr bool kPrune = true;
     static constexpr bool kBlockMode = false;
+    static constexpr bool kAdmit = false;
     static constexpr int kRows = 32 * G;
     using Rows = BitRows<kRows>;
     float lo, hi;
@@ -1155,6 +1214,7 @@ struct EpiTopk {
     static constexpr int kG = G;
     static constexpr bool kPrune = false;
     static constexpr bool kBlockMode = false;
+    static constexpr bool kAdmit = true;
     static constexpr int kRows = 32 * G;
     static constexpr int kRegs = (kRows + kWave - 1) / kWave;
     using Rows = BitRows<kRows>;
@@ -1166,14 +1226,45 @@ struct EpiTopk {
     const uint64_t* theta0;  // per row: a key at most the row's k-th largest (0: none), or null
     struct Lds {
         uint32_t hist[4 * G][256];
+        // per row: (1 + s, s, s * ma) of the admission test below
+        float4 adm[kRows];
     };
     struct Reg {
         uint64_t thr[kRegs];
         int32_t cnt[kRegs];
     };
     __device__ void init(Lds&) const {}
-    __device__ void begin(Lds&, Reg& R, const Rows& ri, int64_t, int64_t) const {
+    // Every wave calls begin once, after the workgroup prologue.
+    __device__ void begin(Lds& L, Reg& R, const Rows& ri, int64_t, int64_t) const {
         const int lane = lane_id();
+        // Admission constants.  A seeded row buffers only keys >= its seed,
+        // so only pairs with T >= th, the seed's T (the key's high word is
+        // T's bit pattern with the sign bit set for T >= 0).  With
+        // s = th (1 - 2^-10) - 2^-10, every such pair passes
+        // I (1 + s) >= s (ma + mb): T = fl(I / U) >= th gives
+        // I - s U >= 2^-11 U >= 2^-12 (ma + mb) in exact arithmetic (U >= I,
+        // so U >= (ma + mb) / 2), while the float evaluation in admits()
+        // errs by less than 2^-20 (ma + mb).  Rows without a seed (or with
+        // s <= 0) get (1, 0, 0): every pair passes.
+        if (threadIdx.x < kRows) {
+            const int i = threadIdx.x;
+            float a = 1.0f, b = 0.0f, c = 0.0f;
+            if (theta0 && ri.row[i] >= 0) {
+                const uint64_t s0 = theta0[(int64_t)ri.row[i] - rb];
+                const uint32_t hi = (uint32_t)(s0 >> 32);
+                if (s0 != 0ull && (hi & 0x80000000u)) {
+                    const float th = __uint_as_float(hi & 0x7FFFFFFFu);
+                    const float sl = th * (1.0f - 0x1p-10f) - 0x1p-10f;
+                    if (sl > 0.0f) {
+                        a = 1.0f + sl;
+                        b = sl;
+                        c = sl * (float)ri.ma[i];
+                    }
+                }
+            }
+            L.adm[i] = make_float4(a, b, c, 0.0f);
+        }
+        __syncthreads();
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) {
             // a seeded row admits only keys >= its seed: threshold = seed - 1
@@ -1211,6 +1302,14 @@ struct EpiTopk {
             R.thr[j] = thr;
             R.cnt[j] = cnt;
         }
+    }
+    // Whether some lane's pair of row r (intersection fi, column magnitude
+    // mbf) can pass the row's seed; wave-uniform.  Rows that cannot skip the
+    // division and apply().
+    __device__ bool admits(const Lds& L, int r, float fi, float mbf, bool valid) const {
+        if (!theta0) return true;
+        const float4 p = L.adm[r];
+        return __ballot(valid && fi * p.x >= fmaf(p.y, mbf, p.z)) != 0ull;
     }
     // with pos_only a pair with T = 0 is never a candidate
     __device__ bool zero_skips() const { return pos_only; }
@@ -1558,8 +1657,11 @@ __global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
                     epi.skip_row(el, R, rr, row, valid, col);
                     continue;
                 }
-                const int ma = __builtin_amdgcn_readlane(ma_lane[q], r);
                 const float fi = count_of<NB>(P[q], r);
+                if constexpr (Epi::kAdmit) {
+                    if (!epi.admits(el, rr, fi, mbf, valid)) continue;
+                }
+                const int ma = __builtin_amdgcn_readlane(ma_lane[q], r);
                 const float tv = pair_tanimoto<FAST>(fi, (float)ma, mbf, ma == 0 || mb == 0);
                 epi.apply(el, R, rr, row, valid, col, ma, mb, tv, gw, nw, (int)fi);
             }
@@ -3636,17 +3738,23 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
     int32_t* d_scnt2 = reinterpret_cast<int32_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes);
     // reduce kTopkFan buffers per workgroup until one workgroup per row can
     // take the rest, then select and sort
-    auto select = [&](int32_t* out_idx, float* out_t, uint64_t* kth) -> int {
+    // sparse: the buffers of a seeded pass hold few keys (about k times the
+    // seed stride per row, over all of the row's waves), so a selection
+    // workgroup streams kTopkStreamFan of them instead of kTopkFan
+    auto select = [&](int32_t* out_idx, float* out_t, uint64_t* kth, bool sparse = false) -> int {
         const uint64_t* src = d_buf;
         const int32_t* srccnt = d_cnt;
         int64_t nseg = nw;
         int segcap = cap;
-        for (int level = 0; nseg > kTopkFan; ++level) {
-            const int64_t ns = (nseg + kTopkFan - 1) / kTopkFan;
+        const int fan = sparse ? kTopkStreamFan : kTopkFan;
+        for (int level = 0; nseg > fan; ++level) {
+            // a sparse level has fewer outputs than a full one: the level
+            // buffers (sized for full levels) hold them
+            const int64_t ns = (nseg + fan - 1) / fan;
             uint64_t* dst = level % 2 ? d_sbuf2 : d_sbuf;
             int32_t* dstcnt = level % 2 ? d_scnt2 : d_scnt;
             k_topk_stage<<<dim3((unsigned)nr, (unsigned)ns), dim3(kBlock), topk_lds(segcap), ctx->stream>>>(
-                src, srccnt, (int)nseg, segcap, k, dst, dstcnt);
+                src, srccnt, (int)nseg, segcap, k, fan, dst, dstcnt);
             TRY(launch_check("k_topk_stage"));
             src = dst;
             srccnt = dstcnt;
@@ -3673,7 +3781,16 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
         ColArgs S = ctx->qry->args();
         S.tstride = stride;
         S.ntiles = (ctx->qry->ntiles + stride - 1) / stride;
-        const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid.y, (S.ntiles + 4 * gt - 1) / (4 * gt)));
+        // about kSampleTiles sampled tiles per wave when the launch has row
+        // groups enough to fill the device (each wave's buffer then keeps
+        // its own k largest of several tiles, so the seed's selection reads
+        // a few keys per sampled tile instead of nearly every pair); at least
+        // kSampleWorkgroups workgroups otherwise; at least one tile per wave
+        const int64_t per = 4 * gt * kSampleTiles;
+        const int64_t most = std::min<int64_t>(grid.y, (S.ntiles + 4 * gt - 1) / (4 * gt));
+        const int64_t want = std::max<int64_t>((S.ntiles + per - 1) / per,
+                                               (kSampleWorkgroups + grid.x - 1) / std::max<int64_t>(grid.x, 1));
+        const unsigned gys = (unsigned)std::max<int64_t>(1, std::min<int64_t>(most, want));
         const int64_t nws = (int64_t)gys * (kBlock / kWave) * gt;
         TRY(with_tables(gt, [&](auto gc) {
             return launch_bits(ctx, bp, gys, rowbase, rowlimit,
@@ -3704,7 +3821,8 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
         TRY(launch_check("k_topk"));
     }
     tr.select();
-    TRY(select(mem ? mem->idx : ctx->res.as<int32_t>(), mem ? mem->t : ctx->res2.as<float>(), nullptr));
+    TRY(select(mem ? mem->idx : ctx->res.as<int32_t>(), mem ? mem->t : ctx->res2.as<float>(), nullptr,
+               seed != nullptr));
     tr.done();
     ctx->last_kernel = bp.use ? "topk_bits" : "topk_merge";
     return SIML_OK;

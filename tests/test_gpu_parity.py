@@ -483,6 +483,25 @@ def test_topk_seeded_real_smiles(demo_lines, k, stride):
     assert g.launch_shape()[2] == stride
 
 
+@pytest.mark.parametrize("stride", [16, 4])
+@pytest.mark.parametrize("k", [1, 50])
+def test_topk_seeded_admission_with_ties(demo_lines, k, stride):
+    """Seeded rows skip pairs that cannot reach their seed's T before the
+    division (an I, ma, mb test with a margin).  Eight copies of every query
+    molecule make every T value, the seed's included, shared by at least
+    eight columns: ties at and next to the admission boundary, where a lost
+    pair would change the (T desc, column asc) lists."""
+    g = GPULingo(0)
+    g.set_launch_shape(8192, 1024, stride)
+    ref = oracle.pack(demo_lines[300:420])
+    qry = oracle.pack(demo_lines[:1024] * 8)
+    _set(g, ref, qry)
+    idx, t = g.getTopK(0, 120, k)
+    oi, ot = oracle.topk(ref, qry, k)
+    assert np.array_equal(idx, oi)
+    assert_same_f32(t, ot)
+
+
 def test_valu_peak_microbenchmark():
     """The measured int32 VALU rate of the device (independent add and min
     chains) is positive and not above the vector peak: 256 CUs x 128 lanes
