@@ -725,6 +725,61 @@ def serving_part(args, rates, requests_for):
             "best_pairs_per_s_with_p99_below_50ms": best["pairs_per_s"] if ok else None}
 
 
+def workload_request_split(args):
+    """Where a serving request's time goes, per step of GpuExecutor.run, on
+    the serving database: building and uploading the request's row groups
+    (set_refsmiles_ragged), enqueueing the call, waiting for its kernels,
+    copying the result; kernel time from HIP events.  Medians of 10 calls
+    per request size and operation."""
+    from siml_amd import synth
+    from siml_amd.GPULingo import GPULingo, LingoDB
+
+    ndb = args.db_serve or 10_000_000
+    log(f"request split: packing the {ndb}-molecule database (lognormal, seed 5)")
+    db = LingoDB(*lognormal_set(ndb, 5))
+    pool = lognormal_set(4096, 55)
+    g = GPULingo(0)
+    g.attach_db(db)
+    g.set_group_tables(args.group_tables)
+    h = g._h
+    out = []
+    for n in (1, 4, 16, 32, 64, 128, 512):
+        for op in ("topk", "threshold"):
+            rec = []
+            for it in range(12):
+                ids, counts, offs, mags = synth.ragged_slice(*pool, it * 3, it * 3 + n)
+                a = time.perf_counter()
+                g.set_refsmiles_ragged(ids, counts, offs, mags)
+                b = time.perf_counter()
+                h.call("siml_set_timing", 1)
+                h.kernel_stats(reset=True)
+                if op == "topk":
+                    h.call("siml_topk_async", 0, n, 50)
+                else:
+                    h.call("siml_neighbors_async", 0, n, 0.8, 1.1)
+                c = time.perf_counter()
+                h.call("siml_synchronize")
+                d = time.perf_counter()
+                h.retrieve_topk() if op == "topk" else h.retrieve_csr()
+                e = time.perf_counter()
+                kms, _, _ = h.kernel_stats()
+                h.call("siml_set_timing", 0)
+                rec.append(((b - a) * 1e3, (c - b) * 1e3, (d - c) * 1e3, (e - d) * 1e3, kms, (e - a) * 1e3))
+            m = np.median(np.array(rec[2:]), axis=0)
+            out.append({"rows": n, "op": op, "set_ref_ms": m[0], "enqueue_ms": m[1], "wait_ms": m[2],
+                        "retrieve_ms": m[3], "kernel_ms": m[4], "total_ms": m[5]})
+            log(f"{op:9s} rows {n:4d}: set_ref {m[0]:.3f}  enqueue {m[1]:.3f}  wait {m[2]:.3f}  "
+                f"retrieve {m[3]:.3f}  kernels {m[4]:.3f}  total {m[5]:.3f} ms")
+    top1 = next(r for r in out if r["rows"] == 1 and r["op"] == "topk")
+    rec = base_record(args, "serving request time split (1 GPU)", top1["total_ms"], "ms", top1["total_ms"] / 1e3,
+                      {"workload": f"single requests (top-50, threshold 0.8) on the {ndb}-molecule resident "
+                                   f"database, 1 to 512 query molecules"}, steps=1, warmup=2)
+    rec["higher_is_better"] = False
+    rec["split"] = out
+    db.close()
+    return rec
+
+
 def workload_serve(args):
     rates = [float(r) for r in args.rates.split(",")]
     part = serving_part(args, rates, lambda r: args.requests)
@@ -802,7 +857,8 @@ def main():
     ap.add_argument("--counters-dir", default="",
                     help="only print the `counters:` line of the rocprofv3 passes under this directory")
     ap.add_argument("--counters-kernel", default="k_bits", help="with --counters-dir: kernel name regex")
-    ap.add_argument("--workload", choices=["default", "allpairs", "zinc", "cpu4096", "batched", "threshold", "serve"],
+    ap.add_argument("--workload", choices=["default", "allpairs", "zinc", "cpu4096", "batched", "threshold", "serve",
+                                           "request-split"],
                     default="default")
     ap.add_argument("--quick", action="store_true", help="default workload: the all-pairs line only")
     ap.add_argument("--gpus", type=int, default=1)
@@ -843,7 +899,7 @@ def main():
         return 2
     rec = {"default": workload_default, "allpairs": workload_allpairs, "zinc": workload_zinc,
            "cpu4096": workload_cpu4096, "batched": workload_batched, "threshold": workload_threshold,
-           "serve": workload_serve}[args.workload](args)
+           "serve": workload_serve, "request-split": workload_request_split}[args.workload](args)
     print(json.dumps(rec), flush=True)
     return 0
 
