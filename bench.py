@@ -522,6 +522,8 @@ def batched_part(args, check_oracle=False, peaks=None):
         pi = np.concatenate([p[0] for p in plain])
         pt = np.concatenate([p[1] for p in plain])
         same = bool(np.array_equal(pi, bi) and np.array_equal(pt.view(np.int32), bt.view(np.int32)))
+    graphs = graph_calls(args, g, sets, q, k, pairs, (bi, bt)) if args.graphs else None
+    persistent = persistent_call(g, q, k, pairs, (bi, bt)) if args.graphs else None
     out = {"workload": f"{nsets} query sets (geometric 1/128 sizes clipped [1, 4096], {nq} molecules) x "
                        f"{ndb}-molecule resident database (lognormal lengths, median 45), top-{k} per query "
                        f"molecule", "sets": nsets, "queries": nq, "database": ndb, "k": k, "seed": 3,
@@ -532,7 +534,8 @@ def batched_part(args, check_oracle=False, peaks=None):
                        "similarity_ms": sim_b, "selection_ms": sel_b, "calls": 1},
            "speedup_wall": t_plain / t_batched, "speedup_kernel": kms_plain / max(kms_b, 1e-9),
            "speedup_similarity_kernel": sim_plain / max(sim_b, 1e-9),
-           "plain_equals_batched": same, "kernel": kernel, "autotune": bool(args.autotune),
+           "plain_equals_batched": same, "graphs": graphs, "persistent_grid": persistent, "kernel": kernel,
+           "autotune": bool(args.autotune),
            "batched_launch_shape": {"target_workgroups": shape[0], "topk_wave_cap": shape[1],
                                     "topk_seed": shape[2]}}
     if peaks is not None:
@@ -555,6 +558,77 @@ def batched_part(args, check_oracle=False, peaks=None):
                                         np.array_equal(ot.view(np.int32), bt[rows].view(np.int32)))
     del g
     db.close()
+    return out
+
+
+def graph_calls(args, g, sets, q, k, pairs, eager):
+    """The plain and batched calls again with graph mode on (each call's
+    kernel sequence launched as one HIP graph): times, and whether the
+    batched result equals the eager one."""
+    h = g._h
+    log("batched: the same calls in graph mode")
+    g.set_graphs(True)
+    h.call("siml_set_timing", 0)
+    # warm-up: instantiates the graphs of every kernel sequence the calls use
+    if not args.no_plain:
+        for s in sets:
+            g.set_refsmiles_ragged(*s)
+            g.getTopK(0, len(s[2]) - 1, k)
+    g.set_refsmiles_ragged(*q)
+    g.getTopK(0, len(q[2]) - 1, k)
+    h.call("siml_set_timing", 1)
+    h.kernel_stats(reset=True)
+    out = {}
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        for s in sets:
+            g.set_refsmiles_ragged(*s)
+            g.getTopK(0, len(s[2]) - 1, k)
+        wall = time.perf_counter() - t0
+        kms, _, _ = h.kernel_stats(reset=True)
+        out["plain"] = {"wall_s": wall, "pairs_per_s": pairs / wall, "kernel_ms": kms, "calls": len(sets)}
+    t0 = time.perf_counter()
+    g.set_refsmiles_ragged(*q)
+    gi, gt = g.getTopK(0, len(q[2]) - 1, k)
+    wall = time.perf_counter() - t0
+    kms, _, _ = h.kernel_stats(reset=True)
+    out["batched"] = {"wall_s": wall, "pairs_per_s": pairs / wall, "kernel_ms": kms, "calls": 1}
+    out["batched_equals_eager"] = bool(np.array_equal(gi, eager[0]) and
+                                       np.array_equal(gt.view(np.int32), eager[1].view(np.int32)))
+    out["graph_launches"], out["graph_instantiations"] = g.graph_stats()
+    g.set_graphs(False)
+    log(f"batched, graph mode: plain {out.get('plain', {}).get('wall_s', float('nan')):.2f} s, "
+        f"batched {wall:.2f} s ({kms:.0f} ms kernels), equal {out['batched_equals_eager']}")
+    return out
+
+
+def persistent_call(g, q, k, pairs, eager):
+    """The batched call at the same launch shape with the grid mode
+    switched (persistent grid with per-XCD work queues <-> ordinary grid):
+    time, and whether the result equals the first one."""
+    h = g._h
+    shape = g.launch_shape()
+    out = {}
+    for mode in (True, False):
+        g.set_launch_shape(*shape)
+        g.set_persistent(mode)
+        g.set_refsmiles_ragged(*q)
+        g.getTopK(0, len(q[2]) - 1, k)          # warm-up at this mode
+        h.call("siml_set_timing", 1)
+        h.kernel_stats(reset=True)
+        t0 = time.perf_counter()
+        g.set_refsmiles_ragged(*q)
+        pi, pt = g.getTopK(0, len(q[2]) - 1, k)
+        wall = time.perf_counter() - t0
+        sim, sel = h.kernel_split()
+        kms, _, _ = h.kernel_stats(reset=True)
+        key = "persistent" if mode else "ordinary"
+        out[key] = {"wall_s": wall, "pairs_per_s": pairs / wall, "kernel_ms": kms, "similarity_ms": sim,
+                    "selection_ms": sel,
+                    "equals_first": bool(np.array_equal(pi, eager[0]) and
+                                         np.array_equal(pt.view(np.int32), eager[1].view(np.int32)))}
+        log(f"batched, {key} grid at shape {shape}: {wall:.3f} s ({kms:.0f} ms kernels, {sim:.0f} similarity)")
+    g.set_persistent(False)
     return out
 
 
@@ -665,10 +739,10 @@ def workload_threshold(args):
 # serving loop
 # ---------------------------------------------------------------------------
 def serving_part(args, rates, requests_for):
-    """The serving loop at each arrival rate: p50/p99 latency and pairs/s."""
+    """The serving loop at each arrival rate: p50/p99 latency and pairs/s
+    (and again in graph mode)."""
     from siml_amd import synth
     from siml_amd.GPULingo import LingoDB
-    from siml_amd.serving import Request, Server, run_poisson
 
     ndb = args.db_serve or 10_000_000
     log(f"serving: packing the {ndb}-molecule database (lognormal, seed 5)")
@@ -679,13 +753,35 @@ def serving_part(args, rates, requests_for):
     pool = lognormal_set(int(sizes.sum()), 55)
     bounds = np.zeros(nmax + 1, np.int64)
     np.cumsum(sizes, out=bounds[1:])
+    modes = {}
+    for graphs in ((False, True) if args.graphs else (False,)):
+        modes[graphs] = serve_rates(args, db, pool, sizes, bounds, rates, requests_for, ndb, graphs)
+    results = modes[False]
+    ok = [r for r in results if r["p99_ms"] < 50.0]
+    best = max(ok or results, key=lambda r: r["pairs_per_s"])
+    db.close()
+    return {"workload": f"{ndb}-molecule resident database, Poisson arrivals of query batches (geometric 1/32 "
+                        f"sizes clipped [1, 512], lognormal lengths), half top-50, half threshold 0.8",
+            "database": ndb, "slots": args.slots, "max_rows": args.max_rows, "seed": 5, "rates": results,
+            "graphs": ({"what": "the same arrivals with graph mode on (top-k batches launched as HIP graphs)",
+                        "rates": modes[True]} if True in modes else None),
+            "best_rate_with_p99_below_50ms": best["rate_per_s"] if ok else None,
+            "best_pairs_per_s_with_p99_below_50ms": best["pairs_per_s"] if ok else None}
+
+
+def serve_rates(args, db, pool, sizes, bounds, rates, requests_for, ndb, graphs):
+    """One server over `db` (graph mode or not): p50/p99 per arrival rate."""
+    from siml_amd import synth
+    from siml_amd.serving import Request, Server, run_poisson
+
     results = []
+    tag = "serving (graph mode)" if graphs else "serving"
     with Server(db, slots=args.slots, max_rows=args.max_rows, mem_budget=args.mem_budget << 30,
-                autotune=args.autotune, group_tables=args.group_tables) as srv:
+                autotune=args.autotune, group_tables=args.group_tables, graphs=graphs) as srv:
         # warm-up: one request of each kind per power-of-two size up to the
         # batch limit (with the autotuner this also fixes every shape bucket
         # before the timed arrivals)
-        log("serving: warm-up requests" + (" (tuning launch shapes)" if args.autotune else ""))
+        log(f"{tag}: warm-up requests" + (" (tuning launch shapes)" if args.autotune else ""))
         n = 1
         while n <= args.max_rows:
             for op, prm in (("topk", 50), ("threshold", 0.8)):
@@ -713,16 +809,9 @@ def serving_part(args, rates, requests_for):
                             "p99_ms": float(np.percentile(lat, 99) * 1e3),
                             "max_ms": float(lat.max() * 1e3), "pairs_per_s": pairs / wall,
                             "requests_per_s": nreq / wall})
-            log(f"serving: rate {rate}/s: p50 {results[-1]['p50_ms']:.1f} ms, p99 {results[-1]['p99_ms']:.1f} ms, "
+            log(f"{tag}: rate {rate}/s: p50 {results[-1]['p50_ms']:.2f} ms, p99 {results[-1]['p99_ms']:.2f} ms, "
                 f"{results[-1]['pairs_per_s']:.3g} pairs/s")
-    ok = [r for r in results if r["p99_ms"] < 50.0]
-    best = max(ok or results, key=lambda r: r["pairs_per_s"])
-    db.close()
-    return {"workload": f"{ndb}-molecule resident database, Poisson arrivals of query batches (geometric 1/32 "
-                        f"sizes clipped [1, 512], lognormal lengths), half top-50, half threshold 0.8",
-            "database": ndb, "slots": args.slots, "max_rows": args.max_rows, "seed": 5, "rates": results,
-            "best_rate_with_p99_below_50ms": best["rate_per_s"] if ok else None,
-            "best_pairs_per_s_with_p99_below_50ms": best["pairs_per_s"] if ok else None}
+    return results
 
 
 def workload_request_split(args):
@@ -743,9 +832,11 @@ def workload_request_split(args):
     g.set_group_tables(args.group_tables)
     h = g._h
     out = []
+    ops = ("topk", "topk_graph", "threshold") if args.graphs else ("topk", "threshold")
     for n in (1, 4, 16, 32, 64, 128, 512):
-        for op in ("topk", "threshold"):
+        for op in ops:
             rec = []
+            g.set_graphs(op == "topk_graph")
             for it in range(12):
                 ids, counts, offs, mags = synth.ragged_slice(*pool, it * 3, it * 3 + n)
                 a = time.perf_counter()
@@ -753,14 +844,14 @@ def workload_request_split(args):
                 b = time.perf_counter()
                 h.call("siml_set_timing", 1)
                 h.kernel_stats(reset=True)
-                if op == "topk":
+                if op.startswith("topk"):
                     h.call("siml_topk_async", 0, n, 50)
                 else:
                     h.call("siml_neighbors_async", 0, n, 0.8, 1.1)
                 c = time.perf_counter()
                 h.call("siml_synchronize")
                 d = time.perf_counter()
-                h.retrieve_topk() if op == "topk" else h.retrieve_csr()
+                h.retrieve_topk() if op.startswith("topk") else h.retrieve_csr()
                 e = time.perf_counter()
                 kms, _, _ = h.kernel_stats()
                 h.call("siml_set_timing", 0)
@@ -874,6 +965,8 @@ def main():
     ap.add_argument("--max-rows", type=int, default=1024, help="serve: query molecules per launch")
     ap.add_argument("--mem-budget", type=int, default=16, help="serve: admission budget, GiB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="batched/serve: also run the calls in graph mode (siml_set_graphs) and report both")
     ap.add_argument("--autotune", type=int, default=1,
                     help="batched/serve: launch shapes chosen by the autotuner (1) or the defaults (0)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)

@@ -267,6 +267,28 @@ const char* siml_last_kernel(siml_ctx* ctx);
  * Default (and after siml_set_autotune(ctx, 0)): 8192 workgroups, 1024
  * waves per top-k row, seed stride 1 (from the size). */
 int siml_set_autotune(siml_ctx* ctx, int on);
+/* Graph mode (replaces the reference's per-call prepared_async_call
+ * launches, GPULingo.py:391-397,501-509): when on, the kernel sequence of
+ * each siml_topk_async / siml_topk_noalloc / siml_hist1000_async call
+ * (similarity passes, selection stages, bin clearing) is recorded by stream
+ * capture and launched as one HIP graph; host uploads of the call's row
+ * lists stay outside the graph.  Instantiated graphs are cached per handle
+ * by their kernel sequence and updated in place with the next call's
+ * arguments.  When the handle's stream is already being captured by the
+ * caller, the launches go to the caller's graph instead.  Results are
+ * identical; off by default.  siml_graph_stats reports the graph launches
+ * and the instantiations (cache misses) since the handle was created. */
+int siml_set_graphs(siml_ctx* ctx, int on);
+/* Persistent grid (no counterpart in the reference, whose kernels launch
+ * one block per row, kernels.py:315-352): when on, the bit-sliced similarity
+ * pass of siml_topk_async / siml_topk_noalloc / siml_hist1000_async runs as
+ * many workgroups as the device holds at once, which take (row-group slot)
+ * work items from one queue per XCD and run each slot's column ranges of
+ * that XCD with the slot's tables loaded once.  Results are identical.  Turns
+ * the autotuner off (whose candidates include persistent shapes); off by
+ * default. */
+int siml_set_persistent(siml_ctx* ctx, int on);
+int siml_graph_stats(siml_ctx* ctx, int64_t* launches, int64_t* builds);
 int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap, int32_t* topk_seed);
 int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed);
 /* Bit-sliced row groups of the current reference set (no counterpart in

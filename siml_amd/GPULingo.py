@@ -135,6 +135,25 @@ class GPULingo:
         shape bucket (results are unchanged; see siml_set_autotune)."""
         self._h.call("siml_set_autotune", 1 if on else 0)
 
+    def set_graphs(self, on=True):
+        """Top-k and global-histogram calls recorded by stream capture and
+        launched as one HIP graph each (results are unchanged; see
+        siml_set_graphs)."""
+        self._h.call("siml_set_graphs", 1 if on else 0)
+
+    def set_persistent(self, on=True):
+        """Top-k and global-histogram similarity passes on a persistent grid
+        with per-XCD work queues (results are unchanged; turns the autotuner
+        off; see siml_set_persistent)."""
+        self._h.call("siml_set_persistent", 1 if on else 0)
+
+    def graph_stats(self):
+        """(graph launches, graph instantiations) of this comparator."""
+        import ctypes
+        n, b = ctypes.c_int64(), ctypes.c_int64()
+        self._h.call("siml_graph_stats", ctypes.byref(n), ctypes.byref(b))
+        return n.value, b.value
+
     def launch_shape(self):
         """(workgroups aimed at per launch, most waves per top-k row,
         top-k seed stride: 0 = no seeding) of the last call."""

@@ -85,6 +85,9 @@ SIGNATURES = {
     "siml_work_stats": (_int, [_vp, _i64p, _i64p, _int]),
     "siml_selftest_division": (_int, [_int, _int, _i64p, _i64p, _i64p]),
     "siml_set_autotune": (_int, [_vp, _int]),
+    "siml_set_graphs": (_int, [_vp, _int]),
+    "siml_set_persistent": (_int, [_vp, _int]),
+    "siml_graph_stats": (_int, [_vp, _i64p, _i64p]),
     "siml_selftest_valu": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
     "siml_selftest_lds": (_int, [_int, _int, ctypes.POINTER(ctypes.c_double)]),
     "siml_row_groups": (_int, [_vp, _i64p, _i64p]),

@@ -811,6 +811,11 @@ struct BitGroupArgs {
     const int32_t* rows;   // kBitRowStride row indices per group
     const uint4* ent;      // (key, mask, slot, -) per entry
     const int32_t* glist;  // the groups of this launch
+    // persistent grid (k_bits only; null: one work item per workgroup):
+    // kXcds counters, zero at launch, each handing out the nslot group
+    // slots once; ny column ranges per group slot
+    int32_t* queue = nullptr;
+    int32_t nslot = 0, ny = 0;
 };
 
 // Row mask of the slot holding `key` (0 when neither candidate does); sw
@@ -1365,12 +1370,13 @@ __device__ __forceinline__ void bits_wg(int& gslot, int& y, int& ny) {
 // The workgroup's GT groups are glist[gslot * GT + q], q < GT; a negative
 // entry (padding of the last slot) is an empty group: no live rows, an empty
 // table that matches no level.
-template <int GT, class Epi>
-__device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
-                                              int64_t rl, const Epi& epi, uint2 (*tab)[siml::kBitSlots],
-                                              BitRows<32 * GT>& ri, typename Epi::Lds& el) {
-    int gslot, y, ny;
-    bits_wg(gslot, y, ny);
+// Loads the rows and the tables of group slot `gslot` (GT groups) into
+// LDS; ends with a workgroup barrier.  The caller has passed a barrier since
+// the previous slot's tables were last read.
+template <int GT, bool PRUNE>
+__device__ __forceinline__ void bits_load_slot(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
+                                               int64_t rl, int gslot, uint2 (*tab)[siml::kBitSlots],
+                                               BitRows<32 * GT>& ri) {
     for (int s = threadIdx.x; s < GT * siml::kBitSlots; s += blockDim.x) (&tab[0][0])[s] = make_uint2(0u, 0u);
     if (threadIdx.x < 32 * GT) {
         const int q = threadIdx.x / 32, r = threadIdx.x % 32;
@@ -1383,9 +1389,8 @@ __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32
         ri.ma[threadIdx.x] = ma;
         ri.maf[threadIdx.x] = (float)ma;
     }
-    epi.init(el);
     __syncthreads();
-    if (Epi::kPrune && threadIdx.x == 0) {
+    if (PRUNE && threadIdx.x == 0) {
         int lo = INT32_MAX, hi = INT32_MIN;
         for (int r = 0; r < 32 * GT; ++r)
             if (ri.row[r] >= 0) {
@@ -1406,6 +1411,18 @@ __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32
         }
     }
     __syncthreads();
+}
+
+// Prologue of a workgroup of an ordinary grid: its group slot's rows and
+// tables, and the epilogue's LDS.
+template <int GT, class Epi>
+__device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32_t* __restrict__ rmag, int64_t rb,
+                                              int64_t rl, const Epi& epi, uint2 (*tab)[siml::kBitSlots],
+                                              BitRows<32 * GT>& ri, typename Epi::Lds& el) {
+    int gslot, y, ny;
+    bits_wg(gslot, y, ny);
+    epi.init(el);
+    bits_load_slot<GT, Epi::kPrune>(G, rmag, rb, rl, gslot, tab, ri);
 }
 
 // Streams the Lt (a multiple of 8) levels of one query molecule against the
@@ -1561,27 +1578,20 @@ __global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_
     }
 }
 
-// Every other epilogue: each wave walks a contiguous range of tiles and
-// probes the workgroup's Epi::kG tables (32 rows each) with every level it
-// loads.
-// Workgroups serving several tables take the LDS of 2 or 4 single-table
-// ones, so they are held to 128 VGPRs: 4 waves per SIMD (the second bound
-// is the minimum number of waves per SIMD).
+// One work item of a bit-sliced workgroup: column range y of ny against the
+// rows and tables in LDS (loaded by bits_load_slot).  Initialises the
+// epilogue's LDS, and ends with a barrier after the epilogue's finish.
 template <int NB, bool FAST, class Epi>
-__global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
-    BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C, int64_t rb, int64_t rl, Epi epi) {
+__device__ __forceinline__ void bits_item(const ColArgs& C, const Epi& epi, const uint2 (*tab)[siml::kBitSlots],
+                                          const BitRows<32 * Epi::kG>& ri, typename Epi::Lds& el, int y, int ny) {
     constexpr int GT = Epi::kG;
-    __shared__ uint2 tab[GT][siml::kBitSlots];
-    __shared__ BitRows<32 * GT> ri;
-    __shared__ typename Epi::Lds el;
-    bits_prologue<GT>(G, rmag, rb, rl, epi, tab, ri, el);
+    epi.init(el);
+    __syncthreads();
     const int lane = lane_id();
     const int4* bt4 = reinterpret_cast<const int4*>(C.btiles);
     typename Epi::Reg R;
     int64_t t0, t1, gw, nw;
     {
-        int gslot, y, ny;
-        bits_wg(gslot, y, ny);
         const int wpb = blockDim.x / kWave;
         // wave-uniform: the tile loop, its offsets and lengths then live in
         // scalar registers and are read through the scalar cache
@@ -1670,7 +1680,55 @@ __global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
     epi.end(el, R, ri, gw, nw);
     __syncthreads();
     epi.finish(el, ri);
+    __syncthreads();
 }
+
+// Every other epilogue: each wave walks a contiguous range of tiles and
+// probes the workgroup's Epi::kG tables (32 rows each) with every level it
+// loads.  An ordinary grid has one work item per workgroup: (group slot,
+// column range) from the block index (bits_wg).  A persistent grid (G.queue
+// set, about as many workgroups as are resident at once) takes group slots
+// from kXcds queues instead: queue x hands out each slot once, and the
+// workgroup that takes it loads the slot's tables once and runs its column
+// ranges x, x + kXcds, ... (the ranges bits_wg gives XCD x).  A workgroup
+// starts at queue (its block index mod kXcds), the XCD the hardware
+// dispatcher places it on, and when that queue is empty goes on to the
+// next; it exits when all kXcds are empty, so every workgroup finishes
+// after at most nslot + 1 takes per queue.
+// Workgroups serving several tables take the LDS of 2 or 4 single-table
+// ones, so they are held to 128 VGPRs: 4 waves per SIMD (the second bound
+// is the minimum number of waves per SIMD).
+template <int NB, bool FAST, class Epi>
+__global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
+    BitGroupArgs G, const int32_t* __restrict__ rmag, ColArgs C, int64_t rb, int64_t rl, Epi epi) {
+    constexpr int GT = Epi::kG;
+    __shared__ uint2 tab[GT][siml::kBitSlots];
+    __shared__ BitRows<32 * GT> ri;
+    __shared__ typename Epi::Lds el;
+    __shared__ int taken;
+    if (!G.queue) {
+        int gslot, y, ny;
+        bits_wg(gslot, y, ny);
+        bits_load_slot<GT, Epi::kPrune>(G, rmag, rb, rl, gslot, tab, ri);
+        bits_item<NB, FAST, Epi>(C, epi, tab, ri, el, y, ny);
+        return;
+    }
+    const int home = (int)(blockIdx.x % kXcds);
+    for (int tried = 0; tried < kXcds;) {
+        const int x = (home + tried) % kXcds;
+        if (threadIdx.x == 0) taken = atomicAdd(G.queue + x, 1);
+        __syncthreads();
+        const int slot = taken;
+        __syncthreads();
+        if (slot >= G.nslot || x >= G.ny) {
+            ++tried;
+            continue;
+        }
+        bits_load_slot<GT, Epi::kPrune>(G, rmag, rb, rl, slot, tab, ri);
+        for (int y = x; y < G.ny; y += kXcds) bits_item<NB, FAST, Epi>(C, epi, tab, ri, el, y, G.ny);
+    }
+}
+
 
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
 // per-row 101-bin histogram; per-workgroup LDS bins, flushed with global
@@ -2197,7 +2255,24 @@ struct siml_ctx {
     int topk_seed = 1;
     // most row groups (tables) one bit-sliced workgroup serves: 1, 2 or 4
     int max_tables = 2;
+    // top-k and 1000-bin histogram launches on a persistent grid of the
+    // bit-sliced kernel (work queue per XCD) instead of an ordinary grid
+    bool persistent = false;
+    int cus = 256;  // compute units of the device
     bool autotune = false;
+    // graph mode (siml_set_graphs): the kernel sequence of a top-k or
+    // 1000-bin histogram call is recorded by stream capture and launched as
+    // one HIP graph.  Instantiated graphs are kept by the sequence's
+    // signature (node count and kernels in order) and updated in place with
+    // the next call's arguments; `done` marks the end of an instance's last
+    // launch, which must be complete before its arguments are updated.
+    bool graphs = false;
+    struct GraphExec {
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::map<uint64_t, GraphExec> graph_exec;
+    int64_t graph_launches = 0, graph_builds = 0;
     DevBuf rowlist, csr_dev, glist;
     std::vector<int32_t> h_glist;
     std::vector<int32_t> h_rowlist;  // host source of the last rowlist upload
@@ -2258,6 +2333,98 @@ struct TimedRegion {
         if (b) ctx->ev_pool.push_back(b);
     }
 };
+
+// Records the kernel launches issued between begin() and launch() on the
+// handle's stream as a HIP graph and launches it, when graph mode is on and
+// the stream is not already being captured by the caller (then the launches
+// go to the caller's graph).  Only kernels may be issued in between: host
+// copies and memsets stay outside, before begin().  A region left without
+// launch() (an error return) ends the capture and drops the graph.
+struct GraphRegion {
+    siml_ctx* ctx;
+    bool on = false;
+    explicit GraphRegion(siml_ctx* c) : ctx(c) {}
+    int begin() {
+        if (!ctx->graphs) return SIML_OK;
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(ctx->stream, &st));
+        if (st != hipStreamCaptureStatusNone) return SIML_OK;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        on = true;
+        return SIML_OK;
+    }
+    int launch();
+    ~GraphRegion() {
+        if (!on) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+    }
+};
+
+// Signature of a captured graph: its node count, node types and kernel
+// functions in node order.  Graphs of equal signature differ only in
+// arguments and launch dimensions, which hipGraphExecUpdate changes.
+int graph_signature(hipGraph_t g, uint64_t* sig) {
+    size_t n = 0;
+    HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
+    uint64_t h = 0xcbf29ce484222325ull ^ (uint64_t)n;
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType t = hipGraphNodeTypeEmpty;
+        HIP_TRY(hipGraphNodeGetType(nd, &t));
+        uint64_t v = (uint64_t)t;
+        if (t == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams p{};
+            HIP_TRY(hipGraphKernelNodeGetParams(nd, &p));
+            v ^= (uint64_t)(uintptr_t)p.func << 8;
+        }
+        h = (h ^ v) * 0x100000001b3ull;
+    }
+    *sig = h;
+    return SIML_OK;
+}
+
+int GraphRegion::launch() {
+    if (!on) return SIML_OK;
+    on = false;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
+    if (!g) return fail(SIML_EDEVICE, "stream capture produced no graph");
+    uint64_t sig = 0;
+    int rc = graph_signature(g, &sig);
+    siml_ctx::GraphExec* ge = rc == SIML_OK ? &ctx->graph_exec[sig] : nullptr;
+    if (ge && ge->exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+        hipError_t e = ge->done ? hipEventSynchronize(ge->done) : hipSuccess;
+        if (e != hipSuccess) {
+            rc = fail(SIML_EDEVICE, "waiting for the previous graph launch failed: %s", hipGetErrorString(e));
+        } else if (hipGraphExecUpdate(ge->exec, g, &bad, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(ge->exec);
+            ge->exec = nullptr;
+        }
+    }
+    if (rc == SIML_OK && !ge->exec) {
+        hipError_t e = hipGraphInstantiate(&ge->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            ge->exec = nullptr;
+            rc = fail(SIML_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        } else {
+            ++ctx->graph_builds;
+        }
+    }
+    (void)hipGraphDestroy(g);
+    if (rc != SIML_OK) return rc;
+    HIP_TRY(hipGraphLaunch(ge->exec, ctx->stream));
+    if (!ge->done) HIP_TRY(hipEventCreateWithFlags(&ge->done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ge->done, ctx->stream));
+    ++ctx->graph_launches;
+    return SIML_OK;
+}
 
 int collect_timing(siml_ctx* ctx) {
     if (ctx->pending_ev.empty()) return SIML_OK;
@@ -2702,9 +2869,14 @@ constexpr int kNbrCap = 16;
 
 // Launches k_bits<NB, Epi> over the plan's groups, Epi::kG groups per
 // workgroup, with grid.y = gy column ranges.
+// With `queue` (kXcds int32 of device memory), the k_bits launch runs on a
+// persistent grid: as many workgroups as are resident at once (at most
+// kXcds per group slot), taking the grid.x group slots from per-XCD queues,
+// gy (a multiple of kXcds) column ranges each; the counters are cleared by
+// a kernel on the stream first.
 template <class Epi>
 int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_t rl, const Epi& epi,
-                const char* name, const ColArgs* cols = nullptr) {
+                const char* name, const ColArgs* cols = nullptr, int32_t* queue = nullptr) {
     if (p.groups.empty()) return SIML_OK;
     constexpr int GT = Epi::kG;
     if (GT > 1 && p.nb != 8) return fail(SIML_EINVAL, "%s: %d tables per workgroup need 8 bit planes", name, GT);
@@ -2726,15 +2898,35 @@ int launch_bits(siml_ctx* ctx, const BitPlan& p, unsigned gy, int64_t rb, int64_
             k_bits_block<16, false><<<grid, b16, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
     } else {
         const dim3 block(kBlock * GT);
+        if (queue && gy % kXcds != 0)
+            return fail(SIML_EINVAL, "%s: a persistent grid needs a multiple of %d column ranges", name, kXcds);
+        auto go = [&](auto kern) -> int {
+            dim3 g = grid;
+            BitGroupArgs GA = G;
+            if (queue) {
+                int per_cu = 0;
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block.x, 0));
+                const int64_t resident =
+                    std::max<int64_t>((int64_t)std::max(per_cu, 1) * ctx->cus / kXcds * kXcds, kXcds);
+                g = dim3((unsigned)std::min<int64_t>(resident, (int64_t)kXcds * grid.x), 1, 1);
+                GA.queue = queue;
+                GA.nslot = (int32_t)grid.x;
+                GA.ny = (int32_t)gy;
+                k_fill<int32_t><<<dim3(1), dim3(kWave), 0, ctx->stream>>>(queue, kXcds, 0);
+                TRY(launch_check("k_fill<queue>"));
+            }
+            kern<<<g, block, 0, ctx->stream>>>(GA, rmag, C, rb, rl, epi);
+            return SIML_OK;
+        };
         if (p.nb == 8 && fast)
-            k_bits<8, true, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            TRY(go(k_bits<8, true, Epi>));
         else if (p.nb == 8)
-            k_bits<8, false, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+            TRY(go(k_bits<8, false, Epi>));
         else if constexpr (GT == 1) {
             if (fast)
-                k_bits<16, true, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+                TRY(go(k_bits<16, true, Epi>));
             else
-                k_bits<16, false, Epi><<<grid, block, 0, ctx->stream>>>(G, rmag, C, rb, rl, epi);
+                TRY(go(k_bits<16, false, Epi>));
         }
     }
     return launch_check(name);
@@ -3013,6 +3205,10 @@ int siml_create(int device, siml_ctx** out) {
     HIP_TRY(hipSetDevice(device));
     siml_ctx* ctx = new siml_ctx();
     ctx->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->cus = cus;
+    (void)hipGetLastError();
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;
@@ -3043,6 +3239,10 @@ int siml_destroy(siml_ctx* ctx) {
         (void)hipEventDestroy(t.b);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto& kv : ctx->graph_exec) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SIML_OK;
@@ -3313,14 +3513,18 @@ namespace {
 // candidate buffers); otherwise every row is on the merge kernel.
 struct TopkShape {
     int gt = 1, cap = 0;
+    bool persist = false;  // the full pass runs on a persistent grid
     dim3 grid;
     int64_t nw = 0;
-    size_t buf = 0, cnt = 0, s1 = 0, s2 = 0, c1 = 0, c2 = 0, seed = 0;
-    size_t scratch() const { return buf + cnt + s1 + s2 + c1 + c2 + seed; }
+    size_t buf = 0, cnt = 0, s1 = 0, s2 = 0, c1 = 0, c2 = 0, seed = 0, queue = 0;
+    size_t scratch() const { return buf + cnt + s1 + s2 + c1 + c2 + seed + queue; }
 };
 
+// `persistent`: the bit-sliced full pass on a persistent grid, which needs a
+// multiple of kXcds column ranges (at least kXcds; an ordinary grid when the
+// wave cap or the buffer budget allows fewer).
 TopkShape topk_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, int gt, int32_t k, int64_t target_wg,
-                     unsigned wave_cap, int64_t budget = kTopkBufferBudget) {
+                     unsigned wave_cap, int64_t budget = kTopkBufferBudget, bool persistent = false) {
     TopkShape t;
     t.gt = bits ? gt : 1;
     t.cap = (int)(((k + kTopkSlack) + kWave - 1) / kWave * kWave);
@@ -3334,6 +3538,11 @@ TopkShape topk_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups,
         1, std::min<int64_t>({(int64_t)(wave_cap / (unsigned)t.gt), budget / range_bytes, 65535}));
     t.grid.y = std::min<unsigned>(t.grid.y, ycap);
     if (bits) t.grid.y = xcd_gy(t.grid.y, ycap);
+    if (bits && persistent && ycap >= (unsigned)kXcds) {
+        t.persist = true;
+        t.grid.y = std::min<unsigned>(std::max<unsigned>((t.grid.y + kXcds - 1) / kXcds * kXcds, kXcds),
+                                      ycap / kXcds * kXcds);
+    }
     t.nw = (int64_t)t.grid.y * (kBlock / kWave) * t.gt;
     const int64_t nslots = nr * t.nw;
     auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -3348,7 +3557,8 @@ TopkShape topk_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups,
     t.s2 = r256(sizeof(uint64_t) * (size_t)ns2 * (size_t)k);
     t.c1 = r256(sizeof(int32_t) * (size_t)ns1);
     t.c2 = r256(sizeof(int32_t) * (size_t)ns2);
-    t.seed = sizeof(uint64_t) * (size_t)nr;
+    t.seed = r256(sizeof(uint64_t) * (size_t)nr);
+    t.queue = t.persist ? 256 : 0;
     return t;
 }
 
@@ -3705,7 +3915,7 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
                   "top-k buffer exceeds wave registers");
     const int gt_plan = bp.use ? bits_tables(ctx, bp) : 1;
     TopkShape shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), gt_plan, k, ctx->target_wg,
-                               ctx->topk_wave_cap);
+                               ctx->topk_wave_cap, kTopkBufferBudget, ctx->persistent);
     if (mem) {
         const size_t room = mem->cap - used;
         for (int64_t budget = (int64_t)room; shp.scratch() > room; budget /= 2) {
@@ -3713,7 +3923,7 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
                 return fail(SIML_ELIMIT, "getTopK: %lld rows with k=%d need %zu bytes of scratch at the narrowest "
                             "launch, more than the %zu available", (long long)nr, k, used + shp.scratch(), mem->cap);
             shp = topk_shape(ctx, nr, bp.use, (int64_t)bp.groups.size(), gt_plan, k, ctx->target_wg,
-                             ctx->topk_wave_cap, budget);
+                             ctx->topk_wave_cap, budget, ctx->persistent);
         }
     }
     const int cap = shp.cap, gt = shp.gt;
@@ -3730,6 +3940,7 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
     }
     uint64_t* d_seed = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes + s1_bytes + s2_bytes + c1_bytes +
                                                    c2_bytes);
+    int32_t* d_queue = shp.persist ? reinterpret_cast<int32_t*>(reinterpret_cast<char*>(d_seed) + shp.seed) : nullptr;
     uint64_t* d_buf = reinterpret_cast<uint64_t*>(base);
     int32_t* d_cnt = reinterpret_cast<int32_t*>(base + buf_bytes);
     uint64_t* d_sbuf = reinterpret_cast<uint64_t*>(base + buf_bytes + cnt_bytes);
@@ -3770,7 +3981,9 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
                                                                          pos_only ? 1 : 0, nq, out_idx, out_t);
         return launch_check("k_topk_final");
     };
-    TimedRegion tr(ctx, nr * nq, mem == nullptr);
+    GraphRegion gr(ctx);
+    TRY(gr.begin());
+    TimedRegion tr(ctx, nr * nq, mem == nullptr && !gr.on);
     const uint64_t* seed = nullptr;
     const int64_t stride = seed_stride(ctx->topk_seed, ctx->qry->ntiles);
     if (bp.use && pos_only && stride > 1 && ctx->qry->ntiles >= 4 * stride) {
@@ -3808,7 +4021,7 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
         TRY(with_tables(gt, [&](auto gc) {
             return launch_bits(ctx, bp, grid.y, rowbase, rowlimit,
                                EpiTopk<decltype(gc)::value>{k, cap, d_buf, d_cnt, rowbase, pos_only, seed},
-                               "k_bits<topk>");
+                               "k_bits<topk>", nullptr, d_queue);
         }));
         if (!mrows.empty()) {
             k_topk<<<dim3((unsigned)mrows.size(), grid.y * gt), dim3(kBlock), lds, ctx->stream>>>(
@@ -3824,6 +4037,11 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
     TRY(select(mem ? mem->idx : ctx->res.as<int32_t>(), mem ? mem->t : ctx->res2.as<float>(), nullptr,
                seed != nullptr));
     tr.done();
+    if (gr.on) {
+        TimedRegion tg(ctx, nr * nq, mem == nullptr);
+        TRY(gr.launch());
+        tg.done();
+    }
     ctx->last_kernel = bp.use ? "topk_bits" : "topk_merge";
     return SIML_OK;
 }
@@ -3842,13 +4060,20 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     ctx->rows = 1;
     ctx->cols = 1000;
     unsigned long long* hist = ctx->res.as<unsigned long long>();
-    k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
-    TRY(launch_check("k_fill"));
-    if (nr == 0 || nq == 0) return SIML_OK;
+    if (nr == 0 || nq == 0) {
+        k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
+        return launch_check("k_fill");
+    }
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
+    if (bp.use && ctx->persistent) TRY(ctx->scratch.reserve(256));  // persistent grid's queue counters
+    // the bins are cleared by a kernel, which a graph may hold
+    GraphRegion gr(ctx);
+    TRY(gr.begin());
+    k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
+    TRY(launch_check("k_fill"));
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     const int gt = bp.use ? bits_tables(ctx, bp) : 1;
     dim3 grid = row_grid(bp.use ? std::max<int64_t>(((int64_t)bp.groups.size() + gt - 1) / gt, 1) : nr,
@@ -3857,11 +4082,17 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     const int64_t min_gy = (siml::kBitRows * gt * kWave * ctx->qry->ntiles) / (int64_t(1) << 31) + 1;
     grid.y = (unsigned)std::min<int64_t>(65535, std::max<int64_t>(grid.y, min_gy));
     if (bp.use) grid.y = std::max<unsigned>(xcd_gy(grid.y, 65535), (unsigned)std::min<int64_t>(min_gy, 65535));
-    TimedRegion tr(ctx, nr * nq);
+    int32_t* d_queue = nullptr;
+    if (bp.use && ctx->persistent) {
+        grid.y = std::min<unsigned>(std::max<unsigned>((grid.y + kXcds - 1) / kXcds * kXcds, kXcds),
+                                    65535 / kXcds * kXcds);
+        d_queue = ctx->scratch.as<int32_t>();
+    }
+    TimedRegion tr(ctx, nr * nq, !gr.on);
     if (bp.use) {
         TRY(with_tables(gt, [&](auto gc) {
             return launch_bits(ctx, bp, grid.y, rowbase, rowlimit, EpiHist1000<decltype(gc)::value>{hist},
-                               "k_bits<hist1000>");
+                               "k_bits<hist1000>", nullptr, d_queue);
         }));
         if (!mrows.empty()) {
             k_hist1000<<<dim3((unsigned)mrows.size(), grid.y), dim3(kBlock), lds, ctx->stream>>>(
@@ -3873,6 +4104,11 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
         TRY(launch_check("k_hist1000"));
     }
     tr.done();
+    if (gr.on) {
+        TimedRegion tg(ctx, nr * nq);
+        TRY(gr.launch());
+        tg.done();
+    }
     ctx->last_kernel = bp.use ? "hist1000_bits" : "hist1000_merge";
     return SIML_OK;
 }
@@ -3895,18 +4131,26 @@ struct TuneShape {
     int64_t target_wg;
     unsigned topk_wave_cap;
     int topk_seed;  // sample stride, 0: no seeding, 1: from the query set's size
+    bool persistent = false;
 };
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, int>, TuneShape> g_tune;
 
-// Candidate launch shapes: the first 6 apply to every operation, the rest
-// to top-k only.
-const TuneShape kCands[] = {{8192, 256, 0},     {2048, 256, 0},    {4096, 256, 0},   {16384, 256, 0},
-                            {32768, 1024, 0},   {65536, 1024, 0},  {8192, 1024, 1},  {2048, 1024, 1},
-                            {8192, 256, 16},    {2048, 256, 16},   {32768, 1024, 16}, {8192, 256, 4},
-                            {8192, 64, 0}};
-int tune_candidates(int op) { return op == kTuneTopk ? 13 : 6; }
+// Candidate launch shapes: the first 6 apply to every operation, the next
+// 7 to top-k only, then 2 persistent-grid shapes for top-k and 2 for the
+// 1000-bin histogram.
+const TuneShape kCands[] = {{8192, 256, 0},     {2048, 256, 0},      {4096, 256, 0},      {16384, 256, 0},
+                            {32768, 1024, 0},   {65536, 1024, 0},    {8192, 1024, 1},     {2048, 1024, 1},
+                            {8192, 256, 16},    {2048, 256, 16},     {32768, 1024, 16},   {8192, 256, 4},
+                            {8192, 64, 0},      {8192, 1024, 1, true}, {32768, 1024, 1, true},
+                            {8192, 256, 0, true}, {32768, 256, 0, true}};
+constexpr int kNumCands = (int)(sizeof(kCands) / sizeof(kCands[0]));
+bool tune_candidate(int op, int i) {
+    if (i < 6) return true;
+    if (i < 15) return op == kTuneTopk;
+    return op == kTuneHist1000;
+}
 
 int log2_bucket(int64_t x) {
     int b = 0;
@@ -3928,10 +4172,10 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
             ctx->target_wg = it->second.target_wg;
             ctx->topk_wave_cap = it->second.topk_wave_cap;
             ctx->topk_seed = it->second.topk_seed;
+            ctx->persistent = it->second.persistent;
             return impl();
         }
     }
-    const int ncand = tune_candidates(op);
     TRY(set_device(ctx));
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -3942,10 +4186,13 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
     TuneShape best = kCands[0];
     float best_ms = 0.f;
     int rc = SIML_OK;
-    for (int i = 0; i < ncand && rc == SIML_OK; ++i) {
+    bool first = true;
+    for (int i = 0; i < kNumCands && rc == SIML_OK; ++i) {
+        if (!tune_candidate(op, i)) continue;
         ctx->target_wg = kCands[i].target_wg;
         ctx->topk_wave_cap = kCands[i].topk_wave_cap;
         ctx->topk_seed = kCands[i].topk_seed;
+        ctx->persistent = kCands[i].persistent;
         (void)hipEventRecord(a, ctx->stream);
         rc = impl();
         if (rc != SIML_OK) break;
@@ -3955,9 +4202,10 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
             rc = fail(SIML_EDEVICE, "timing a candidate launch shape failed");
             break;
         }
-        if (i == 0 || ms < best_ms) {
+        if (first || ms < best_ms) {
             best_ms = ms;
             best = kCands[i];
+            first = false;
         }
     }
     (void)hipEventDestroy(a);
@@ -3970,6 +4218,7 @@ int tuned_call(siml_ctx* ctx, int op, int64_t rows, int k, F&& impl) {
     ctx->target_wg = best.target_wg;
     ctx->topk_wave_cap = best.topk_wave_cap;
     ctx->topk_seed = best.topk_seed;
+    ctx->persistent = best.persistent;
     return SIML_OK;
 }
 
@@ -4020,7 +4269,28 @@ int siml_set_autotune(siml_ctx* ctx, int on) {
         ctx->target_wg = 8192;
         ctx->topk_wave_cap = 1024;
         ctx->topk_seed = 1;
+        ctx->persistent = false;
     }
+    return SIML_OK;
+}
+
+int siml_set_persistent(siml_ctx* ctx, int on) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    ctx->autotune = false;
+    ctx->persistent = on != 0;
+    return SIML_OK;
+}
+
+int siml_set_graphs(siml_ctx* ctx, int on) {
+    if (!ctx) return fail(SIML_EINVAL, "null handle");
+    ctx->graphs = on != 0;
+    return SIML_OK;
+}
+
+int siml_graph_stats(siml_ctx* ctx, int64_t* launches, int64_t* builds) {
+    if (!ctx || !launches || !builds) return fail(SIML_EINVAL, "null pointer");
+    *launches = ctx->graph_launches;
+    *builds = ctx->graph_builds;
     return SIML_OK;
 }
 
@@ -4061,10 +4331,11 @@ int siml_call_bytes(siml_ctx* ctx, int op, int64_t rows, int32_t k, int64_t* byt
         return fail(SIML_EINVAL, "siml_call_bytes: unknown operation %d", op);
     // launch shapes the call may take: the handle's, or every candidate of
     // the autotuner (a bucket's first call runs them all)
-    std::vector<std::pair<int64_t, unsigned>> shapes{{ctx->target_wg, ctx->topk_wave_cap}};
+    std::vector<TuneShape> shapes{{ctx->target_wg, ctx->topk_wave_cap, ctx->topk_seed, ctx->persistent}};
     if (ctx->autotune) {
-        const int top = op == SIML_OP_TOPK ? kTuneTopk : kTuneNeighbors;
-        for (int i = 0; i < tune_candidates(top); ++i) shapes.emplace_back(kCands[i].target_wg, kCands[i].topk_wave_cap);
+        const int top = op == SIML_OP_TOPK ? kTuneTopk : op == SIML_OP_HIST1000 ? kTuneHist1000 : kTuneNeighbors;
+        for (int i = 0; i < kNumCands; ++i)
+            if (tune_candidate(top, i)) shapes.push_back(kCands[i]);
     }
     // plans: the fewest row groups the rows can form (the most column
     // ranges per group), with each table count allowed, or every row on the
@@ -4078,9 +4349,13 @@ int siml_call_bytes(siml_ctx* ctx, int op, int64_t rows, int32_t k, int64_t* byt
         for (const auto& pl : plans) {
             size_t b = 0;
             if (op == SIML_OP_TOPK)
-                b = topk_shape(ctx, rows, pl.first, gmin, pl.second, k, sh.first, sh.second).scratch();
+                b = topk_shape(ctx, rows, pl.first, gmin, pl.second, k, sh.target_wg, sh.topk_wave_cap,
+                               kTopkBufferBudget, sh.persistent)
+                        .scratch();
             else if (op == SIML_OP_NEIGHBORS)
-                b = nbr_shape(ctx, rows, pl.first, gmin, 1, true, sh.first).scratch();
+                b = nbr_shape(ctx, rows, pl.first, gmin, 1, true, sh.target_wg).scratch();
+            else if (sh.persistent)
+                b = 256;  // the persistent grid's queue counters
             most = std::max(most, b);
         }
     // results on the device and the per-call lists (row list, group list)

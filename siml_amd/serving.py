@@ -136,13 +136,15 @@ class GpuExecutor:
     """Runs batches on one comparator (its own HIP stream) attached to the
     database."""
 
-    def __init__(self, db, device=0, autotune=False, group_tables=2):
+    def __init__(self, db, device=0, autotune=False, group_tables=2, graphs=False):
         from .GPULingo import GPULingo
         self.g = GPULingo(device)
         self.g.attach_db(db)
         self.g.set_group_tables(group_tables)
         if autotune:
             self.g.set_autotune(True)
+        if graphs:
+            self.g.set_graphs(True)
 
     def call_bytes(self, op, rows, k):
         """siml_call_bytes of this comparator."""
@@ -175,11 +177,12 @@ class Server:
     ``executor_factory(slot)`` (optional) builds a slot's executor: an
     object with ``run(batch)`` returning the batch's (idx, t) or (offsets,
     columns) and ``call_bytes(op, rows, k)``; the default builds
-    ``GpuExecutor``s.
+    ``GpuExecutor``s, whose top-k batches run as HIP graphs when
+    ``graphs`` is set (siml_set_graphs).
     """
 
     def __init__(self, db, slots=2, max_rows=2048, mem_budget=8 << 30, device=0, autotune=False, group_tables=2,
-                 nbr_allowance=64 << 10, executor_factory=None):
+                 nbr_allowance=64 << 10, executor_factory=None, graphs=False):
         self.db = db
         self.max_rows = int(max_rows)
         self.mem_budget = int(mem_budget)
@@ -192,7 +195,7 @@ class Server:
         self.workers = []
         if executor_factory is None:
             def executor_factory(slot):
-                return GpuExecutor(db, device, autotune, group_tables)
+                return GpuExecutor(db, device, autotune, group_tables, graphs)
         self.executor_factory = executor_factory
         self.executors = [executor_factory(i) for i in range(slots)]
         self.alive = slots
