@@ -865,30 +865,46 @@ __device__ __forceinline__ void probe8(const uint2* tab, const int4 (&v)[4], uin
     csa8<NB>(P, m);
 }
 
-__device__ __forceinline__ void load8(int4 (&v)[4], const int4* __restrict__ p) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = p[q * kWave];
+// This lane's byte offset (16 x lane) in a tile of the level stream,
+// computed where it is used: a value held across the tile loops is one the
+// register allocator spills under the kernels' register limits, and the
+// reload's wait then also waits for every store in flight.
+__device__ __forceinline__ uint32_t lane_x16() {
+    uint32_t l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0\n\tv_lshlrev_b32 %0, 4, %0"
+                 : "=v"(l));
+    return l;
 }
 
-// Streams the Lt (a multiple of 8) levels of one query molecule; p is the
-// lane's first 16-byte pair.  Two register sets alternate, so the loads of
-// the next 8 levels are in flight while the current 8 are probed.
+// Loads the 8 levels of block k of a tile: `base` is the tile's stream
+// (wave-uniform), voff the lane's byte offset.  Addresses are a scalar base
+// plus 32-bit vector offsets (a tile holds at most 256 x 64 levels).
+__device__ __forceinline__ void load8(int4 (&v)[4], const char* __restrict__ base, uint32_t voff, int k) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        v[q] = *reinterpret_cast<const int4*>(base + (voff + (uint32_t)(k * 4 + q) * (uint32_t)(kWave * 16)));
+}
+
+// Streams the Lt (a multiple of 8) levels of one query molecule; base is
+// the tile's stream.  Two register sets alternate, so the loads of the
+// next 8 levels are in flight while the current 8 are probed.
 template <int NB>
-__device__ __forceinline__ void bits_accumulate(const uint2* tab, const int4* __restrict__ p, int Lt,
+__device__ __forceinline__ void bits_accumulate(const uint2* tab, const char* __restrict__ base, int Lt,
                                                 uint32_t (&P)[NB]) {
 #pragma unroll
     for (int k = 0; k < NB; ++k) P[k] = 0u;
     const int nblk = Lt >> 3;
     if (nblk <= 0) return;
+    const uint32_t voff = lane_x16();
     int4 a[4], b[4];
-    load8(a, p);
+    load8(a, base, voff, 0);
     int k = 0;
     for (; k + 2 <= nblk; k += 2) {
-        load8(b, p + (size_t)(k + 1) * 4 * kWave);
+        load8(b, base, voff, k + 1);
         probe8<NB>(tab, a, P);
         // may read the 8 levels after this molecule's last: the stream
         // carries one spare block at its end, so this stays in bounds
-        load8(a, p + (size_t)(k + 2) * 4 * kWave);
+        load8(a, base, voff, k + 2);
         probe8<NB>(tab, b, P);
     }
     if (k < nblk) probe8<NB>(tab, a, P);
@@ -1429,7 +1445,7 @@ __device__ __forceinline__ void bits_prologue(const BitGroupArgs& G, const int32
 // GT tables of the workgroup: each block of 8 levels is loaded once and
 // probed in every table, so the level stream is read once per GT groups.
 template <int NB, int GT>
-__device__ __forceinline__ void bits_accumulate_g(const uint2 (*tab)[siml::kBitSlots], const int4* __restrict__ p,
+__device__ __forceinline__ void bits_accumulate_g(const uint2 (*tab)[siml::kBitSlots], const char* __restrict__ base,
                                                   int Lt, uint32_t (&P)[GT][NB]) {
 #pragma unroll
     for (int q = 0; q < GT; ++q)
@@ -1440,11 +1456,12 @@ __device__ __forceinline__ void bits_accumulate_g(const uint2 (*tab)[siml::kBitS
     // one table's 16 table reads are in flight at a time: the scheduling
     // barriers keep the compiler from hoisting every table's reads (and
     // their result registers) ahead of the first table's adder tree
+    const uint32_t voff = lane_x16();
     int4 a[4], b[4];
-    load8(a, p);
+    load8(a, base, voff, 0);
     int k = 0;
     for (; k + 2 <= nblk; k += 2) {
-        load8(b, p + (size_t)(k + 1) * 4 * kWave);
+        load8(b, base, voff, k + 1);
 #pragma unroll
         for (int q = 0; q < GT; ++q) {
             probe8<NB>(tab[q], a, P[q]);
@@ -1452,7 +1469,7 @@ __device__ __forceinline__ void bits_accumulate_g(const uint2 (*tab)[siml::kBitS
         }
         // may read the 8 levels after this molecule's last: the stream
         // carries one spare block at its end, so this stays in bounds
-        load8(a, p + (size_t)(k + 2) * 4 * kWave);
+        load8(a, base, voff, k + 2);
 #pragma unroll
         for (int q = 0; q < GT; ++q) {
             probe8<NB>(tab[q], b, P[q]);
@@ -1512,6 +1529,10 @@ __global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_
     const int32_t row_lane = ri.row[lane & (siml::kBitRows - 1)];
     const int32_t ma_lane = ri.ma[lane & (siml::kBitRows - 1)];
     const uint32_t live = (uint32_t)__ballot(lane < siml::kBitRows && row_lane >= 0);
+    // tile indices and in-tile column counts are 32-bit (tiles < 2^31): the
+    // block loop's tests are then scalar compares, with no 64-bit values
+    // copied to vector registers
+    const int ntl = (int)C.ntiles;
     for (int64_t blk = b0; blk < b1; ++blk) {
         uint32_t Q[4][NB];
         float mbf[4];
@@ -1519,42 +1540,50 @@ __global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_
         int loc[4];
         int64_t toff[4];
         int32_t tlen[4];
+        const int tb = (int)blk * 4;
+        // the four tiles' level offsets and lengths: unconditional scalar
+        // loads (clamped to the last tile) issued together, a missing tile
+        // gets length 0
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int64_t t = blk * 4 + q;
-            toff[q] = t < C.ntiles ? load_uniform(C.btoff, t) : 0;
-            tlen[q] = t < C.ntiles ? load_uniform(C.btlen, t) : 0;
+            const int t = min(tb + q, ntl - 1);
+            toff[q] = load_uniform(C.btoff, t);
+            const int32_t len = load_uniform(C.btlen, t);
+            tlen[q] = tb + q < ntl ? len : 0;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int64_t t = blk * 4 + q;
-            const int64_t c = t * kWave + lane;
-            if (t < C.ntiles) {
-                bits_accumulate<NB>(tab, bt4 + (toff[q] >> 1) + lane, tlen[q], Q[q]);
-            } else {
-#pragma unroll
-                for (int k = 0; k < NB; ++k) Q[q][k] = 0u;
-            }
+            bits_accumulate<NB>(tab, reinterpret_cast<const char*>(bt4 + (toff[q] >> 1)), tlen[q], Q[q]);
+
             transpose8(Q[q]);
             if constexpr (NB == 16) transpose8(Q[q] + 8);
-            (void)c;
         }
         // the columns' magnitudes and in-block positions, read after the
         // accumulation so that they hold no registers during it
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int64_t c = (blk * 4 + q) * kWave + lane;
-            const bool valid = c < C.n;
+            const int64_t c0 = (int64_t)(tb + q) * kWave;
+            const int in_tile = (int)std::max<int64_t>(0, std::min<int64_t>(C.n - c0, kWave));
+            const bool valid = lane < in_tile;
+            const int64_t c = c0 + lane;
             const int mb = valid ? C.mag[c] : 0;
             mbf[q] = (float)mb;
             zb[q] = mb == 0;
             loc[q] = valid ? (int)(col_of(C, c) - blk * 4 * kWave) : 4 * kWave + lane;
         }
         const int64_t cbase = blk * 4 * kWave;
-        const bool full = cbase + 4 * kWave <= C.n;
+        // columns of this block (4 kWave except in the last block), so that
+        // the store test is one 32-bit compare per lane and keeps no 64-bit
+        // column values live across the rows
+        const int rem = (int)std::min<int64_t>(C.n - cbase, 4 * kWave);
+        const int o4 = 4 * lane;
+        // opaque to the compiler per block: otherwise it hoists the 32 row
+        // tests below out of the block loop into 64 live SGPRs
+        uint32_t lv = live;
+        asm volatile("" : "+s"(lv));
 #pragma unroll
         for (int r = 0; r < siml::kBitRows; ++r) {
-            if (!((live >> r) & 1u)) continue;
+            if (!((lv >> r) & 1u)) continue;
             const int32_t row = __builtin_amdgcn_readlane(row_lane, r);
             const int ma = __builtin_amdgcn_readlane(ma_lane, r);
             const bool za = ma == 0;
@@ -1564,14 +1593,16 @@ __global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_
                 stage[loc[q]] = pair_tanimoto<FAST>(count_of<NB>(Q[q], r), maf, mbf[q], za || zb[q]);
             wave_lds_sync();
             float* orow = epi.out + (int64_t)(row - epi.rb) * epi.opitch + cbase;
-            if (full) {
-                reinterpret_cast<float4*>(orow)[lane] = reinterpret_cast<const float4*>(stage)[lane];
+            if (o4 + 4 <= rem) {
+                // the lane index recomputed here (two VALU ops) rather than
+                // held: under the register limit the compiler otherwise
+                // spills it and reloads it for every row, and the reload's
+                // wait also waits for the previous rows' stores
+                int l;
+                asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+                reinterpret_cast<float4*>(orow)[l] = reinterpret_cast<const float4*>(stage)[l];
             } else {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int o = 4 * lane + i;
-                    if (cbase + o < C.n) orow[o] = stage[o];
-                }
+                for (int i = 0; i < 4 && o4 + i < rem; ++i) orow[o4 + i] = stage[o4 + i];
             }
             wave_lds_sync();
         }
@@ -1638,7 +1669,7 @@ __device__ __forceinline__ void bits_item(const ColArgs& C, const Epi& epi, cons
         const int mb = valid ? C.mag[c] : 0;
         const int64_t col = valid ? col_of(C, c) : 0;
         uint32_t P[GT][NB];
-        bits_accumulate_g<NB, GT>(tab, bt4 + (off >> 1) + lane, len, P);
+        bits_accumulate_g<NB, GT>(tab, reinterpret_cast<const char*>(bt4 + (off >> 1)), len, P);
         // bit r of rows_any[q]: row r of table q has a non-zero count in some
         // lane of the tile
         uint32_t rows_any[GT];
@@ -2342,7 +2373,8 @@ struct TimedRegion {
 // launch() (an error return) ends the capture and drops the graph.
 struct GraphRegion {
     siml_ctx* ctx;
-    bool on = false;
+    bool on = false;                          // capturing
+    siml_ctx::GraphExec* ready = nullptr;     // instance to launch after end()
     explicit GraphRegion(siml_ctx* c) : ctx(c) {}
     int begin() {
         if (!ctx->graphs) return SIML_OK;
@@ -2353,6 +2385,9 @@ struct GraphRegion {
         on = true;
         return SIML_OK;
     }
+    // Ends the capture and brings the cached instance of the graph's
+    // signature up to date (or instantiates one); launch() then launches it.
+    int end();
     int launch();
     ~GraphRegion() {
         if (!on) return;
@@ -2387,7 +2422,7 @@ int graph_signature(hipGraph_t g, uint64_t* sig) {
     return SIML_OK;
 }
 
-int GraphRegion::launch() {
+int GraphRegion::end() {
     if (!on) return SIML_OK;
     on = false;
     hipGraph_t g = nullptr;
@@ -2419,6 +2454,14 @@ int GraphRegion::launch() {
     }
     (void)hipGraphDestroy(g);
     if (rc != SIML_OK) return rc;
+    ready = ge;
+    return SIML_OK;
+}
+
+int GraphRegion::launch() {
+    if (!ready) return SIML_OK;
+    siml_ctx::GraphExec* ge = ready;
+    ready = nullptr;
     HIP_TRY(hipGraphLaunch(ge->exec, ctx->stream));
     if (!ge->done) HIP_TRY(hipEventCreateWithFlags(&ge->done, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(ge->done, ctx->stream));
@@ -4038,6 +4081,8 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
                seed != nullptr));
     tr.done();
     if (gr.on) {
+        // the capture ends before the timing event is recorded
+        TRY(gr.end());
         TimedRegion tg(ctx, nr * nq, mem == nullptr);
         TRY(gr.launch());
         tg.done();
@@ -4105,6 +4150,7 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     }
     tr.done();
     if (gr.on) {
+        TRY(gr.end());
         TimedRegion tg(ctx, nr * nq);
         TRY(gr.launch());
         tg.done();
