@@ -189,7 +189,8 @@ def configure(g, args):
     h = g._h
     h.call("siml_set_layout", {"natural": _lib.LAYOUT_NATURAL, "block": _lib.LAYOUT_BLOCK,
                                "sorted": _lib.LAYOUT_SORTED}[args.layout])
-    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "merge": _lib.POLICY_MERGE}[args.policy])
+    h.call("siml_set_policy", {"auto": _lib.POLICY_AUTO, "merge": _lib.POLICY_MERGE, "bits": _lib.POLICY_BITS,
+                               "sparse": _lib.POLICY_SPARSE}[args.policy])
     g.set_group_tables(args.group_tables)
 
 
@@ -975,8 +976,10 @@ def main():
                     help="most row groups one workgroup of the bit-sliced kernels serves")
     ap.add_argument("--launch-shape", default="",
                     help="batched: fixed launch shape 'workgroups,topk_wave_cap,topk_seed' instead of the autotuner")
-    ap.add_argument("--policy", choices=["auto", "merge"], default="auto",
-                    help="auto: bit-sliced 32-row kernels; merge: one-row-per-workgroup merge kernel")
+    ap.add_argument("--policy", choices=["auto", "merge", "bits", "sparse"], default="auto",
+                    help="auto: bit-sliced 32-row kernels, and for the float32 matrix the sparse (inverted-index) "
+                         "path when the query set's postings are short; bits: never the sparse path; sparse: "
+                         "always it; merge: one-row-per-workgroup merge kernel")
     ap.add_argument("--layout", choices=["block", "natural", "sorted"], default="block",
                     help="query storage order: size-sorted 256-column blocks, the caller's order, or fully "
                          "size-sorted")

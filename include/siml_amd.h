@@ -234,12 +234,18 @@ int siml_kernel_split(siml_ctx* ctx, double* similarity_ms, double* selection_ms
 int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset);
 /* Kernel policy: AUTO serves sorted-unique sets with the bit-sliced
  * multi-row kernels (rows that do not qualify, and unsorted sets, use the
- * merge kernel); MERGE uses the merge kernel for everything.  Both give the
- * same bits.  The bit-sliced row groups are built by set_ref when the policy
- * in force then is AUTO; a reference set given under MERGE is served by the
- * merge kernel. */
+ * merge kernel) and, for the float32 matrix (multirow), chooses per call
+ * between the bit-sliced kernel and the sparse path (an inverted index of
+ * the query set's levels, built on first use, and per-pair counts in LDS)
+ * from the index's expected postings per level; BITS never takes the sparse
+ * path; SPARSE takes it whenever the query set has an index; MERGE uses the
+ * merge kernel for everything.  All give the same bits.  The bit-sliced row
+ * groups are built by set_ref when the policy in force then is not MERGE; a
+ * reference set given under MERGE is served by the merge kernel. */
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
+#define SIML_POLICY_BITS 2
+#define SIML_POLICY_SPARSE 3
 int siml_set_policy(siml_ctx* ctx, int policy);
 /* Storage order used by the next set_query: NATURAL keeps the caller's
  * order; BLOCK (default) sorts the query molecules by size inside blocks

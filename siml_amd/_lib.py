@@ -22,7 +22,7 @@ SIML_ELIMIT = -5
 RESULT_NONE, RESULT_F32, RESULT_I32, RESULT_CSR, RESULT_TOPK, RESULT_I64 = range(6)
 MAX_ROW_LINGOS = 8192
 MAX_TOPK = 256
-POLICY_AUTO, POLICY_MERGE = 0, 1
+POLICY_AUTO, POLICY_MERGE, POLICY_BITS, POLICY_SPARSE = 0, 1, 2, 3
 LAYOUT_NATURAL, LAYOUT_BLOCK, LAYOUT_SORTED = 0, 1, 2
 OP_TOPK, OP_NEIGHBORS, OP_HIST1000 = 1, 2, 3
 NOALLOC_SCRATCH_BYTES = 128 << 20

@@ -1761,6 +1761,240 @@ __global__ __launch_bounds__(kBlock * Epi::kG, Epi::kG > 1 ? 4 : 1) void k_bits(
 }
 
 
+// ---------------------------------------------------------------------------
+// Sparse path of the float32 matrix: an inverted index of the query set.
+//
+// The level stream of a query set holds, for every query molecule, the keys
+// of its levels (x, t) (bitgroups.h).  The index maps each distinct key to
+// the query molecules (caller column indices) holding that level: an
+// open-addressing table of S = 2^k slots (key, count, offset) over one
+// posting array.  A grouped reference row's intersection with every query
+// molecule is then the number of its levels' postings naming that molecule,
+// counted in LDS; the row's float32 Tanimoto values follow from the counts
+// for all columns, most of them 0.  When the query molecules share few
+// levels (postings are short) the counting is a small fraction of the
+// matrix's pairs and the kernel is bound by writing the matrix; the handle
+// chooses this path per call from the index's posting statistics (the
+// expected postings per level).
+// ---------------------------------------------------------------------------
+
+// Level key of (x, t), as siml::level_key (bitgroups.h) on the host.
+__device__ __forceinline__ uint32_t dev_level_key(int32_t x, uint32_t t) {
+    return (uint32_t)x | ((t & 1u) << 7) | ((t & 2u) << 14) | ((t & 4u) << 21) | ((t & 8u) << 28);
+}
+
+// Home slot of a key in the index table (a 32-bit mix, masked).
+__device__ __forceinline__ uint32_t post_home(uint32_t key, uint32_t mask) {
+    uint32_t h = key * 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h & mask;
+}
+
+struct PostingArgs {
+    uint32_t* key;        // slot keys, 0 = empty
+    int32_t* cnt;         // postings per slot
+    const int64_t* off;   // first posting of each slot (exclusive scan of cnt)
+    int32_t* col;         // caller column indices, grouped by slot
+    int32_t* cur;         // fill cursors (index build only)
+    uint32_t mask;        // slots - 1
+    int32_t* overflow;    // set when a key found no slot (index build only)
+};
+
+// Slot of `key`, or -1 when the table holds no such key.  Linear probing
+// from the home slot ends at the key or at an empty slot; the table is at
+// most half full, so an empty slot exists.
+__device__ __forceinline__ int64_t post_find(const PostingArgs& P, uint32_t key) {
+    uint32_t s = post_home(key, P.mask);
+    for (uint32_t i = 0; i <= P.mask; ++i) {
+        const uint32_t k = P.key[s];
+        if (k == key) return s;
+        if (k == 0u) return -1;
+        s = (s + 1) & P.mask;
+    }
+    return -1;
+}
+
+// Visits every level of the level stream: f(key, storage position).  One
+// wave per tile, lane l holds the levels of the tile's molecule l; padding
+// levels (key 0) are skipped.
+template <class F>
+__device__ __forceinline__ void for_stream_levels(const ColArgs& C, F&& f) {
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t t = wave; t < C.ntiles; t += nwaves) {
+        const int64_t p = t * kWave + lane;
+        if (p >= C.n) continue;
+        const int64_t base = load_uniform(C.btoff, t);
+        const int32_t lt = load_uniform(C.btlen, t);
+        for (int32_t j = 0; j < lt; ++j) {
+            const int2 e = C.btiles[base + (int64_t)(j >> 1) * 2 * kWave + 2 * lane + (j & 1)];
+            if (e.x != 0) f((uint32_t)e.x, p);
+        }
+    }
+}
+
+// Index build, pass 1: every stream level's key gets a slot (inserted by
+// compare-and-swap) and counts one posting there.
+__global__ __launch_bounds__(kBlock) void k_post_insert(ColArgs C, PostingArgs P) {
+    for_stream_levels(C, [&](uint32_t key, int64_t) {
+        uint32_t s = post_home(key, P.mask);
+        for (uint32_t i = 0; i <= P.mask; ++i) {
+            uint32_t k = P.key[s];
+            if (k == 0u) k = atomicCAS(P.key + s, 0u, key);
+            if (k == 0u || k == key) {
+                atomicAdd(P.cnt + s, 1);
+                return;
+            }
+            s = (s + 1) & P.mask;
+        }
+        atomicExch(P.overflow, 1);
+    });
+}
+
+// Index build, pass 2 (after the scan of the counts into P.off): every
+// stream level writes its caller column into its slot's postings.
+__global__ __launch_bounds__(kBlock) void k_post_fill(ColArgs C, PostingArgs P) {
+    for_stream_levels(C, [&](uint32_t key, int64_t p) {
+        const int64_t s = post_find(P, key);
+        if (s < 0) return;
+        const int32_t at = atomicAdd(P.cur + s, 1);
+        P.col[P.off[s] + at] = (int32_t)col_of(C, p);
+    });
+}
+
+// Sum over slots of count^2 (stats[0]) and of count (stats[1]): the
+// expected postings of a stream level drawn at random is stats[0] / stats[1].
+__global__ __launch_bounds__(kBlock) void k_post_stats(const int32_t* __restrict__ cnt, int64_t slots,
+                                                       unsigned long long* stats) {
+    unsigned long long sq = 0, sum = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long c = (unsigned long long)cnt[i];
+        sq += c * c;
+        sum += c;
+    }
+    sq = wave_sum_u64(sq);
+    sum = wave_sum_u64(sum);
+    if (lane_id() == 0) {
+        atomicAdd(stats, sq);
+        atomicAdd(stats + 1, sum);
+    }
+}
+
+// Magnitudes in caller column order (4 trailing zeros: the sparse kernel
+// reads them four at a time).
+__global__ __launch_bounds__(kBlock) void k_caller_mags(ColArgs C, int32_t* __restrict__ cmag) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < C.n + 4; p += (int64_t)gridDim.x * blockDim.x)
+        if (p < C.n)
+            cmag[col_of(C, p)] = C.mag[p];
+        else
+            cmag[p] = 0;
+}
+
+// Reference rows per workgroup and query molecules per chunk of the sparse
+// kernel: one byte per pair in LDS, kSparseRows x kSparseCols = 64 KiB.
+constexpr int kSparseRows = 2;
+constexpr int kSparseCols = 32768;
+// most levels of a listed row (a byte holds every count)
+constexpr int kSparseMaxLevels = 255;
+
+// Float32 rows of the listed reference rows (grouped rows with at most 255
+// levels) from the inverted index: workgroup (x, y) counts rows
+// [kSparseRows x, kSparseRows (x + 1)) of the list against query molecules
+// [y kSparseCols, (y + 1) kSparseCols), then writes their Tanimoto values
+// with the same float expression as every other kernel.  Counting first
+// lists the rows' levels with their posting ranges in LDS; each wave then
+// walks whole levels, its lanes reading 64 consecutive postings at a time,
+// so a level with long postings is spread over a wave.
+constexpr int kSparseBlock = 1024;  // 16 waves: posting reads in flight to hide their latency
+
+template <bool FAST>
+__global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const int32_t* __restrict__ rowlist, int64_t nlist,
+                                                       PostingArgs P, const int32_t* __restrict__ cmag, int64_t nq,
+                                                       int64_t rb, float* __restrict__ out, int64_t pitch) {
+    __shared__ uint32_t cnt[kSparseRows][kSparseCols / 4];
+    __shared__ int64_t lv_start[kSparseRows * kSparseMaxLevels];
+    __shared__ int32_t lv_len[kSparseRows * kSparseMaxLevels];
+    __shared__ int32_t nlv;
+    const int64_t c0 = (int64_t)blockIdx.y * kSparseCols;
+    const int cw = (int)min((int64_t)kSparseCols, nq - c0);
+    int32_t rows[kSparseRows];
+#pragma unroll
+    for (int i = 0; i < kSparseRows; ++i) {
+        const int64_t k = (int64_t)blockIdx.x * kSparseRows + i;
+        rows[i] = k < nlist ? load_uniform(rowlist, k) : -1;
+    }
+    for (int i = threadIdx.x; i < kSparseRows * kSparseCols / 4; i += blockDim.x) (&cnt[0][0])[i] = 0u;
+    if (threadIdx.x == 0) nlv = 0;
+    __syncthreads();
+    // the rows' levels found in the index: (posting start, count, row)
+#pragma unroll
+    for (int i = 0; i < kSparseRows; ++i) {
+        if (rows[i] < 0) continue;
+        const int64_t o = R.off[rows[i]];
+        const int32_t len = R.len[rows[i]];
+        for (int e = threadIdx.x; e < len; e += blockDim.x) {
+            const int2 el = R.elems[o + e];
+            for (int t = 1; t <= el.y; ++t) {
+                const int64_t s = post_find(P, dev_level_key(el.x, (uint32_t)t));
+                if (s < 0) break;  // no query molecule has level t, so none has t + 1
+                const int at = atomicAdd(&nlv, 1);
+                lv_start[at] = P.off[s];
+                lv_len[at] = P.cnt[s] | (i << 28);
+            }
+        }
+    }
+    __syncthreads();
+    const int n = nlv;
+    const int lane = lane_id(), w = (int)(threadIdx.x / kWave), nwv = (int)(blockDim.x / kWave);
+    // each wave walks whole levels; a level with more than 192 postings is
+    // read four passes at a time (four independent reads in flight per lane)
+    for (int l = w; l < n; l += nwv) {
+        const int64_t st = lv_start[l];
+        const int32_t lr = lv_len[l];
+        const int32_t ln = lr & 0x0FFFFFFF;
+        uint32_t* crow = cnt[lr >> 28];
+        int q = lane;
+        for (; q + 3 * kWave < ln; q += 4 * kWave) {
+            int64_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = (int64_t)P.col[st + q + u * kWave] - c0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (c[u] >= 0 && c[u] < cw) atomicAdd(crow + (c[u] >> 2), 1u << (8 * (c[u] & 3)));
+        }
+        for (; q < ln; q += kWave) {
+            const int64_t c = (int64_t)P.col[st + q] - c0;
+            if (c >= 0 && c < cw) atomicAdd(crow + (c >> 2), 1u << (8 * (c & 3)));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSparseRows; ++i) {
+        if (rows[i] < 0) continue;
+        const int ma = R.mag[rows[i]];
+        const float maf = (float)ma;
+        float* orow = out + ((int64_t)rows[i] - rb) * pitch + c0;
+        for (int j = threadIdx.x; 4 * j < cw; j += blockDim.x) {
+            const uint32_t wd = cnt[i][j];
+            const int4 mb = *reinterpret_cast<const int4*>(cmag + c0 + 4 * j);
+            float4 v;
+            v.x = pair_tanimoto<FAST>((float)(wd & 0xFFu), maf, (float)mb.x, ma == 0 || mb.x == 0);
+            v.y = pair_tanimoto<FAST>((float)((wd >> 8) & 0xFFu), maf, (float)mb.y, ma == 0 || mb.y == 0);
+            v.z = pair_tanimoto<FAST>((float)((wd >> 16) & 0xFFu), maf, (float)mb.z, ma == 0 || mb.z == 0);
+            v.w = pair_tanimoto<FAST>((float)(wd >> 24), maf, (float)mb.w, ma == 0 || mb.w == 0);
+            if (4 * j + 4 <= cw) {
+                reinterpret_cast<float4*>(orow)[j] = v;
+            } else {
+                const float vv[4] = {v.x, v.y, v.z, v.w};
+                for (int k = 0; k < cw - 4 * j; ++k) orow[4 * j + k] = vv[k];
+            }
+        }
+    }
+}
+
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
 // per-row 101-bin histogram; per-workgroup LDS bins, flushed with global
 // atomics because several workgroups share a row.
@@ -2221,6 +2455,23 @@ struct ColSet {
     // level stream of the bit-sliced kernels (build_colset)
     DevBuf btiles, btoff, btlen;
     bool levels_ok = false;
+    // inverted index of the level stream (the sparse path of the float32
+    // matrix), built on first use by any handle using the set
+    struct Postings {
+        bool built = false, ok = false;
+        int64_t slots = 0, total = 0;
+        double per_level = 0.0;  // expected postings of a stream level
+        DevBuf key, cnt, off, col, cmag;
+        void release() {
+            key.release();
+            cnt.release();
+            off.release();
+            col.release();
+            cmag.release();
+            built = ok = false;
+        }
+    } post;
+    std::mutex post_mu;
     ColArgs args() const {
         return {tiles.as<int2>(), toff.as<int64_t>(), tlen.as<int32_t>(), len.as<int32_t>(), mag.as<int32_t>(),
                 csum.as<int32_t>(), identity ? nullptr : perm.as<int32_t>(), btiles.as<int2>(), btoff.as<int64_t>(),
@@ -2238,6 +2489,7 @@ struct ColSet {
         btiles.release();
         btoff.release();
         btlen.release();
+        post.release();
         n = -1;
     }
 };
@@ -2626,7 +2878,7 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
     R.h_len = std::move(info.len);
-    if (ctx->policy == SIML_POLICY_AUTO) TRY(ensure_bits(ctx, 0));
+    if (ctx->policy != SIML_POLICY_MERGE) TRY(ensure_bits(ctx, 0));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
 }
@@ -2831,7 +3083,7 @@ struct BitPlan {
 int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false, int32_t* glist_dst = nullptr) {
     RowSet& R = ctx->ref;
     p = BitPlan();
-    if (ctx->policy != SIML_POLICY_AUTO || !ctx->qry->levels_ok || !R.bits[0].built) return SIML_OK;
+    if (ctx->policy == SIML_POLICY_MERGE || !ctx->qry->levels_ok || !R.bits[0].built) return SIML_OK;
     p.which = by_mag ? 1 : 0;
     TRY(ensure_bits(ctx, p.which));
     const RowSet::BitGrouping& B = R.bits[p.which];
@@ -3646,6 +3898,123 @@ NbrShape nbr_shape(const siml_ctx* ctx, int64_t nr, bool bits, int64_t groups, i
 
 extern "C" {
 
+// Largest index table (slots) built for a query set: the tables take 20 B
+// per slot; a set with more distinct level keys than fit at half load is
+// not indexed and keeps the bit-sliced path.
+constexpr int64_t kPostMaxSlots = int64_t(1) << 25;
+
+// Builds the inverted index of the handle's query set unless it exists
+// (ColSet::post), on the handle's stream; returns when it is complete.
+// Nothing is indexed for a set without a level stream.
+int ensure_postings(siml_ctx* ctx) {
+    ColSet& C = *ctx->qry;
+    std::lock_guard<std::mutex> lk(C.post_mu);
+    if (C.post.built) return SIML_OK;
+    ColSet::Postings& X = C.post;
+    X.built = true;
+    X.ok = false;
+    if (!C.levels_ok || C.n <= 0) return SIML_OK;
+    // slots: a power of two at least twice the stream's entries (a bound on
+    // its distinct keys), at most kPostMaxSlots
+    int64_t slots = 1024;
+    while (slots < 2 * C.btiles_total && slots < kPostMaxSlots) slots <<= 1;
+    X.slots = slots;
+    TRY(X.key.reserve(sizeof(uint32_t) * (size_t)slots));
+    TRY(X.cnt.reserve(sizeof(int32_t) * (size_t)slots));
+    TRY(X.off.reserve(sizeof(int64_t) * (size_t)slots));
+    DevBuf cur, tmp;
+    TRY(cur.reserve(sizeof(int32_t) * (size_t)slots));
+    const size_t scan_bytes = siml::csr_scan_bytes(slots);
+    TRY(tmp.reserve(scan_bytes + 256));
+    // after the scan's scratch: {0, total} row offsets, the overflow flag,
+    // then {sum of count^2, sum of count}
+    int64_t* tail = reinterpret_cast<int64_t*>(tmp.as<char>() + scan_bytes);
+    int32_t* flag = reinterpret_cast<int32_t*>(tail + 2);
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(tail + 4);
+    const unsigned fb = (unsigned)std::min<int64_t>((slots + 255) / 256, 65535);
+    k_fill<uint32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(X.key.as<uint32_t>(), slots, 0u);
+    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(X.cnt.as<int32_t>(), slots, 0);
+    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cur.as<int32_t>(), slots, 0);
+    k_fill<int64_t><<<dim3(1), dim3(64), 0, ctx->stream>>>(tail, 8, 0);
+    TRY(launch_check("k_fill<postings>"));
+    PostingArgs P{X.key.as<uint32_t>(), X.cnt.as<int32_t>(), X.off.as<int64_t>(), nullptr, cur.as<int32_t>(),
+                  (uint32_t)(slots - 1), flag};
+    const ColArgs CA = C.args();
+    const unsigned wb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((C.ntiles + 3) / 4, 16384));
+    k_post_insert<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
+    TRY(launch_check("k_post_insert"));
+    TRY(siml::csr_scan(X.cnt.as<int32_t>(), 1, slots, X.off.as<int64_t>(), tail, tmp.p, ctx->stream));
+    k_post_stats<<<dim3(std::min<unsigned>(fb, 2048)), dim3(256), 0, ctx->stream>>>(X.cnt.as<int32_t>(), slots, stats);
+    TRY(launch_check("k_post_stats"));
+    int64_t host[8] = {0};
+    HIP_TRY(hipMemcpyAsync(host, tail, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t total = host[1];
+    const int32_t overflow = (int32_t)(host[2] & 0xFFFFFFFF);
+    const unsigned long long sq = (unsigned long long)host[4], sum = (unsigned long long)host[5];
+    if (overflow || total <= 0 || (unsigned long long)total != sum) {
+        X.release();
+        X.built = true;
+        return SIML_OK;
+    }
+    TRY(X.col.reserve(sizeof(int32_t) * (size_t)total));
+    TRY(X.cmag.reserve(sizeof(int32_t) * (size_t)(C.n + 4)));
+    P.col = X.col.as<int32_t>();
+    k_post_fill<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
+    TRY(launch_check("k_post_fill"));
+    k_caller_mags<<<dim3((unsigned)std::min<int64_t>((C.n + 4 + 255) / 256, 65535)), dim3(256), 0, ctx->stream>>>(
+        CA, X.cmag.as<int32_t>());
+    TRY(launch_check("k_caller_mags"));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    X.total = total;
+    X.per_level = (double)sq / (double)sum;
+    X.ok = true;
+    return SIML_OK;
+}
+
+// Cost model of AUTO's choice (float32 matrix), in picoseconds of one
+// MI355X, measured on the 32768 x 32768 synthetic matrix: the sparse kernel
+// takes about 0.78 per pair (writing the matrix) plus 2.8 per posting its
+// rows' levels meet; the bit-sliced kernel about 0.0226 per pair and query
+// level (1.51 per pair at 67 levels per query molecule).
+constexpr double kSparsePsPerPair = 0.78, kSparsePsPerPosting = 2.8, kBitsPsPerPairLevel = 0.0226;
+
+// Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
+// levels) and the rest (merge kernel), when the sparse path applies to the
+// call: policy SPARSE, or AUTO with few expected postings.
+bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, const BitPlan& bp, std::vector<int32_t>& sparse_rows,
+                 std::vector<int32_t>& merge_rows, int* rc) {
+    *rc = SIML_OK;
+    if (!bp.use || (ctx->policy != SIML_POLICY_AUTO && ctx->policy != SIML_POLICY_SPARSE)) return false;
+    *rc = ensure_postings(ctx);
+    if (*rc != SIML_OK) return false;
+    const ColSet::Postings& X = ctx->qry->post;
+    // the sparse kernel packs a level's posting count in 28 bits
+    if (!X.ok || ctx->qry->n >= (int64_t(1) << 28)) return false;
+    const RowSet& R = ctx->ref;
+    sparse_rows.clear();
+    merge_rows.clear();
+    double levels = 0.0;
+    size_t m = 0;
+    for (int64_t r = rb; r < rl; ++r) {
+        while (m < bp.merge.size() && bp.merge[m] < r) ++m;
+        const bool grouped = !(m < bp.merge.size() && bp.merge[m] == r);
+        if (grouped && R.h_csum[(size_t)r] <= kSparseMaxLevels) {
+            sparse_rows.push_back((int32_t)r);
+            levels += (double)R.h_csum[(size_t)r];
+        } else {
+            merge_rows.push_back((int32_t)r);
+        }
+    }
+    if (ctx->policy == SIML_POLICY_SPARSE) return true;
+    // expected postings met: the rows' levels times the postings of a
+    // random stream level (exact when the rows are drawn like the queries)
+    const double pairs = (double)(rl - rb) * (double)ctx->qry->n;
+    const double qlevels = (double)X.total / (double)ctx->qry->n;
+    return kSparsePsPerPair * pairs + kSparsePsPerPosting * levels * X.per_level <
+           kBitsPsPerPairLevel * qlevels * pairs;
+}
+
 static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getMultipleRows"));
     TRY(set_device(ctx));
@@ -3664,10 +4033,48 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp));
+    std::vector<int32_t> srows, mrows;
+    int prc = SIML_OK;
+    const bool sparse = plan_sparse(ctx, rowbase, rowlimit, bp, srows, mrows, &prc);
+    TRY(prc);
+    if (sparse) {
+        // one list: the sparse kernel's rows, then the merge kernel's
+        std::vector<int32_t> both(srows);
+        both.insert(both.end(), mrows.begin(), mrows.end());
+        if (!both.empty()) TRY(upload_rowlist(ctx, both));
+        const ColSet::Postings& X = ctx->qry->post;
+        const PostingArgs P{X.key.as<uint32_t>(), X.cnt.as<int32_t>(), X.off.as<int64_t>(), X.col.as<int32_t>(),
+                            nullptr, (uint32_t)(X.slots - 1), nullptr};
+        const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
+        TimedRegion tr(ctx, nr * nq);
+        if (!srows.empty()) {
+            const dim3 grid((unsigned)((srows.size() + kSparseRows - 1) / kSparseRows),
+                            (unsigned)((nq + kSparseCols - 1) / kSparseCols));
+            if (fast)
+                k_sparse_f32<true><<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(
+                    ctx->ref.args(), ctx->rowlist.as<int32_t>(), (int64_t)srows.size(), P, X.cmag.as<int32_t>(), nq,
+                    rowbase, ctx->res.as<float>(), dp);
+            else
+                k_sparse_f32<false><<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(
+                    ctx->ref.args(), ctx->rowlist.as<int32_t>(), (int64_t)srows.size(), P, X.cmag.as<int32_t>(), nq,
+                    rowbase, ctx->res.as<float>(), dp);
+            TRY(launch_check("k_sparse_f32"));
+        }
+        if (!mrows.empty()) {
+            k_rows_f32<false><<<row_grid((int64_t)mrows.size(), ctx->qry->ntiles), dim3(kBlock), lds, ctx->stream>>>(
+                ctx->ref.args(), ctx->qry->args(), ctx->rowlist.as<int32_t>() + srows.size(), rowbase,
+                ctx->res.as<float>(), dp);
+            TRY(launch_check("k_rows_f32<rowlist>"));
+        }
+        tr.done();
+        ctx->last_kernel = mrows.empty() ? "rows_sparse" : (srows.empty() ? "rows_merge" : "rows_sparse+merge");
+        return SIML_OK;
+    }
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
         const unsigned gy = xcd_gy(
+
             row_grid(std::max<int64_t>((int64_t)bp.groups.size(), 1), ctx->qry->ntiles, ctx->target_wg).y, 65535);
         if (ctx->qry->blocked)
             TRY(launch_bits(ctx, bp, gy, rowbase, rowlimit, EpiF32Block{ctx->res.as<float>(), dp, rowbase},
@@ -4640,7 +5047,9 @@ int siml_retrieve_i64(siml_ctx* ctx, int64_t* host) {
 
 int siml_set_policy(siml_ctx* ctx, int policy) {
     if (!ctx) return fail(SIML_EINVAL, "null handle");
-    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE) return fail(SIML_EINVAL, "unknown policy %d", policy);
+    if (policy != SIML_POLICY_AUTO && policy != SIML_POLICY_MERGE && policy != SIML_POLICY_BITS &&
+        policy != SIML_POLICY_SPARSE)
+        return fail(SIML_EINVAL, "unknown policy %d", policy);
     ctx->policy = policy;
     return SIML_OK;
 }
