@@ -170,6 +170,15 @@ __device__ __forceinline__ void wave_tiles(int64_t ntiles, int64_t& t0, int64_t&
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// Streaming store of 16 bytes of a float32 matrix: the nontemporal hint
+// keeps the written lines from displacing the level stream, tables and
+// postings the kernels read from the caches.
+__device__ __forceinline__ void store_stream(float* p, float4 v) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f4v*>(p));
+}
+
 // Element i of an array no kernel writes while it runs, read through the
 // constant address space: with a wave-uniform index the load is a scalar
 // load (scalar cache, scalar registers) instead of a vector load.
@@ -1600,7 +1609,7 @@ __global__ __launch_bounds__(block_waves<NB>() * kWave, NB == 8 ? 6 : 4) void k_
                 // wait also waits for the previous rows' stores
                 int l;
                 asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
-                reinterpret_cast<float4*>(orow)[l] = reinterpret_cast<const float4*>(stage)[l];
+                store_stream(orow + 4 * l, reinterpret_cast<const float4*>(stage)[l]);
             } else {
                 for (int i = 0; i < 4 && o4 + i < rem; ++i) orow[o4 + i] = stage[o4 + i];
             }
@@ -1986,7 +1995,7 @@ __global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const in
             v.z = pair_tanimoto<FAST>((float)((wd >> 16) & 0xFFu), maf, (float)mb.z, ma == 0 || mb.z == 0);
             v.w = pair_tanimoto<FAST>((float)(wd >> 24), maf, (float)mb.w, ma == 0 || mb.w == 0);
             if (4 * j + 4 <= cw) {
-                reinterpret_cast<float4*>(orow)[j] = v;
+                store_stream(orow + 4 * j, v);
             } else {
                 const float vv[4] = {v.x, v.y, v.z, v.w};
                 for (int k = 0; k < cw - 4 * j; ++k) orow[4 * j + k] = vv[k];
@@ -3973,11 +3982,14 @@ int ensure_postings(siml_ctx* ctx) {
 }
 
 // Cost model of AUTO's choice (float32 matrix), in picoseconds of one
-// MI355X, measured on the 32768 x 32768 synthetic matrix: the sparse kernel
-// takes about 0.78 per pair (writing the matrix) plus 2.8 per posting its
-// rows' levels meet; the bit-sliced kernel about 0.0226 per pair and query
-// level (1.51 per pair at 67 levels per query molecule).
-constexpr double kSparsePsPerPair = 0.78, kSparsePsPerPosting = 2.8, kBitsPsPerPairLevel = 0.0226;
+// MI355X, measured on 32768 x 32768 matrices of synthetic strings (67
+// levels per molecule, 0.2 postings met per pair) and of the reference's
+// ZINC molecules cycled (about 30 levels, 3.1 postings per pair): the
+// sparse kernel takes 0.69 per pair (writing the matrix) plus 0.77 per
+// posting its rows' levels meet; the bit-sliced kernel 0.72 per pair plus
+// 0.0112 per pair and query level.
+constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.77;
+constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
 
 // Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
 // levels) and the rest (merge kernel), when the sparse path applies to the
@@ -4012,7 +4024,7 @@ bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, const BitPlan& bp, std::
     const double pairs = (double)(rl - rb) * (double)ctx->qry->n;
     const double qlevels = (double)X.total / (double)ctx->qry->n;
     return kSparsePsPerPair * pairs + kSparsePsPerPosting * levels * X.per_level <
-           kBitsPsPerPairLevel * qlevels * pairs;
+           (kBitsPsPerPair + kBitsPsPerPairLevel * qlevels) * pairs;
 }
 
 static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {

@@ -145,6 +145,7 @@ def test_fast_division_every_union_up_to_2000():
     sizes = list(range(0, 2001))
     s = prefix_sets(sizes)
     g = GPULingo(0)
+    g._h.call("siml_set_policy", _lib.POLICY_BITS)
     g.set_refsmiles(*s)
     g.set_qsmiles(*s)
     got = g.getMultipleRows(0, len(sizes))
@@ -163,3 +164,23 @@ def test_fast_division_every_union_up_to_2000():
     # the same through the 1000-bin histogram (bins of the short quotient)
     h = g.getGlobalHistogram(0, len(sizes))
     assert np.array_equal(h, oracle.hist1000(s, s))
+
+
+def test_fast_division_sparse_kernel():
+    """The same prefix molecules of 0..255 levels (every row within the
+    sparse kernel's byte counts) on the sparse path: every (I, U) with
+    1 <= I <= U <= 255 against numpy's float32 division."""
+    sizes = list(range(0, 256))
+    s = prefix_sets(sizes)
+    g = GPULingo(0)
+    g._h.call("siml_set_policy", _lib.POLICY_SPARSE)
+    g.set_refsmiles(*s)
+    g.set_qsmiles(*s)
+    got = g.getMultipleRows(0, len(sizes))
+    assert g._h.last_kernel() == "rows_sparse"
+    a = np.array(sizes, np.int64)
+    lo = np.minimum(a[:, None], a[None, :]).astype(np.float32)
+    hi = np.maximum(a[:, None], a[None, :]).astype(np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        want = np.where((a[:, None] == 0) | (a[None, :] == 0), np.float32(0), lo / hi).astype(np.float32)
+    assert np.array_equal(bits(got), bits(want))
