@@ -101,6 +101,19 @@ def recorded_counters(name):
     return None
 
 
+def postings_met_self(s):
+    """Postings the sparse path's rows meet when a padded set is compared
+    with itself: sum over level keys (Lingo, t <= 15) of the number of
+    molecules holding the level, squared."""
+    lingos, counts, lengths = s[0], s[1], s[2].ravel()
+    mask = np.arange(lingos.shape[1])[None, :] < lengths[:, None]
+    ids = lingos[mask].astype(np.int64)
+    cts = np.minimum(counts[mask].astype(np.int64), 15)
+    keys = np.repeat(ids << 4, cts) + (np.arange(int(cts.sum())) - np.repeat(np.cumsum(cts) - cts, cts) + 1)
+    _, n_k = np.unique(keys, return_counts=True)
+    return int((n_k.astype(np.int64) ** 2).sum())
+
+
 def levels_padded(lingos, counts, lengths):
     """Levels per molecule of a padded set: sum over its ASCII Lingos of
     min(count, 15), the column levels the bit-sliced kernels probe."""
@@ -220,17 +233,19 @@ def roofline(alg_bytes, probe_bytes, kernel_s, peaks, counters=None):
         else:
             note = f"{counters['file']} was measured on sources {counters.get('sources_sha')}, not {sha}"
     achieved = alg_bytes / kernel_s
-    lds = probe_bytes / kernel_s
     rec = {"bound": "hbm", "achieved": achieved / 1e9, "peak": B_HBM / 1e9, "unit": "GB/s",
            "frac": achieved / B_HBM, "traffic": traffic,
            "alg_bytes_per_launch": alg_bytes, "kernel_ms": kernel_s * 1e3,
            "hbm_measured_frac": (traffic / kernel_s / B_HBM) if traffic else None,
-           "lds": {"achieved": lds / 1e9, "unit": "GB/s", "probe_bytes_per_launch": probe_bytes,
-                   "peak_conflict_free": peaks["lds_conflict_free"] / 1e9,
-                   "frac": lds / peaks["lds_conflict_free"],
-                   "peak_random_slots": peaks["lds_random_slots"] / 1e9,
-                   "frac_of_random_slot_rate": lds / peaks["lds_random_slots"]},
-           "sources_sha": sha}
+           "lds": {}, "sources_sha": sha}
+    if probe_bytes is not None:
+        # table probes of the bit-sliced kernels
+        lds = probe_bytes / kernel_s
+        rec["lds"] = {"achieved": lds / 1e9, "unit": "GB/s", "probe_bytes_per_launch": probe_bytes,
+                      "peak_conflict_free": peaks["lds_conflict_free"] / 1e9,
+                      "frac": lds / peaks["lds_conflict_free"],
+                      "peak_random_slots": peaks["lds_random_slots"] / 1e9,
+                      "frac_of_random_slot_rate": lds / peaks["lds_random_slots"]}
     if counters is not None and counters.get("sources_sha") == sha:
         rec["counters"] = {k: v for k, v in counters.items() if k not in ("fetch_kib", "write_kib")}
         c = counters
@@ -292,19 +307,33 @@ def workload_allpairs(args):
     kernel_s = kms / 1e3 / max(launches, 1)
     lv = levels_padded(s[0], s[1], s[2])
     groups, merged = h.row_groups()
-    # algorithmic HBM bytes: the float32 matrix written once, the column
-    # level stream read once (8 B per level), the row tables (16 B per table
-    # entry: at most one per row level) and 8 B of per-column vectors
-    alg_bytes = 4.0 * n * n + 8.0 * lv.sum() + 16.0 * lv.sum() + 8.0 * n
-    probe_bytes = 16.0 * groups * lv.sum()
     peaks = device_peaks()
-    counters = recorded_counters("k_bits_f32block_allpairs32768") if (
-        n == N_ALLPAIRS and args.policy == "auto" and args.layout == "block") else None
+    if kernel.startswith("rows_sparse"):
+        # sparse path: the float32 matrix written once, the postings read
+        # once (4 B per query level), the rows' elements (8 B per Lingo) and
+        # 4 B of column magnitudes; the rows' levels meet sum_k n_k^2
+        # postings (n_k: query molecules holding level key k; rows ==
+        # columns here), each an LDS add
+        alg_bytes = 4.0 * n * n + 4.0 * lv.sum() + 8.0 * int(s[2].sum()) + 4.0 * n
+        probe_bytes = None
+        postings_met = postings_met_self(s)
+        profile = "k_sparse_f32_allpairs32768"
+    else:
+        # bit-sliced: the float32 matrix written once, the column level
+        # stream read once (8 B per level), the row tables (16 B per table
+        # entry: at most one per row level) and 8 B of per-column vectors
+        alg_bytes = 4.0 * n * n + 8.0 * lv.sum() + 16.0 * lv.sum() + 8.0 * n
+        probe_bytes = 16.0 * groups * lv.sum()
+        postings_met = None
+        profile = "k_bits_f32block_allpairs32768"
+    counters = recorded_counters(profile) if (n == N_ALLPAIRS and args.layout == "block") else None
     rec = base_record(args, "Tanimoto pairs/s (1 GPU, bit-exact)", value, "pairs/s", wall,
                       {"workload": f"all-pairs {n} x {n}, strings U[20,120], full float32 matrix in HBM",
                        "rows": n, "columns": n, "seed": 2, "policy": args.policy, "layout": args.layout})
     rec.update({"kernel": kernel, "kernel_ms_mean": kernel_s * 1e3, "row_groups": groups, "merge_rows": merged,
                 "roofline": roofline(alg_bytes, probe_bytes, kernel_s, peaks, counters)})
+    if postings_met is not None:
+        rec["postings_met_per_launch"] = postings_met
     if not args.no_cpu:
         import oracle
         rows = [0, 1, n // 3, n // 2, n - 2, n - 1]

@@ -1825,6 +1825,50 @@ __device__ __forceinline__ int64_t post_find(const PostingArgs& P, uint32_t key)
     return -1;
 }
 
+// The index as the sparse kernel reads it: one 16-byte slot per key (key,
+// posting count, first posting), so a probe is one load, and the postings
+// as uint16 when the query set has at most 65536 molecules (int32 else).
+struct PostingQuery {
+    const uint4* slot;        // (key, count, first posting low, high); key 0 = empty
+    const int32_t* col;       // postings (int32), when col16 is null
+    const uint16_t* col16;    // postings (uint16), or null
+    uint32_t mask;            // slots - 1
+};
+
+// Posting range of `key`: true with (first, count) when the index holds it.
+__device__ __forceinline__ bool post_lookup(const PostingQuery& Q, uint32_t key, int64_t& first, int32_t& n) {
+    uint32_t s = post_home(key, Q.mask);
+    for (uint32_t i = 0; i <= Q.mask; ++i) {
+        const uint4 v = Q.slot[s];
+        if (v.x == key) {
+            first = (int64_t)((uint64_t)v.z | ((uint64_t)v.w << 32));
+            n = (int32_t)v.y;
+            return true;
+        }
+        if (v.x == 0u) return false;
+        s = (s + 1) & Q.mask;
+    }
+    return false;
+}
+
+// Index build, last pass: the build's separate key / count / offset arrays
+// packed into 16-byte slots.
+__global__ __launch_bounds__(kBlock) void k_post_pack(const uint32_t* __restrict__ key, const int32_t* __restrict__ cnt,
+                                                      const int64_t* __restrict__ off, int64_t slots,
+                                                      uint4* __restrict__ slot) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t o = (uint64_t)off[i];
+        slot[i] = make_uint4(key[i], (uint32_t)cnt[i], (uint32_t)o, (uint32_t)(o >> 32));
+    }
+}
+
+// Postings narrowed to uint16 (caller columns < 65536).
+__global__ __launch_bounds__(kBlock) void k_post_narrow(const int32_t* __restrict__ col, int64_t n,
+                                                        uint16_t* __restrict__ col16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        col16[i] = (uint16_t)col[i];
+}
+
 // Visits every level of the level stream: f(key, storage position).  One
 // wave per tile, lane l holds the levels of the tile's molecule l; padding
 // levels (key 0) are skipped.
@@ -1919,10 +1963,19 @@ constexpr int kSparseMaxLevels = 255;
 // so a level with long postings is spread over a wave.
 constexpr int kSparseBlock = 1024;  // 16 waves: posting reads in flight to hide their latency
 
-template <bool FAST>
-__global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const int32_t* __restrict__ rowlist, int64_t nlist,
-                                                       PostingArgs P, const int32_t* __restrict__ cmag, int64_t nq,
-                                                       int64_t rb, float* __restrict__ out, int64_t pitch) {
+template <bool NARROW>
+__device__ __forceinline__ int64_t post_col(const PostingQuery& P, int64_t i) {
+    if constexpr (NARROW)
+        return (int64_t)P.col16[i];
+    else
+        return (int64_t)P.col[i];
+}
+
+template <bool FAST, bool NARROW>
+__global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const int32_t* __restrict__ rowlist,
+                                                             int64_t nlist, PostingQuery P,
+                                                             const int32_t* __restrict__ cmag, int64_t nq, int64_t rb,
+                                                             float* __restrict__ out, int64_t pitch) {
     __shared__ uint32_t cnt[kSparseRows][kSparseCols / 4];
     __shared__ int64_t lv_start[kSparseRows * kSparseMaxLevels];
     __shared__ int32_t lv_len[kSparseRows * kSparseMaxLevels];
@@ -1947,11 +2000,13 @@ __global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const in
         for (int e = threadIdx.x; e < len; e += blockDim.x) {
             const int2 el = R.elems[o + e];
             for (int t = 1; t <= el.y; ++t) {
-                const int64_t s = post_find(P, dev_level_key(el.x, (uint32_t)t));
-                if (s < 0) break;  // no query molecule has level t, so none has t + 1
+                int64_t first;
+                int32_t cnt_t;
+                // no query molecule has level t, so none has t + 1
+                if (!post_lookup(P, dev_level_key(el.x, (uint32_t)t), first, cnt_t)) break;
                 const int at = atomicAdd(&nlv, 1);
-                lv_start[at] = P.off[s];
-                lv_len[at] = P.cnt[s] | (i << 28);
+                lv_start[at] = first;
+                lv_len[at] = cnt_t | (i << 28);
             }
         }
     }
@@ -1969,13 +2024,13 @@ __global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const in
         for (; q + 3 * kWave < ln; q += 4 * kWave) {
             int64_t c[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = (int64_t)P.col[st + q + u * kWave] - c0;
+            for (int u = 0; u < 4; ++u) c[u] = post_col<NARROW>(P, st + q + u * kWave) - c0;
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (c[u] >= 0 && c[u] < cw) atomicAdd(crow + (c[u] >> 2), 1u << (8 * (c[u] & 3)));
         }
         for (; q < ln; q += kWave) {
-            const int64_t c = (int64_t)P.col[st + q] - c0;
+            const int64_t c = post_col<NARROW>(P, st + q) - c0;
             if (c >= 0 && c < cw) atomicAdd(crow + (c >> 2), 1u << (8 * (c & 3)));
         }
     }
@@ -2468,16 +2523,19 @@ struct ColSet {
     // matrix), built on first use by any handle using the set
     struct Postings {
         bool built = false, ok = false;
+        bool narrow = false;     // postings stored as uint16
         int64_t slots = 0, total = 0;
         double per_level = 0.0;  // expected postings of a stream level
-        DevBuf key, cnt, off, col, cmag;
+        DevBuf slot, col, cmag;
         void release() {
-            key.release();
-            cnt.release();
-            off.release();
+            slot.release();
             col.release();
             cmag.release();
-            built = ok = false;
+            built = ok = narrow = false;
+        }
+        PostingQuery query() const {
+            return {slot.as<uint4>(), narrow ? nullptr : col.as<int32_t>(), narrow ? col.as<uint16_t>() : nullptr,
+                    (uint32_t)(slots - 1)};
         }
     } post;
     std::mutex post_mu;
@@ -3928,10 +3986,10 @@ int ensure_postings(siml_ctx* ctx) {
     int64_t slots = 1024;
     while (slots < 2 * C.btiles_total && slots < kPostMaxSlots) slots <<= 1;
     X.slots = slots;
-    TRY(X.key.reserve(sizeof(uint32_t) * (size_t)slots));
-    TRY(X.cnt.reserve(sizeof(int32_t) * (size_t)slots));
-    TRY(X.off.reserve(sizeof(int64_t) * (size_t)slots));
-    DevBuf cur, tmp;
+    DevBuf key, cnt, off, col32, cur, tmp;
+    TRY(key.reserve(sizeof(uint32_t) * (size_t)slots));
+    TRY(cnt.reserve(sizeof(int32_t) * (size_t)slots));
+    TRY(off.reserve(sizeof(int64_t) * (size_t)slots));
     TRY(cur.reserve(sizeof(int32_t) * (size_t)slots));
     const size_t scan_bytes = siml::csr_scan_bytes(slots);
     TRY(tmp.reserve(scan_bytes + 256));
@@ -3941,19 +3999,19 @@ int ensure_postings(siml_ctx* ctx) {
     int32_t* flag = reinterpret_cast<int32_t*>(tail + 2);
     unsigned long long* stats = reinterpret_cast<unsigned long long*>(tail + 4);
     const unsigned fb = (unsigned)std::min<int64_t>((slots + 255) / 256, 65535);
-    k_fill<uint32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(X.key.as<uint32_t>(), slots, 0u);
-    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(X.cnt.as<int32_t>(), slots, 0);
+    k_fill<uint32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(key.as<uint32_t>(), slots, 0u);
+    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cnt.as<int32_t>(), slots, 0);
     k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cur.as<int32_t>(), slots, 0);
     k_fill<int64_t><<<dim3(1), dim3(64), 0, ctx->stream>>>(tail, 8, 0);
     TRY(launch_check("k_fill<postings>"));
-    PostingArgs P{X.key.as<uint32_t>(), X.cnt.as<int32_t>(), X.off.as<int64_t>(), nullptr, cur.as<int32_t>(),
+    PostingArgs P{key.as<uint32_t>(), cnt.as<int32_t>(), off.as<int64_t>(), nullptr, cur.as<int32_t>(),
                   (uint32_t)(slots - 1), flag};
     const ColArgs CA = C.args();
     const unsigned wb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((C.ntiles + 3) / 4, 16384));
     k_post_insert<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
     TRY(launch_check("k_post_insert"));
-    TRY(siml::csr_scan(X.cnt.as<int32_t>(), 1, slots, X.off.as<int64_t>(), tail, tmp.p, ctx->stream));
-    k_post_stats<<<dim3(std::min<unsigned>(fb, 2048)), dim3(256), 0, ctx->stream>>>(X.cnt.as<int32_t>(), slots, stats);
+    TRY(siml::csr_scan(cnt.as<int32_t>(), 1, slots, off.as<int64_t>(), tail, tmp.p, ctx->stream));
+    k_post_stats<<<dim3(std::min<unsigned>(fb, 2048)), dim3(256), 0, ctx->stream>>>(cnt.as<int32_t>(), slots, stats);
     TRY(launch_check("k_post_stats"));
     int64_t host[8] = {0};
     HIP_TRY(hipMemcpyAsync(host, tail, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
@@ -3966,11 +4024,24 @@ int ensure_postings(siml_ctx* ctx) {
         X.built = true;
         return SIML_OK;
     }
-    TRY(X.col.reserve(sizeof(int32_t) * (size_t)total));
+    TRY(col32.reserve(sizeof(int32_t) * (size_t)total));
     TRY(X.cmag.reserve(sizeof(int32_t) * (size_t)(C.n + 4)));
-    P.col = X.col.as<int32_t>();
+    TRY(X.slot.reserve(sizeof(uint4) * (size_t)slots));
+    P.col = col32.as<int32_t>();
     k_post_fill<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
     TRY(launch_check("k_post_fill"));
+    k_post_pack<<<dim3(fb), dim3(kBlock), 0, ctx->stream>>>(key.as<uint32_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
+                                                          slots, X.slot.as<uint4>());
+    TRY(launch_check("k_post_pack"));
+    X.narrow = C.n <= 65536;
+    if (X.narrow) {
+        TRY(X.col.reserve(sizeof(uint16_t) * (size_t)total));
+        const unsigned nb = (unsigned)std::min<int64_t>((total + 255) / 256, 65535);
+        k_post_narrow<<<dim3(nb), dim3(kBlock), 0, ctx->stream>>>(col32.as<int32_t>(), total, X.col.as<uint16_t>());
+        TRY(launch_check("k_post_narrow"));
+    } else {
+        std::swap(X.col, col32);
+    }
     k_caller_mags<<<dim3((unsigned)std::min<int64_t>((C.n + 4 + 255) / 256, 65535)), dim3(256), 0, ctx->stream>>>(
         CA, X.cmag.as<int32_t>());
     TRY(launch_check("k_caller_mags"));
@@ -4055,21 +4126,25 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
         both.insert(both.end(), mrows.begin(), mrows.end());
         if (!both.empty()) TRY(upload_rowlist(ctx, both));
         const ColSet::Postings& X = ctx->qry->post;
-        const PostingArgs P{X.key.as<uint32_t>(), X.cnt.as<int32_t>(), X.off.as<int64_t>(), X.col.as<int32_t>(),
-                            nullptr, (uint32_t)(X.slots - 1), nullptr};
+        const PostingQuery P = X.query();
         const bool fast = ctx->ref.mags_small && ctx->qry->mags_small;
         TimedRegion tr(ctx, nr * nq);
         if (!srows.empty()) {
             const dim3 grid((unsigned)((srows.size() + kSparseRows - 1) / kSparseRows),
                             (unsigned)((nq + kSparseCols - 1) / kSparseCols));
-            if (fast)
-                k_sparse_f32<true><<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(
-                    ctx->ref.args(), ctx->rowlist.as<int32_t>(), (int64_t)srows.size(), P, X.cmag.as<int32_t>(), nq,
-                    rowbase, ctx->res.as<float>(), dp);
+            auto go = [&](auto kern) {
+                kern<<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(ctx->ref.args(), ctx->rowlist.as<int32_t>(),
+                                                                    (int64_t)srows.size(), P, X.cmag.as<int32_t>(),
+                                                                    nq, rowbase, ctx->res.as<float>(), dp);
+            };
+            if (fast && X.narrow)
+                go(k_sparse_f32<true, true>);
+            else if (fast)
+                go(k_sparse_f32<true, false>);
+            else if (X.narrow)
+                go(k_sparse_f32<false, true>);
             else
-                k_sparse_f32<false><<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(
-                    ctx->ref.args(), ctx->rowlist.as<int32_t>(), (int64_t)srows.size(), P, X.cmag.as<int32_t>(), nq,
-                    rowbase, ctx->res.as<float>(), dp);
+                go(k_sparse_f32<false, false>);
             TRY(launch_check("k_sparse_f32"));
         }
         if (!mrows.empty()) {
