@@ -1801,6 +1801,7 @@ __device__ __forceinline__ uint32_t post_home(uint32_t key, uint32_t mask) {
     return h & mask;
 }
 
+// The index while it is built: separate key, count and offset arrays.
 struct PostingArgs {
     uint32_t* key;        // slot keys, 0 = empty
     int32_t* cnt;         // postings per slot
@@ -1812,8 +1813,9 @@ struct PostingArgs {
 };
 
 // Slot of `key`, or -1 when the table holds no such key.  Linear probing
-// from the home slot ends at the key or at an empty slot; the table is at
-// most half full, so an empty slot exists.
+// from the home slot ends at the key or at an empty slot (the table has at
+// least twice as many slots as the stream has entries, unless capped; a
+// full table stops after one round).
 __device__ __forceinline__ int64_t post_find(const PostingArgs& P, uint32_t key) {
     uint32_t s = post_home(key, P.mask);
     for (uint32_t i = 0; i <= P.mask; ++i) {
