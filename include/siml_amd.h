@@ -280,10 +280,9 @@ int siml_set_autotune(siml_ctx* ctx, int on);
  * capture and launched as one HIP graph; host uploads of the call's row
  * lists stay outside the graph.  Instantiated graphs are cached per handle
  * by their kernel sequence and updated in place with the next call's
- * arguments.  When the handle's stream is already being captured by the
- * caller, the launches go to the caller's graph instead.  Results are
- * identical; off by default.  siml_graph_stats reports the graph launches
- * and the instantiations (cache misses) since the handle was created. */
+ * arguments.  Results are identical; off by default.  siml_graph_stats
+ * reports the graph launches and the instantiations (cache misses) since
+ * the handle was created. */
 int siml_set_graphs(siml_ctx* ctx, int on);
 /* Persistent grid (no counterpart in the reference, whose kernels launch
  * one block per row, kernels.py:315-352): when on, the bit-sliced similarity

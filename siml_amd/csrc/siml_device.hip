@@ -2686,12 +2686,11 @@ struct TimedRegion {
     }
 };
 
-// Records the kernel launches issued between begin() and launch() on the
-// handle's stream as a HIP graph and launches it, when graph mode is on and
-// the stream is not already being captured by the caller (then the launches
-// go to the caller's graph).  Only kernels may be issued in between: host
-// copies and memsets stay outside, before begin().  A region left without
-// launch() (an error return) ends the capture and drops the graph.
+// Records the kernel launches issued between begin() and end() on the
+// handle's stream as a HIP graph, when graph mode is on; launch() then
+// launches it.  Only kernels may be issued in between: host copies and
+// memsets stay outside, before begin().  A region left without end() (an
+// error return) ends the capture and drops the graph.
 struct GraphRegion {
     siml_ctx* ctx;
     bool on = false;                          // capturing
@@ -2699,9 +2698,6 @@ struct GraphRegion {
     explicit GraphRegion(siml_ctx* c) : ctx(c) {}
     int begin() {
         if (!ctx->graphs) return SIML_OK;
-        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-        HIP_TRY(hipStreamIsCapturing(ctx->stream, &st));
-        if (st != hipStreamCaptureStatusNone) return SIML_OK;
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         on = true;
         return SIML_OK;
