@@ -6,7 +6,8 @@ strings (where AUTO takes the sparse path), with partial column chunks and
 float4 groups, row ranges that split the 2-row workgroups, rows the sparse
 kernel leaves to the merge kernel, magnitudes other than the count sums,
 every column layout, ragged sets and a resident database shared by two
-handles."""
+handles, query sets too wide for uint16 postings, and query sets of one to
+five molecules."""
 import numpy as np
 import pytest
 
@@ -144,3 +145,30 @@ def test_ragged_sets_and_shared_database():
     assert_same_f32(hs[1].getMultipleRows(17, 150), want[17:])
     assert hs[1]._h.last_kernel() in ("rows_sparse", "rows_bits")
     db.close()
+
+
+def test_wide_query_set_int32_postings():
+    """70000 query molecules (postings kept as int32, three column chunks)
+    with IEEE-division magnitudes on some rows and the short division on
+    the rest of the calls."""
+    ref = list(synth.packed_padded(40, 10, 150, seed=71))
+    qry = synth.packed_padded(70000, 10, 150, seed=72)
+    want = oracle.tanimoto_block(tuple(ref), qry, nthreads=0)
+    g = handle(_lib.POLICY_SPARSE, tuple(ref), qry)
+    assert_same_f32(g.getMultipleRows(0, 40), want)
+    assert g._h.last_kernel() == "rows_sparse"
+    ref[3] = ref[2].copy()                                          # magnitudes != count sums
+    want2 = oracle.tanimoto_block(tuple(ref), qry, nthreads=0)
+    g2 = handle(_lib.POLICY_SPARSE, tuple(ref), qry)
+    assert_same_f32(g2.getMultipleRows(7, 40), want2[7:])
+
+
+@pytest.mark.parametrize("nq", [1, 3, 5])
+def test_tiny_query_sets(nq):
+    """One to five query molecules: a single partial float4 per row."""
+    ref = synth.packed_padded(9, 10, 60, seed=81)
+    qry = synth.packed_padded(nq, 10, 60, seed=82)
+    want = oracle.tanimoto_block(ref, qry, nthreads=0)
+    g = handle(_lib.POLICY_SPARSE, ref, qry)
+    assert_same_f32(g.getMultipleRows(0, 9), want)
+    assert_same_f32(g.getMultipleRows(4, 5), want[4:5])
