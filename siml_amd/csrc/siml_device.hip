@@ -1949,8 +1949,12 @@ __global__ __launch_bounds__(kBlock) void k_caller_mags(ColArgs C, int32_t* __re
 }
 
 // Reference rows per workgroup and query molecules per chunk of the sparse
-// kernel: one byte per pair in LDS, kSparseRows x kSparseCols = 64 KiB.
-constexpr int kSparseRows = 2;
+// kernel: one byte per pair in LDS, kSparseRows x kSparseCols = 32 KiB, so
+// that four 8-wave workgroups share a CU.  Measured on the 32768^2 matrix:
+// 1 row and 512 threads 0.80 ms; 2 rows (64 KiB) and 1024 threads 0.88, 512
+// threads 1.10; 4 rows of 16384 columns 1.21; 1 row and 1024 threads 0.94,
+// 256 threads 1.01.
+constexpr int kSparseRows = 1;
 constexpr int kSparseCols = 32768;
 // most levels of a listed row (a byte holds every count)
 constexpr int kSparseMaxLevels = 255;
@@ -1963,7 +1967,7 @@ constexpr int kSparseMaxLevels = 255;
 // lists the rows' levels with their posting ranges in LDS; each wave then
 // walks whole levels, its lanes reading 64 consecutive postings at a time,
 // so a level with long postings is spread over a wave.
-constexpr int kSparseBlock = 1024;  // 16 waves: posting reads in flight to hide their latency
+constexpr int kSparseBlock = 512;  // 8 waves per workgroup, 32 per CU: posting reads in flight
 
 template <bool NARROW>
 __device__ __forceinline__ int64_t post_col(const PostingQuery& P, int64_t i) {
@@ -4053,11 +4057,11 @@ int ensure_postings(siml_ctx* ctx) {
 // Cost model of AUTO's choice (float32 matrix), in picoseconds of one
 // MI355X, measured on 32768 x 32768 matrices of synthetic strings (67
 // levels per molecule, 0.2 postings met per pair) and of the reference's
-// ZINC molecules cycled (about 30 levels, 3.1 postings per pair): the
-// sparse kernel takes 0.69 per pair (writing the matrix) plus 0.77 per
-// posting its rows' levels meet; the bit-sliced kernel 0.72 per pair plus
-// 0.0112 per pair and query level.
-constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.77;
+// ZINC molecules cycled (about 28 levels, 6.25 postings per pair): the
+// sparse kernel takes 0.69 per pair (writing the matrix) plus 0.30 per
+// posting its rows' levels meet (0.80 and 2.73 ms); the bit-sliced kernel
+// 0.72 per pair plus 0.0112 per pair and query level (1.57 and 1.11 ms).
+constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.30;
 constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
 
 // Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255

@@ -3,8 +3,8 @@ set's levels, per-pair counts in LDS): bit-identical to the oracle and to
 the bit-sliced kernel under every policy, on real SMILES (where AUTO keeps
 the bit-sliced kernel: common Lingos make long postings) and on synthetic
 strings (where AUTO takes the sparse path), with partial column chunks and
-float4 groups, row ranges that split the 2-row workgroups, rows the sparse
-kernel leaves to the merge kernel, magnitudes other than the count sums,
+float4 groups, odd row ranges, rows the sparse kernel leaves to the merge
+kernel, magnitudes other than the count sums,
 every column layout, ragged sets and a resident database shared by two
 handles, query sets too wide for uint16 postings, and query sets of one to
 five molecules."""
@@ -76,8 +76,8 @@ def test_synthetic_auto_takes_sparse_path():
 @pytest.mark.parametrize("layout", [_lib.LAYOUT_NATURAL, _lib.LAYOUT_BLOCK, _lib.LAYOUT_SORTED])
 def test_partial_chunks_and_row_ranges(layout):
     """32774 query molecules: two column chunks of the sparse kernel, the
-    second of 6 columns (a partial float4); row ranges that start and end
-    inside 2-row workgroups."""
+    second of 6 columns (a partial float4); row ranges of one to 300 rows
+    that start at odd and even rows."""
     ref = synth.packed_padded(300, 10, 120, seed=31)
     qry = synth.packed_padded(32774, 10, 120, seed=32)
     g = handle(_lib.POLICY_SPARSE, ref, qry, layout)
