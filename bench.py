@@ -114,6 +114,27 @@ def postings_met_self(s):
     return int((n_k.astype(np.int64) ** 2).sum())
 
 
+def level_keys_ragged(ids, counts, offsets):
+    """Level keys (Lingo << 4 | t for t <= min(count, 15)) of a ragged set,
+    ASCII Lingos only, and the molecule of each."""
+    n = len(offsets) - 1
+    ids, counts = ids[:offsets[-1]].astype(np.int64), counts[:offsets[-1]].astype(np.int64)
+    cts = np.where((ids & 0x80808080) == 0, np.clip(counts, 0, MAX_LEVEL), 0)
+    keys = np.repeat(ids << 4, cts) + (np.arange(int(cts.sum())) - np.repeat(np.cumsum(cts) - cts, cts) + 1)
+    mol = np.repeat(np.repeat(np.arange(n), np.diff(offsets)), cts)
+    return keys, mol
+
+
+def postings_met_ragged(q, db):
+    """Postings the sparse top-k's rows meet: over the query rows' level
+    keys, the number of database molecules holding the key."""
+    dk, _ = level_keys_ragged(*db[:3])
+    uk, nk = np.unique(dk, return_counts=True)
+    qk, _ = level_keys_ragged(*q[:3])
+    pos = np.minimum(np.searchsorted(uk, qk), len(uk) - 1)
+    return int(nk[pos][uk[pos] == qk].sum())
+
+
 def levels_padded(lingos, counts, lengths):
     """Levels per molecule of a padded set: sum over its ASCII Lingos of
     min(count, 15), the column levels the bit-sliced kernels probe."""
@@ -571,14 +592,23 @@ def batched_part(args, check_oracle=False, peaks=None):
     if peaks is not None:
         lv_db = levels_ragged(*db_set[:3])
         lv_q = levels_ragged(*q[:3])
-        # algorithmic HBM bytes of the batched similarity kernel: the database
-        # level stream once (8 B per level), the query tables (16 B per
-        # entry) and 8 B per candidate key for k keys per query row
-        alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8.0 * k * nq
         out["row_groups"] = groups
         out["merge_rows"] = merged
-        out["roofline"] = roofline(alg, 16.0 * groups * lv_db.sum(), sim_b / 1e3, peaks,
-                                   recorded_counters("k_bits_topk_batched1M"))
+        if kernel == "topk_sparse":
+            # sparse top-k: the postings the query rows' levels meet (4 B
+            # each), a 16-byte index slot per query level and the top-k of
+            # each row (8 B per entry) written once
+            met = postings_met_ragged(q, db_set)
+            alg = 4.0 * met + 16.0 * lv_q.sum() + 8.0 * k * nq
+            out["postings_met_per_launch"] = met
+            out["roofline"] = roofline(alg, None, kms_b / 1e3, peaks, recorded_counters("k_sparse_topk_batched1M"))
+        else:
+            # algorithmic HBM bytes of the batched similarity kernel: the
+            # database level stream once (8 B per level), the query tables
+            # (16 B per entry) and 8 B per candidate key for k keys per row
+            alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8.0 * k * nq
+            out["roofline"] = roofline(alg, 16.0 * groups * lv_db.sum(), sim_b / 1e3, peaks,
+                                       recorded_counters("k_bits_topk_batched1M"))
     if check_oracle:
         rng = np.random.default_rng(7)
         rows = np.sort(rng.choice(nq, size=min(6, nq), replace=False))

@@ -1920,21 +1920,57 @@ __global__ __launch_bounds__(kBlock) void k_post_fill(ColArgs C, PostingArgs P) 
     });
 }
 
-// Sum over slots of count^2 (stats[0]) and of count (stats[1]): the
-// expected postings of a stream level drawn at random is stats[0] / stats[1].
+// Sum over slots of count^2 (stats[0]), of count (stats[1]) and the
+// occupied slots (stats[2]): the expected postings of a stream level drawn
+// at random is stats[0] / stats[1].
 __global__ __launch_bounds__(kBlock) void k_post_stats(const int32_t* __restrict__ cnt, int64_t slots,
                                                        unsigned long long* stats) {
-    unsigned long long sq = 0, sum = 0;
+    unsigned long long sq = 0, sum = 0, used = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (int64_t)gridDim.x * blockDim.x) {
         const unsigned long long c = (unsigned long long)cnt[i];
         sq += c * c;
         sum += c;
+        used += c > 0 ? 1ull : 0ull;
     }
     sq = wave_sum_u64(sq);
     sum = wave_sum_u64(sum);
+    used = wave_sum_u64(used);
     if (lane_id() == 0) {
         atomicAdd(stats, sq);
         atomicAdd(stats + 1, sum);
+        atomicAdd(stats + 2, used);
+    }
+}
+
+// Storage position of every caller column (the inverse of the layout's
+// permutation).
+__global__ __launch_bounds__(kBlock) void k_inv_perm(const int32_t* __restrict__ perm, int64_t n,
+                                                     int32_t* __restrict__ inv) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        inv[perm[p]] = (int32_t)p;
+}
+
+// Index build, pass 2 for the caller columns [c0, c1) only: a thread per
+// column writes the column into the postings of its levels.  Run once per
+// column chunk, in chunk order, it leaves every key's postings grouped by
+// chunk (chunk 0's columns first, in no particular order within a chunk),
+// so a walk over the chunks in order reads each posting list front to back.
+__global__ __launch_bounds__(kBlock) void k_post_fill_cols(ColArgs C, PostingArgs P, const int32_t* __restrict__ inv,
+                                                           int64_t c0, int64_t c1) {
+    const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c1) return;
+    const int64_t p = inv ? (int64_t)inv[c] : c;
+    const int64_t t = p / kWave;
+    const int lane = (int)(p % kWave);
+    const int64_t base = C.btoff[t];
+    const int32_t lt = C.btlen[t];
+    for (int32_t j = 0; j < lt; ++j) {
+        const int2 e = C.btiles[base + (int64_t)(j >> 1) * 2 * kWave + 2 * lane + (j & 1)];
+        if (e.x == 0) continue;
+        const int64_t s = post_find(P, (uint32_t)e.x);
+        if (s < 0) continue;
+        const int32_t at = atomicAdd(P.cur + s, 1);
+        P.col[P.off[s] + at] = (int32_t)c;
     }
 }
 
@@ -1946,6 +1982,26 @@ __global__ __launch_bounds__(kBlock) void k_caller_mags(ColArgs C, int32_t* __re
             cmag[col_of(C, p)] = C.mag[p];
         else
             cmag[p] = 0;
+}
+
+// The caller-order magnitudes narrowed to one byte (valid when every
+// magnitude is at most 255) and to two (below 65536), each followed by 32
+// zeros, and their largest value.
+__global__ __launch_bounds__(kBlock) void k_caller_mags_small(const int32_t* __restrict__ cmag, int64_t n,
+                                                              uint8_t* __restrict__ m8, uint16_t* __restrict__ m16,
+                                                              unsigned* maxmag) {
+    unsigned most = 0u;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n + 32; p += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned v = p < n ? (unsigned)max(cmag[p], 0) : 0u;
+        most = v > most ? v : most;
+        m8[p] = (uint8_t)min(v, 255u);
+        m16[p] = (uint16_t)min(v, 65535u);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const unsigned o = (unsigned)__shfl_xor((int)most, off);
+        most = o > most ? o : most;
+    }
+    if (lane_id() == 0) atomicMax(maxmag, most);
 }
 
 // Reference rows per workgroup and query molecules per chunk of the sparse
@@ -2061,6 +2117,526 @@ __global__ __launch_bounds__(kSparseBlock) void k_sparse_f32(RowArgs R, const in
                 const float vv[4] = {v.x, v.y, v.z, v.w};
                 for (int k = 0; k < cw - 4 * j; ++k) orow[4 * j + k] = vv[k];
             }
+        }
+    }
+}
+
+// Sparse top-k: candidate keys a workgroup holds in LDS, and admitted pairs
+// a wave queues before it computes their keys.
+constexpr int kSparseTopkCap = 768;
+constexpr int kSparseQueue = 2 * kWave;
+
+// Bit 7 of each byte of w that is at least n (1 <= n <= 128): per byte,
+// bit 7 of (b & 0x7F) + 128 - n (no carry leaves the byte) or of b itself.
+__device__ __forceinline__ uint32_t bytes_ge(uint32_t w, uint32_t n) {
+    return (((w & 0x7F7F7F7Fu) + (128u - n) * 0x01010101u) | w) & 0x80808080u;
+}
+
+// The intersection of the pair behind a threshold key: the key holds
+// T0 = fl(I0 / (ma + mb0 - I0)) and T(I) = fl(I / (ma + mb0 - I)) grows
+// strictly with I, so one I in [1, 255] gives T0; 0 when none does.
+__device__ uint32_t sparse_recover_isct(float t0, float maf, float mbf) {
+    const int i0 = (int)rintf(t0 * (maf + mbf) / (1.0f + t0));
+    for (int d = -2; d <= 2; ++d) {
+        const int ii = i0 + d;
+        if (ii >= 1 && ii <= 255 && (float)ii <= maf && (float)ii <= mbf &&
+            pair_tanimoto<true>((float)ii, maf, mbf, false) == t0)
+            return (uint32_t)ii;
+    }
+    return 0u;
+}
+
+struct alignas(16) SparseTopkLds {
+    alignas(16) uint32_t cnt[kSparseCols / 4];  // per-pair counts of the chunk, one byte each
+    uint64_t cand[kSparseTopkCap];              // candidate keys of the row
+    int64_t lv_cur[kSparseMaxLevels];           // per level: next posting to read,
+    int32_t lv_left[kSparseMaxLevels];          // postings left,
+    int32_t lv_next[kSparseMaxLevels];          // the column of the next posting (INT32_MAX: none)
+    uint32_t queue[kSparseBlock / kWave][kSparseQueue];  // per wave: (column in chunk << 8) | count
+    uint32_t hist[256];
+    uint64_t prefix, pmask, thr;
+    unsigned long long kmin;
+    uint32_t need, exact, nlv, ncand;
+    uint32_t imin, ti, tu;                      // the threshold as a ratio ti / tu (ti = 0: none)
+};
+
+// Top-k of reference rows [rb, rb + gridDim.x) (grouped rows of at most 255
+// levels, magnitudes at least the count sums) from the inverted index of
+// the query set, whose postings are grouped by chunk of kSparseCols
+// columns.  Workgroup (x, s) takes row rb + x and the chunks [s cps,
+// (s + 1) cps): it looks up the row's levels once, then per chunk counts
+// the pairs in LDS -- each wave walks whole levels from where the previous
+// chunk stopped, the first reads of four levels in flight together -- and
+// finds the pairs whose key beats the row's threshold, the k-th best key
+// held so far.  The chunks come in column order, so a pair of a later chunk
+// beats the threshold key only with a larger T, hence a larger ratio I / U
+// than the threshold pair's: that exact integer test, after a test of the
+// count against the smallest intersection it allows (imin), passes over
+// most pairs without a division.  The scan tests 16 counts per lane at a
+// time against imin (a byte mask per word); the counts that reach it are
+// queued per wave, and each 64 queued pairs get their magnitudes, the
+// ratio test and, when they pass, their T and key.  Candidates collect in LDS; when a chunk's do not
+// fit, the k-th best of those stored becomes the threshold and the chunk
+// is scanned again against it.  Only pairs with T > 0 are candidates: the
+// T = 0 columns fill a row with fewer than k of them, in ascending order.
+// With one column range (gridDim.y == 1) the workgroup writes the row's
+// sorted top-k (and that fill) to out_idx / out_t; otherwise its candidates
+// (at most k keys) to buf / bufcnt, segment x * gridDim.y + s, for the
+// final selection.  Magnitudes come from the caller-order copy of MB bytes
+// per column.
+template <bool NARROW, int MB>
+__global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sparse_topk(
+    RowArgs R, int64_t rb, PostingQuery P, const void* __restrict__ cmagn, int64_t nq, int k, int64_t cps,
+    uint64_t* __restrict__ buf, int32_t* __restrict__ bufcnt, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_t) {
+    __shared__ SparseTopkLds L;
+    constexpr int kWaves = kSparseBlock / kWave;
+    const int64_t row = rb + blockIdx.x;
+    const int64_t nchunks = (nq + kSparseCols - 1) / kSparseCols;
+    const int64_t y0 = (int64_t)blockIdx.y * cps;
+    const int64_t y1 = min(nchunks, y0 + cps);
+    const int64_t cstart = y0 * kSparseCols;
+    const int ma = R.mag[row];
+    const float maf = (float)ma;
+    const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
+    auto mag_at = [&](int64_t c) -> uint32_t {
+        if constexpr (MB == 1)
+            return static_cast<const uint8_t*>(cmagn)[c];
+        else
+            return static_cast<const uint16_t*>(cmagn)[c];
+    };
+    if (threadIdx.x == 0) {
+        L.nlv = 0u;
+        L.ncand = 0u;
+        L.thr = 0ull;
+        L.imin = 1u;
+        L.ti = 0u;
+        L.tu = 1u;
+    }
+    __syncthreads();
+    {
+        // the row's levels found in the index, each with its postings from
+        // column cstart on (a binary search: the chunks are in order)
+        const int64_t o = R.off[row];
+        const int32_t len = R.len[row];
+        for (int e = threadIdx.x; e < len; e += blockDim.x) {
+            const int2 el = R.elems[o + e];
+            for (int t = 1; t <= el.y; ++t) {
+                int64_t first;
+                int32_t n;
+                // no query molecule has level t, so none has t + 1
+                if (!post_lookup(P, dev_level_key(el.x, (uint32_t)t), first, n)) break;
+                int32_t lo = 0, hi = n;
+                while (cstart > 0 && lo < hi) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (post_col<NARROW>(P, first + mid) < cstart)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                const int at = (int)atomicAdd(&L.nlv, 1u);
+                L.lv_cur[at] = first + lo;
+                L.lv_left[at] = n - lo;
+                L.lv_next[at] = lo < n ? (int32_t)post_col<NARROW>(P, first + lo) : INT32_MAX;
+            }
+        }
+    }
+    __syncthreads();
+    const int nl = (int)L.nlv;
+    int64_t c0 = 0;
+    uint32_t* wq = L.queue[w];
+    // valid lanes' keys to cand, one atomic per wave (every lane calls)
+    auto push = [&](uint64_t key, bool valid) {
+        const uint64_t m = __ballot(valid);
+        if (m == 0ull) return;
+        uint32_t base = 0u;
+        if (lane == 0) base = atomicAdd(&L.ncand, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        const uint32_t at = base + (uint32_t)rank_below(m);
+        if (valid && at < (uint32_t)kSparseTopkCap) L.cand[at] = key;
+    };
+    // The chunk's pairs whose count reaches imin go to the wave's queue
+    // (a lane tests 16 counts at a time, a byte mask per word, and queues
+    // those that reach imin one at a time); each 64 queued pairs get their
+    // magnitude, and those whose ratio beats ti / tu (every one when
+    // ti = 0) their T and key; valid lanes (T > 0, key > thr) are pushed.
+    // Every lane of the workgroup calls.
+    auto chunk_keys = [&](uint64_t thr, uint32_t imin, uint32_t ti, uint32_t tu) {
+        int qn = 0;  // queued pairs (wave-uniform)
+        auto flush = [&](int n) {
+            wave_lds_sync();
+            uint64_t key = 0ull;
+            bool valid = false;
+            if (lane < n) {
+                const uint32_t e = wq[lane];
+                const int64_t c = c0 + (int64_t)(e >> 8);
+                const uint32_t isct = e & 0xFFu, mb = mag_at(c);
+                // I tu > ti (ma + mb - I)
+                if (ti == 0u || isct * tu > ti * ((uint32_t)ma + mb - isct)) {
+                    const float t = pair_tanimoto<true>((float)isct, maf, (float)mb, ma == 0 || mb == 0u);
+                    key = topk_key(t, c);
+                    valid = t > 0.0f && key > thr;
+                }
+            }
+            push(key, valid);
+            const int rem = qn - n;
+            const uint32_t keep = lane < rem ? wq[n + lane] : 0u;
+            wave_lds_sync();
+            if (lane < rem) wq[lane] = keep;
+            qn = rem;
+        };
+#pragma unroll 1
+        for (int i = 0; i < kSparseCols / 16 / kSparseBlock; ++i) {
+            const int q = (int)threadIdx.x + i * kSparseBlock;
+            const uint4 v = reinterpret_cast<const uint4*>(L.cnt)[q];
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                // the word's counts of at least imin, one at a time (a
+                // count of 0 marks a column past the chunk's end)
+                uint32_t mk = bytes_ge(wv[h], imin);
+                while (__ballot(mk != 0u) != 0ull) {
+                    const bool has = mk != 0u;
+                    uint32_t e = 0u;
+                    if (has) {
+                        const uint32_t b = (uint32_t)__builtin_ctz(mk) >> 3;
+                        mk &= mk - 1u;
+                        e = ((uint32_t)(16 * q + 4 * h) + b) << 8 | ((wv[h] >> (8 * b)) & 0xFFu);
+                    }
+                    const uint64_t m = __ballot(has);
+                    if (has) wq[qn + rank_below(m)] = e;
+                    qn += __popcll(m);
+                    if (qn >= kWave) flush(kWave);
+                }
+            }
+        }
+        if (qn > 0) flush(qn);
+    };
+    // The k-th largest of cand[0, n) (n > k; keys distinct): MSB-first 8-bit
+    // radix select in LDS.  Called by every thread; the result is uniform.
+    auto select_kth = [&](uint32_t n) -> uint64_t {
+        __syncthreads();  // every thread has read ncand
+        if (threadIdx.x == 0) {
+            L.prefix = 0ull;
+            L.pmask = 0ull;
+            L.need = (uint32_t)k;
+            L.exact = 0u;
+        }
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            if (threadIdx.x < 256) L.hist[threadIdx.x] = 0u;
+            __syncthreads();
+            const uint64_t prefix = L.prefix, pmask = L.pmask;
+            for (uint32_t i = threadIdx.x; i < ((n + kWave - 1) & ~(uint32_t)(kWave - 1)); i += blockDim.x) {
+                const uint64_t key = i < n ? L.cand[i] : 0ull;
+                const bool act = i < n && (key & pmask) == prefix;
+                const uint32_t dig = (uint32_t)(key >> shift) & 255u;
+                // equal digits of one instruction add once (equal-address
+                // LDS atomics serialise)
+                const uint64_t am = __ballot(act);
+                if (am == 0ull) continue;
+                const int leader = __builtin_ctzll(am);
+                const uint32_t d0 = (uint32_t)__shfl((int)dig, leader);
+                if (__ballot(act && dig == d0) == am) {
+                    if (lane == leader) atomicAdd(&L.hist[d0], (uint32_t)__popcll(am));
+                } else if (act) {
+                    atomicAdd(&L.hist[dig], 1u);
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < kWave) {
+                // wave 0 finds the digit: suffix sums over 4 bins per lane
+                const uint32_t need = L.need;
+                const uint32_t h0 = L.hist[4 * lane], h1 = L.hist[4 * lane + 1], h2 = L.hist[4 * lane + 2],
+                               h3 = L.hist[4 * lane + 3];
+                const uint32_t sl = h0 + h1 + h2 + h3;
+                uint32_t S = sl;
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const uint32_t t = __shfl_down(S, off);
+                    if (lane + off < kWave) S += t;
+                }
+                const uint64_t ge = __ballot(S >= need);
+                const int Lb = 63 - __builtin_clzll(ge);
+                uint32_t above = __shfl(S - sl, Lb);
+                const uint32_t hb[4] = {(uint32_t)__shfl(h0, Lb), (uint32_t)__shfl(h1, Lb), (uint32_t)__shfl(h2, Lb),
+                                        (uint32_t)__shfl(h3, Lb)};
+                int d = 4 * Lb;
+                uint32_t cnt_d = hb[0];
+                for (int q = 3; q >= 0; --q) {
+                    if (above + hb[q] >= need) {
+                        d = 4 * Lb + q;
+                        cnt_d = hb[q];
+                        break;
+                    }
+                    above += hb[q];
+                }
+                if (lane == 0) {
+                    L.need = need - above;
+                    L.prefix = prefix | ((uint64_t)d << shift);
+                    L.pmask = pmask | ((uint64_t)255 << shift);
+                    L.exact = cnt_d == need - above ? 1u : 0u;
+                    L.kmin = ~0ull;
+                }
+            }
+            __syncthreads();
+            if (L.exact) {
+                // every key of the chosen bucket is kept: K is its smallest
+                const uint64_t bp = L.prefix, bm = L.pmask;
+                unsigned long long mn = ~0ull;
+                for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                    const uint64_t key = L.cand[i];
+                    if ((key & bm) == bp && key < mn) mn = key;
+                }
+                atomicMin(&L.kmin, mn);
+                __syncthreads();
+                break;
+            }
+        }
+        const uint64_t K = L.exact ? (uint64_t)L.kmin : L.prefix;
+        __syncthreads();
+        return K;
+    };
+    // cand keeps its keys >= K among cand[0, n) (n <= kSparseTopkCap)
+    auto keep_from = [&](uint32_t n, uint64_t K) {
+        constexpr int kPer = (kSparseTopkCap + kSparseBlock - 1) / kSparseBlock;
+        uint64_t mine[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t i = threadIdx.x + j * kSparseBlock;
+            mine[j] = i < n ? L.cand[i] : 0ull;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) L.ncand = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) push(mine[j], mine[j] != 0ull && mine[j] >= K);
+    };
+    // the threshold K as a ratio (its pair recovered from its column) and
+    // the smallest intersection it allows: I tu > ti (ma + mb - I) >= ti ma
+    // (mb >= I) needs I > ti ma / tu
+    auto ratio_of = [&](uint64_t K, uint32_t& ti, uint32_t& tu, uint32_t& imin) {
+        const uint32_t col = 0xFFFFFFFFu - (uint32_t)K;
+        const uint32_t mb0 = mag_at((int64_t)col);
+        const float t0 = __uint_as_float((uint32_t)(K >> 32) & 0x7FFFFFFFu);
+        const uint32_t i0 = sparse_recover_isct(t0, maf, (float)mb0);
+        if (i0 == 0u) {
+            ti = 0u;
+            tu = 1u;
+            imin = 1u;
+            return;
+        }
+        ti = i0;
+        tu = (uint32_t)ma + mb0 - i0;
+        const uint64_t lo = (uint64_t)i0 * (uint64_t)ma / tu + 1;
+        imin = lo > 128 ? 128u : (uint32_t)lo;
+    };
+    // cuts the candidates to the k keys >= K, their k-th largest, which
+    // becomes the threshold
+    auto reduce_to_k = [&](uint32_t n) {
+        const uint64_t K = select_kth(n);
+        keep_from(n, K);
+        if (threadIdx.x == 0) {
+            L.thr = K;
+            ratio_of(K, L.ti, L.tu, L.imin);
+        }
+        __syncthreads();
+    };
+    // adds one to the count of column c when it lies in the chunk
+    auto count_col = [&](int32_t c, int32_t c0i, int32_t cendi) {
+        if (c >= c0i && c < cendi) {
+            const uint32_t cl = (uint32_t)(c - c0i);
+            atomicAdd(&L.cnt[cl >> 2], 1u << (8 * (cl & 3)));
+        }
+    };
+    for (int64_t y = y0; y < y1; ++y) {
+        c0 = y * kSparseCols;
+        const int cw = (int)min((int64_t)kSparseCols, nq - c0);
+        const int64_t cend = c0 + cw;
+        for (int i = threadIdx.x; i < kSparseCols / 16; i += blockDim.x)
+            reinterpret_cast<uint4*>(L.cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+        // the row's threshold and candidates as the chunk starts (read by
+        // every thread before any of the chunk's keys is added)
+        const uint64_t thr = L.thr;
+        const uint32_t imin = L.imin, n0 = L.ncand, ti = L.ti, tu = L.tu;
+        // each wave walks whole levels, four at a time: the chunk's
+        // postings of a level come first in what is left of it; the first
+        // window (one or two reads of 64, from the level's postings per
+        // remaining chunk) of the four levels is read at once, and a level
+        // whose chunk postings fill it goes on alone, four reads at a time
+        const int32_t c0i = (int32_t)c0, cendi = (int32_t)cend;
+        const int32_t rest = (int32_t)max((int64_t)1, y1 - y);
+        for (int g = w; g < nl; g += 4 * kWaves) {
+            int32_t cv[4][2];
+            bool act[4];
+            int nbf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int l = g + i * kWaves;
+                act[i] = l < nl && L.lv_next[l < nl ? l : 0] < cendi;
+                const int64_t cur = act[i] ? L.lv_cur[l] : 0;
+                const int32_t left = act[i] ? L.lv_left[l] : 0;
+                nbf[i] = left / rest < 40 ? 1 : 2;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int q = u * kWave + lane;
+                    cv[i][u] = (act[i] && u < nbf[i] && q < left) ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!act[i]) continue;
+                const int l = g + i * kWaves;
+                int64_t cur = L.lv_cur[l];
+                int32_t left = L.lv_left[l];
+                int32_t next = INT32_MAX;
+                bool stop = false;
+                int took = 0;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (u >= nbf[i] || stop) break;
+                    count_col(cv[i][u], c0i, cendi);
+                    const int t = __popcll(__ballot(cv[i][u] < cendi));
+                    took += t;
+                    if (t < kWave) {
+                        stop = true;
+                        next = __shfl(cv[i][u], t & (kWave - 1));
+                    }
+                }
+                cur += took;
+                left -= took;
+                if (!stop && left <= 0) stop = true;
+                while (!stop) {
+                    int32_t c[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = u * kWave + lane;
+                        c[u] = q < left ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
+                    }
+                    took = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (stop) break;
+                        count_col(c[u], c0i, cendi);
+                        const int t = __popcll(__ballot(c[u] < cendi));
+                        took += t;
+                        if (t < kWave) {
+                            stop = true;
+                            next = __shfl(c[u], t & (kWave - 1));
+                        }
+                    }
+                    cur += took;
+                    left -= took;
+                    if (!stop && left <= 0) stop = true;
+                }
+                if (lane == 0) {
+                    L.lv_cur[l] = cur;
+                    L.lv_left[l] = left;
+                    L.lv_next[l] = left > 0 ? next : INT32_MAX;
+                }
+            }
+        }
+        __syncthreads();
+        chunk_keys(thr, imin, ti, tu);
+        __syncthreads();
+        uint32_t nc = L.ncand;
+        if (nc > (uint32_t)kSparseTopkCap) {
+            // the chunk's keys did not fit: the k-th best of the stored ones
+            // (a lower bound of the k-th best of all, above thr) becomes the
+            // threshold, the held keys below it go and the chunk is scanned
+            // again (with its own admission test, which every key above thr
+            // passes) for its keys at or above it -- until they fit
+            uint32_t held = n0;
+            uint64_t K = 0ull;
+            while (nc > (uint32_t)kSparseTopkCap) {
+                K = select_kth((uint32_t)kSparseTopkCap);
+                keep_from(held, K);
+                __syncthreads();
+                held = L.ncand;
+                chunk_keys(K - 1, imin, ti, tu);
+                __syncthreads();
+                nc = L.ncand;
+            }
+            if (threadIdx.x == 0) {
+                L.thr = K;
+                ratio_of(K, L.ti, L.tu, L.imin);
+            }
+            __syncthreads();
+        }
+        if (nc > (uint32_t)(kSparseTopkCap / 2) && nc > (uint32_t)k) reduce_to_k(nc);
+    }
+    {
+        const uint32_t nc = L.ncand;
+        if (nc > (uint32_t)k) reduce_to_k(nc);
+    }
+    const uint32_t have = L.ncand;
+    if (gridDim.y > 1) {
+        const int64_t seg = (int64_t)blockIdx.x * gridDim.y + blockIdx.y;
+        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) buf[seg * k + i] = L.cand[i];
+        if (threadIdx.x == 0) bufcnt[seg] = (int32_t)have;
+        return;
+    }
+    // sorted (T descending, then smaller column): bitonic over kSparseBlock keys
+    static_assert(SIML_MAX_TOPK <= kSparseBlock && kSparseBlock <= kSparseTopkCap, "sort width");
+    uint64_t* srt = L.cand;
+    if (threadIdx.x >= have) srt[threadIdx.x] = 0ull;
+    __syncthreads();
+    for (int size = 2; size <= kSparseBlock; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int i = threadIdx.x, j = i ^ stride;
+            if (j > i) {
+                const bool desc = (i & size) == 0;
+                const uint64_t a = srt[i], b = srt[j];
+                if (desc ? (a < b) : (a > b)) {
+                    srt[i] = b;
+                    srt[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int32_t* oi = out_idx + (int64_t)blockIdx.x * k;
+    float* ot = out_t + (int64_t)blockIdx.x * k;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        int32_t ix = -1;
+        float tv = -__builtin_inff();
+        if ((uint32_t)i < have) {
+            const uint64_t key = srt[i];
+            const uint32_t o = (uint32_t)(key >> 32);
+            tv = __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+            ix = (int32_t)(0xFFFFFFFFu - (uint32_t)key);
+        }
+        oi[i] = ix;
+        ot[i] = tv;
+    }
+    if (have >= (uint32_t)k) return;
+    // fewer than k positive pairs: the columns with T == 0 in ascending
+    // order follow (the smallest columns that are not candidates)
+    uint32_t* poscol = L.cnt;
+    poscol[threadIdx.x] = threadIdx.x < have ? 0xFFFFFFFFu - (uint32_t)srt[threadIdx.x] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int size = 2; size <= kSparseBlock; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int i = threadIdx.x, j = i ^ stride;
+            if (j > i) {
+                const bool asc = (i & size) == 0;
+                const uint32_t a = poscol[i], b = poscol[j];
+                if (asc ? (a > b) : (a < b)) {
+                    poscol[i] = b;
+                    poscol[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint32_t p = 0;
+        int64_t c = 0;
+        for (int i = (int)have; i < k && c < nq; ++c) {
+            while (p < have && poscol[p] < (uint32_t)c) ++p;
+            if (p < have && poscol[p] == (uint32_t)c) continue;
+            oi[i] = (int32_t)c;
+            ot[i] = 0.0f;
+            ++i;
         }
     }
 }
@@ -2533,10 +3109,15 @@ struct ColSet {
         int64_t slots = 0, total = 0;
         double per_level = 0.0;  // expected postings of a stream level
         DevBuf slot, col, cmag;
+        DevBuf cmag8, cmag16;    // caller-order magnitudes as 1 and 2 bytes
+        int mag_bytes = 0;       // the narrowest of those that holds every magnitude (0: neither)
         void release() {
             slot.release();
             col.release();
             cmag.release();
+            cmag8.release();
+            cmag16.release();
+            mag_bytes = 0;
             built = ok = narrow = false;
         }
         PostingQuery query() const {
@@ -3971,6 +4552,7 @@ extern "C" {
 // per slot; a set with more distinct level keys than fit at half load is
 // not indexed and keeps the bit-sliced path.
 constexpr int64_t kPostMaxSlots = int64_t(1) << 25;
+constexpr int64_t kPostFirstSlots = int64_t(1) << 20;
 
 // Builds the inverted index of the handle's query set unless it exists
 // (ColSet::post), on the handle's stream; returns when it is complete.
@@ -3983,45 +4565,64 @@ int ensure_postings(siml_ctx* ctx) {
     X.built = true;
     X.ok = false;
     if (!C.levels_ok || C.n <= 0) return SIML_OK;
-    // slots: a power of two at least twice the stream's entries (a bound on
-    // its distinct keys), at most kPostMaxSlots
+    // slots: a power of two at least twice the distinct keys, found by
+    // insertion: the first table holds up to 2^20 slots (or twice the
+    // stream's entries, if fewer); a table more than half full (or full) is
+    // built again with twice the distinct keys' slots, at most kPostMaxSlots
     int64_t slots = 1024;
-    while (slots < 2 * C.btiles_total && slots < kPostMaxSlots) slots <<= 1;
-    X.slots = slots;
-    DevBuf key, cnt, off, col32, cur, tmp;
-    TRY(key.reserve(sizeof(uint32_t) * (size_t)slots));
-    TRY(cnt.reserve(sizeof(int32_t) * (size_t)slots));
-    TRY(off.reserve(sizeof(int64_t) * (size_t)slots));
-    TRY(cur.reserve(sizeof(int32_t) * (size_t)slots));
-    const size_t scan_bytes = siml::csr_scan_bytes(slots);
-    TRY(tmp.reserve(scan_bytes + 256));
-    // after the scan's scratch: {0, total} row offsets, the overflow flag,
-    // then {sum of count^2, sum of count}
-    int64_t* tail = reinterpret_cast<int64_t*>(tmp.as<char>() + scan_bytes);
-    int32_t* flag = reinterpret_cast<int32_t*>(tail + 2);
-    unsigned long long* stats = reinterpret_cast<unsigned long long*>(tail + 4);
-    const unsigned fb = (unsigned)std::min<int64_t>((slots + 255) / 256, 65535);
-    k_fill<uint32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(key.as<uint32_t>(), slots, 0u);
-    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cnt.as<int32_t>(), slots, 0);
-    k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cur.as<int32_t>(), slots, 0);
-    k_fill<int64_t><<<dim3(1), dim3(64), 0, ctx->stream>>>(tail, 8, 0);
-    TRY(launch_check("k_fill<postings>"));
-    PostingArgs P{key.as<uint32_t>(), cnt.as<int32_t>(), off.as<int64_t>(), nullptr, cur.as<int32_t>(),
-                  (uint32_t)(slots - 1), flag};
+    while (slots < 2 * C.btiles_total && slots < kPostFirstSlots) slots <<= 1;
+    DevBuf key, cnt, off, col32, cur, tmp, inv;
     const ColArgs CA = C.args();
     const unsigned wb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((C.ntiles + 3) / 4, 16384));
-    k_post_insert<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
-    TRY(launch_check("k_post_insert"));
-    TRY(siml::csr_scan(cnt.as<int32_t>(), 1, slots, off.as<int64_t>(), tail, tmp.p, ctx->stream));
-    k_post_stats<<<dim3(std::min<unsigned>(fb, 2048)), dim3(256), 0, ctx->stream>>>(cnt.as<int32_t>(), slots, stats);
-    TRY(launch_check("k_post_stats"));
     int64_t host[8] = {0};
-    HIP_TRY(hipMemcpyAsync(host, tail, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    int64_t* tail = nullptr;
+    size_t scan_bytes = 0;
+    PostingArgs P{};
+    for (;;) {
+        TRY(key.reserve(sizeof(uint32_t) * (size_t)slots));
+        TRY(cnt.reserve(sizeof(int32_t) * (size_t)slots));
+        TRY(off.reserve(sizeof(int64_t) * (size_t)slots));
+        TRY(cur.reserve(sizeof(int32_t) * (size_t)slots));
+        scan_bytes = siml::csr_scan_bytes(slots);
+        TRY(tmp.reserve(scan_bytes + 256));
+        // after the scan's scratch: {0, total} row offsets, the overflow
+        // flag, then {sum of count^2, sum of count, occupied slots}
+        tail = reinterpret_cast<int64_t*>(tmp.as<char>() + scan_bytes);
+        const unsigned fb = (unsigned)std::min<int64_t>((slots + 255) / 256, 65535);
+        k_fill<uint32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(key.as<uint32_t>(), slots, 0u);
+        k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cnt.as<int32_t>(), slots, 0);
+        k_fill<int32_t><<<dim3(fb), dim3(256), 0, ctx->stream>>>(cur.as<int32_t>(), slots, 0);
+        k_fill<int64_t><<<dim3(1), dim3(64), 0, ctx->stream>>>(tail, 8, 0);
+        TRY(launch_check("k_fill<postings>"));
+        P = PostingArgs{key.as<uint32_t>(), cnt.as<int32_t>(), off.as<int64_t>(), nullptr, cur.as<int32_t>(),
+                        (uint32_t)(slots - 1), reinterpret_cast<int32_t*>(tail + 2)};
+        k_post_insert<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
+        TRY(launch_check("k_post_insert"));
+        k_post_stats<<<dim3(std::min<unsigned>(fb, 2048)), dim3(256), 0, ctx->stream>>>(
+            cnt.as<int32_t>(), slots, reinterpret_cast<unsigned long long*>(tail + 4));
+        TRY(launch_check("k_post_stats"));
+        HIP_TRY(hipMemcpyAsync(host, tail, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const bool overflow = (host[2] & 0xFFFFFFFF) != 0;
+        const int64_t distinct = host[6];
+        if (!overflow && 2 * distinct <= slots) break;
+        if (slots >= kPostMaxSlots) {
+            if (!overflow) break;  // more than half full at the largest size
+            X.release();
+            X.built = true;
+            return SIML_OK;
+        }
+        int64_t next = slots * 2;
+        while (next < 2 * distinct && next < kPostMaxSlots) next <<= 1;
+        slots = std::min<int64_t>(next, kPostMaxSlots);
+    }
+    X.slots = slots;
+    TRY(siml::csr_scan(cnt.as<int32_t>(), 1, slots, off.as<int64_t>(), tail, tmp.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host, tail, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const int64_t total = host[1];
-    const int32_t overflow = (int32_t)(host[2] & 0xFFFFFFFF);
     const unsigned long long sq = (unsigned long long)host[4], sum = (unsigned long long)host[5];
-    if (overflow || total <= 0 || (unsigned long long)total != sum) {
+    if (total <= 0 || (unsigned long long)total != sum) {
         X.release();
         X.built = true;
         return SIML_OK;
@@ -4030,11 +4631,34 @@ int ensure_postings(siml_ctx* ctx) {
     TRY(X.cmag.reserve(sizeof(int32_t) * (size_t)(C.n + 4)));
     TRY(X.slot.reserve(sizeof(uint4) * (size_t)slots));
     P.col = col32.as<int32_t>();
-    k_post_fill<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
-    TRY(launch_check("k_post_fill"));
-    k_post_pack<<<dim3(fb), dim3(kBlock), 0, ctx->stream>>>(key.as<uint32_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
-                                                          slots, X.slot.as<uint4>());
-    TRY(launch_check("k_post_pack"));
+    if (C.n <= kSparseCols) {
+        k_post_fill<<<dim3(wb), dim3(kBlock), 0, ctx->stream>>>(CA, P);
+        TRY(launch_check("k_post_fill"));
+    } else {
+        // one fill per chunk of kSparseCols caller columns, in order: each
+        // key's postings come out grouped by chunk, as the sparse top-k walk
+        // reads them
+        const int32_t* d_inv = nullptr;
+        if (CA.perm) {
+            TRY(inv.reserve(sizeof(int32_t) * (size_t)C.n));
+            k_inv_perm<<<dim3((unsigned)std::min<int64_t>((C.n + kBlock - 1) / kBlock, 65535)), dim3(kBlock), 0,
+                         ctx->stream>>>(CA.perm, C.n, inv.as<int32_t>());
+            TRY(launch_check("k_inv_perm"));
+            d_inv = inv.as<int32_t>();
+        }
+        for (int64_t c0 = 0; c0 < C.n; c0 += kSparseCols) {
+            const int64_t c1 = std::min<int64_t>(C.n, c0 + kSparseCols);
+            k_post_fill_cols<<<dim3((unsigned)((c1 - c0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream>>>(
+                CA, P, d_inv, c0, c1);
+            TRY(launch_check("k_post_fill_cols"));
+        }
+    }
+    {
+        const unsigned fb = (unsigned)std::min<int64_t>((slots + 255) / 256, 65535);
+        k_post_pack<<<dim3(fb), dim3(kBlock), 0, ctx->stream>>>(key.as<uint32_t>(), cnt.as<int32_t>(),
+                                                              off.as<int64_t>(), slots, X.slot.as<uint4>());
+        TRY(launch_check("k_post_pack"));
+    }
     X.narrow = C.n <= 65536;
     if (X.narrow) {
         TRY(X.col.reserve(sizeof(uint16_t) * (size_t)total));
@@ -4047,7 +4671,18 @@ int ensure_postings(siml_ctx* ctx) {
     k_caller_mags<<<dim3((unsigned)std::min<int64_t>((C.n + 4 + 255) / 256, 65535)), dim3(256), 0, ctx->stream>>>(
         CA, X.cmag.as<int32_t>());
     TRY(launch_check("k_caller_mags"));
+    TRY(X.cmag8.reserve((size_t)C.n + 32));
+    TRY(X.cmag16.reserve(sizeof(uint16_t) * ((size_t)C.n + 32)));
+    k_caller_mags_small<<<dim3((unsigned)std::min<int64_t>((C.n + 32 + 255) / 256, 65535)), dim3(256), 0,
+                          ctx->stream>>>(X.cmag.as<int32_t>(), C.n, X.cmag8.as<uint8_t>(), X.cmag16.as<uint16_t>(),
+                                         reinterpret_cast<unsigned*>(tail + 7));
+    TRY(launch_check("k_caller_mags_small"));
+    HIP_TRY(hipMemcpyAsync(host + 7, tail + 7, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    {
+        const uint64_t maxmag = (uint64_t)host[7] & 0xFFFFFFFFull;
+        X.mag_bytes = maxmag <= 255 ? 1 : (maxmag <= 65535 ? 2 : 0);
+    }
     X.total = total;
     X.per_level = (double)sq / (double)sum;
     X.ok = true;
@@ -4063,12 +4698,19 @@ int ensure_postings(siml_ctx* ctx) {
 // 0.72 per pair plus 0.0112 per pair and query level (1.57 and 1.11 ms).
 constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.30;
 constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
+// Top-k (no matrix to write), measured on the top-100 of 257684 lognormal
+// strings over 1M (0.09 postings met per pair, 45 levels per molecule):
+// the sparse kernel takes 0.51 per pair (clearing and scanning the counts,
+// its per-chunk steps) plus 0.30 per posting met (138 ms), the seeded
+// bit-sliced passes 0.84 per pair plus 0.0112 per pair and query level
+// (347 ms).
+constexpr double kSparseTopkPsPerPair = 0.51, kBitsTopkPsPerPair = 0.84;
 
 // Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
 // levels) and the rest (merge kernel), when the sparse path applies to the
 // call: policy SPARSE, or AUTO with few expected postings.
 bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, const BitPlan& bp, std::vector<int32_t>& sparse_rows,
-                 std::vector<int32_t>& merge_rows, int* rc) {
+                 std::vector<int32_t>& merge_rows, int* rc, bool topk = false) {
     *rc = SIML_OK;
     if (!bp.use || (ctx->policy != SIML_POLICY_AUTO && ctx->policy != SIML_POLICY_SPARSE)) return false;
     *rc = ensure_postings(ctx);
@@ -4096,8 +4738,88 @@ bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, const BitPlan& bp, std::
     // random stream level (exact when the rows are drawn like the queries)
     const double pairs = (double)(rl - rb) * (double)ctx->qry->n;
     const double qlevels = (double)X.total / (double)ctx->qry->n;
+    if (topk)
+        return kSparseTopkPsPerPair * pairs + kSparsePsPerPosting * levels * X.per_level <
+               (kBitsTopkPsPerPair + kBitsPsPerPairLevel * qlevels) * pairs;
     return kSparsePsPerPair * pairs + kSparsePsPerPosting * levels * X.per_level <
            (kBitsPsPerPair + kBitsPsPerPairLevel * qlevels) * pairs;
+}
+
+// Launch shape of the sparse top-k: column ranges per row (several when
+// the rows alone give too few workgroups, at most kTopkFan so that one
+// final selection workgroup per row merges them) and chunks per range.
+constexpr int64_t kSparseTopkWorkgroups = 2048;
+struct SparseTopkShape {
+    int64_t nsplit = 1, cps = 1;
+    size_t buf = 0, cnt = 0;  // candidate segments (nsplit > 1)
+    size_t scratch() const { return buf + cnt; }
+};
+SparseTopkShape sparse_topk_shape(int64_t nr, int64_t nq, int k) {
+    SparseTopkShape s;
+    const int64_t nchunks = std::max<int64_t>(1, (nq + kSparseCols - 1) / kSparseCols);
+    const int64_t want = std::max<int64_t>(1, (kSparseTopkWorkgroups + nr - 1) / std::max<int64_t>(nr, 1));
+    s.nsplit = std::min<int64_t>(std::min<int64_t>(nchunks, kTopkFan), want);
+    s.cps = (nchunks + s.nsplit - 1) / s.nsplit;
+    s.nsplit = (nchunks + s.cps - 1) / s.cps;
+    if (s.nsplit > 1) {
+        auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+        s.buf = r256(sizeof(uint64_t) * (size_t)nr * (size_t)s.nsplit * (size_t)k);
+        s.cnt = r256(sizeof(int32_t) * (size_t)nr * (size_t)s.nsplit);
+    }
+    return s;
+}
+
+// Top-k of rows [rb, rl) on the sparse path (every row grouped with at most
+// 255 levels, magnitudes at least the count sums); results into the
+// handle's result buffers.
+static int sparse_topk(siml_ctx* ctx, int64_t rb, int64_t rl, int32_t k) {
+    const int64_t nr = rl - rb, nq = ctx->qry->n;
+    const ColSet::Postings& X = ctx->qry->post;
+    const PostingQuery P = X.query();
+    const SparseTopkShape shp = sparse_topk_shape(nr, nq, k);
+    uint64_t* d_buf = nullptr;
+    int32_t* d_cnt = nullptr;
+    if (shp.nsplit > 1) {
+        TRY(ctx->scratch.reserve(shp.scratch()));
+        d_buf = ctx->scratch.as<uint64_t>();
+        d_cnt = reinterpret_cast<int32_t*>(ctx->scratch.as<char>() + shp.buf);
+    }
+    int32_t* out_idx = ctx->res.as<int32_t>();
+    float* out_t = ctx->res2.as<float>();
+    GraphRegion gr(ctx);
+    TRY(gr.begin());
+    TimedRegion tr(ctx, nr * nq, !gr.on);
+    const dim3 grid((unsigned)nr, (unsigned)shp.nsplit);
+    auto go = [&](auto kern, const void* mags) {
+        kern<<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(ctx->ref.args(), rb, P, mags, nq, k, shp.cps, d_buf, d_cnt,
+                                                           out_idx, out_t);
+    };
+    const void* m8 = X.cmag8.p;
+    const void* m16 = X.cmag16.p;
+    if (X.narrow && X.mag_bytes == 1)
+        go(k_sparse_topk<true, 1>, m8);
+    else if (X.narrow)
+        go(k_sparse_topk<true, 2>, m16);
+    else if (X.mag_bytes == 1)
+        go(k_sparse_topk<false, 1>, m8);
+    else
+        go(k_sparse_topk<false, 2>, m16);
+    TRY(launch_check("k_sparse_topk"));
+    tr.select();
+    if (shp.nsplit > 1) {
+        k_topk_final<<<dim3((unsigned)nr), dim3(kBlock), topk_lds(k), ctx->stream>>>(d_buf, d_cnt, (int)shp.nsplit, k,
+                                                                                   k, 1, nq, out_idx, out_t);
+        TRY(launch_check("k_topk_final"));
+    }
+    tr.done();
+    if (gr.on) {
+        TRY(gr.end());
+        TimedRegion tg(ctx, nr * nq);
+        TRY(gr.launch());
+        tg.done();
+    }
+    ctx->last_kernel = "topk_sparse";
+    return SIML_OK;
 }
 
 static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
@@ -4450,6 +5172,15 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
     // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
     // final selection fills up with T == 0 columns in index order)
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
+    if (pos_only && !mem && nq > 0 && nq <= 0x7FFFFFFF && nr <= 0x7FFFFFFF) {
+        // the sparse path when every row can take it and the model (or the
+        // policy) prefers it
+        std::vector<int32_t> srows, smerge;
+        int prc = SIML_OK;
+        const bool sp = plan_sparse(ctx, rowbase, rowlimit, bp, srows, smerge, &prc, true);
+        TRY(prc);
+        if (sp && smerge.empty() && ctx->qry->post.mag_bytes != 0) return sparse_topk(ctx, rowbase, rowlimit, k);
+    }
     static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
                   "top-k buffer exceeds wave registers");
     const int gt_plan = bp.use ? bits_tables(ctx, bp) : 1;
@@ -4891,9 +5622,10 @@ int siml_call_bytes(siml_ctx* ctx, int op, int64_t rows, int32_t k, int64_t* byt
         for (const auto& pl : plans) {
             size_t b = 0;
             if (op == SIML_OP_TOPK)
-                b = topk_shape(ctx, rows, pl.first, gmin, pl.second, k, sh.target_wg, sh.topk_wave_cap,
-                               kTopkBufferBudget, sh.persistent)
-                        .scratch();
+                b = std::max(topk_shape(ctx, rows, pl.first, gmin, pl.second, k, sh.target_wg, sh.topk_wave_cap,
+                                        kTopkBufferBudget, sh.persistent)
+                                 .scratch(),
+                             sparse_topk_shape(rows, ctx->qry->n, k).scratch());
             else if (op == SIML_OP_NEIGHBORS)
                 b = nbr_shape(ctx, rows, pl.first, gmin, 1, true, sh.target_wg).scratch();
             else if (sh.persistent)

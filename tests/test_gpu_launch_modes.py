@@ -9,13 +9,14 @@ whose sequence differs (seeded and unseeded top-k, merge-kernel rows, 1 to
 300 rows).
 
 Persistent grid (siml_set_persistent): the bit-sliced similarity pass run by
-resident workgroups taking row-group slots from per-XCD work queues.
+resident workgroups taking row-group slots from per-XCD work queues (the
+handles keep the bit-sliced kernels: policy BITS).
 """
 import numpy as np
 import pytest
 
 import oracle
-from siml_amd import compiler, synth
+from siml_amd import _lib, compiler, synth
 from siml_amd.GPULingo import GPULingo, LingoDB
 
 pytestmark = pytest.mark.gpu
@@ -141,6 +142,7 @@ def test_persistent_grid_matches_oracle(db_and_queries):
     n = Q[0].shape[0]
     ordinary, pers = GPULingo(0), GPULingo(0)
     for g in (ordinary, pers):
+        g._h.call("siml_set_policy", _lib.POLICY_BITS)
         g.attach_db(db)
         g.set_refsmiles(*Q)
     for wg, cap, seed in ((8192, 1024, 1), (2048, 256, 0), (65536, 1024, 16), (512, 64, 4)):
