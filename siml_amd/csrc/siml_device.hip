@@ -2256,8 +2256,8 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         if (valid && at < (uint32_t)kSparseTopkCap) L.cand[at] = key;
     };
     // The chunk's pairs whose count reaches imin go to the wave's queue
-    // (a lane tests 16 counts at a time, a byte mask per word, and queues
-    // those that reach imin one at a time); each 64 queued pairs get their
+    // (a lane tests 16 counts at a time, a byte mask per word packed into
+    // one mask, and queues those that reach imin one at a time); each 64 queued pairs get their
     // magnitude, and those whose ratio beats ti / tu (every one when
     // ti = 0) their T and key; valid lanes (T > 0, key > thr) are pushed.
     // Every lane of the workgroup calls.
@@ -2289,25 +2289,28 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         for (int i = 0; i < kSparseCols / 16 / kSparseBlock; ++i) {
             const int q = (int)threadIdx.x + i * kSparseBlock;
             const uint4 v = reinterpret_cast<const uint4*>(L.cnt)[q];
-            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                // the word's counts of at least imin, one at a time (a
-                // count of 0 marks a column past the chunk's end)
-                uint32_t mk = bytes_ge(wv[h], imin);
-                while (__ballot(mk != 0u) != 0ull) {
-                    const bool has = mk != 0u;
-                    uint32_t e = 0u;
-                    if (has) {
-                        const uint32_t b = (uint32_t)__builtin_ctz(mk) >> 3;
-                        mk &= mk - 1u;
-                        e = ((uint32_t)(16 * q + 4 * h) + b) << 8 | ((wv[h] >> (8 * b)) & 0xFFu);
-                    }
-                    const uint64_t m = __ballot(has);
-                    if (has) wq[qn + rank_below(m)] = e;
-                    qn += __popcll(m);
-                    if (qn >= kWave) flush(kWave);
+            // one bit per count of at least imin among the 16 (bit 4h + b
+            // for byte b of word h; a count of 0 marks a column past the
+            // chunk's end), then those counts one at a time
+            auto pack = [](uint32_t m) {  // bits 7, 15, 23, 31 -> 0..3
+                return ((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u);
+            };
+            uint32_t mk = pack(bytes_ge(v.x, imin)) | (pack(bytes_ge(v.y, imin)) << 4) |
+                          (pack(bytes_ge(v.z, imin)) << 8) | (pack(bytes_ge(v.w, imin)) << 12);
+            while (__ballot(mk != 0u) != 0ull) {
+                const bool has = mk != 0u;
+                uint32_t e = 0u;
+                if (has) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(mk);
+                    mk &= mk - 1u;
+                    const uint32_t h = j >> 2, b = j & 3u;
+                    const uint32_t wd = h == 0 ? v.x : h == 1 ? v.y : h == 2 ? v.z : v.w;
+                    e = ((uint32_t)(16 * q) + j) << 8 | ((wd >> (8 * b)) & 0xFFu);
                 }
+                const uint64_t m = __ballot(has);
+                if (has) wq[qn + rank_below(m)] = e;
+                qn += __popcll(m);
+                if (qn >= kWave) flush(kWave);
             }
         }
         if (qn > 0) flush(qn);
