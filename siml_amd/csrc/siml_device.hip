@@ -2146,6 +2146,132 @@ __device__ uint32_t sparse_recover_isct(float t0, float maf, float mbf) {
     return 0u;
 }
 
+// The row's levels found in the index (Lingo level keys; a level t missing
+// from the index ends its Lingo's levels, since a molecule with level t + 1
+// has level t), each with its postings from column cstart on -- a binary
+// search, the postings being grouped by chunk in chunk order: per level the
+// next posting, the postings left and the column of the next posting
+// (INT32_MAX: none); *nlv counts the levels.  Every thread of the
+// workgroup calls, between barriers.
+template <bool NARROW>
+__device__ void sparse_row_levels(const RowArgs& R, int64_t row, const PostingQuery& P, int64_t cstart,
+                                  int64_t* lv_cur, int32_t* lv_left, int32_t* lv_next, uint32_t* nlv) {
+    const int64_t o = R.off[row];
+    const int32_t len = R.len[row];
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+        const int2 el = R.elems[o + e];
+        for (int t = 1; t <= el.y; ++t) {
+            int64_t first;
+            int32_t n;
+            if (!post_lookup(P, dev_level_key(el.x, (uint32_t)t), first, n)) break;
+            int32_t lo = 0, hi = n;
+            while (cstart > 0 && lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (post_col<NARROW>(P, first + mid) < cstart)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            const int at = (int)atomicAdd(nlv, 1u);
+            lv_cur[at] = first + lo;
+            lv_left[at] = n - lo;
+            lv_next[at] = lo < n ? (int32_t)post_col<NARROW>(P, first + lo) : INT32_MAX;
+        }
+    }
+}
+
+// Counts of the row's pairs with the chunk's columns [c0, cend) into the
+// byte counters cnt (cleared by the caller), from the levels' positions
+// left by the previous chunk.  Each wave walks whole levels, four at a
+// time: the chunk's postings of a level come first in what is left of it;
+// the first window of the four levels (one or two reads of 64, from the
+// level's postings per remaining chunk, `rest` chunks) is read at once, and
+// a level whose chunk postings fill it goes on alone, four reads at a time.
+// Every thread of the workgroup calls, between barriers.
+template <bool NARROW>
+__device__ void sparse_walk_chunk(uint32_t* cnt, int64_t* lv_cur, int32_t* lv_left, int32_t* lv_next, int nl,
+                                  const PostingQuery& P, int64_t c0, int64_t cend, int32_t rest) {
+    constexpr int kWaves = kSparseBlock / kWave;
+    const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
+    const int32_t c0i = (int32_t)c0, cendi = (int32_t)cend;
+    auto count_col = [&](int32_t c) {
+        if (c >= c0i && c < cendi) {
+            const uint32_t cl = (uint32_t)(c - c0i);
+            atomicAdd(&cnt[cl >> 2], 1u << (8 * (cl & 3)));
+        }
+    };
+    for (int g = w; g < nl; g += 4 * kWaves) {
+        int32_t cv[4][2];
+        bool act[4];
+        int nbf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int l = g + i * kWaves;
+            act[i] = l < nl && lv_next[l < nl ? l : 0] < cendi;
+            const int64_t cur = act[i] ? lv_cur[l] : 0;
+            const int32_t left = act[i] ? lv_left[l] : 0;
+            nbf[i] = left / rest < 40 ? 1 : 2;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int q = u * kWave + lane;
+                cv[i][u] = (act[i] && u < nbf[i] && q < left) ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!act[i]) continue;
+            const int l = g + i * kWaves;
+            int64_t cur = lv_cur[l];
+            int32_t left = lv_left[l];
+            int32_t next = INT32_MAX;
+            bool stop = false;
+            int took = 0;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (u >= nbf[i] || stop) break;
+                count_col(cv[i][u]);
+                const int t = __popcll(__ballot(cv[i][u] < cendi));
+                took += t;
+                if (t < kWave) {
+                    stop = true;
+                    next = __shfl(cv[i][u], t & (kWave - 1));
+                }
+            }
+            cur += took;
+            left -= took;
+            if (!stop && left <= 0) stop = true;
+            while (!stop) {
+                int32_t c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = u * kWave + lane;
+                    c[u] = q < left ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
+                }
+                took = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (stop) break;
+                    count_col(c[u]);
+                    const int t = __popcll(__ballot(c[u] < cendi));
+                    took += t;
+                    if (t < kWave) {
+                        stop = true;
+                        next = __shfl(c[u], t & (kWave - 1));
+                    }
+                }
+                cur += took;
+                left -= took;
+                if (!stop && left <= 0) stop = true;
+            }
+            if (lane == 0) {
+                lv_cur[l] = cur;
+                lv_left[l] = left;
+                lv_next[l] = left > 0 ? next : INT32_MAX;
+            }
+        }
+    }
+}
+
 struct alignas(16) SparseTopkLds {
     alignas(16) uint32_t cnt[kSparseCols / 4];  // per-pair counts of the chunk, one byte each
     uint64_t cand[kSparseTopkCap];              // candidate keys of the row
@@ -2190,7 +2316,6 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
     uint64_t* __restrict__ buf, int32_t* __restrict__ bufcnt, int32_t* __restrict__ out_idx,
     float* __restrict__ out_t) {
     __shared__ SparseTopkLds L;
-    constexpr int kWaves = kSparseBlock / kWave;
     const int64_t row = rb + blockIdx.x;
     const int64_t nchunks = (nq + kSparseCols - 1) / kSparseCols;
     const int64_t y0 = (int64_t)blockIdx.y * cps;
@@ -2214,33 +2339,7 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         L.tu = 1u;
     }
     __syncthreads();
-    {
-        // the row's levels found in the index, each with its postings from
-        // column cstart on (a binary search: the chunks are in order)
-        const int64_t o = R.off[row];
-        const int32_t len = R.len[row];
-        for (int e = threadIdx.x; e < len; e += blockDim.x) {
-            const int2 el = R.elems[o + e];
-            for (int t = 1; t <= el.y; ++t) {
-                int64_t first;
-                int32_t n;
-                // no query molecule has level t, so none has t + 1
-                if (!post_lookup(P, dev_level_key(el.x, (uint32_t)t), first, n)) break;
-                int32_t lo = 0, hi = n;
-                while (cstart > 0 && lo < hi) {
-                    const int32_t mid = (lo + hi) >> 1;
-                    if (post_col<NARROW>(P, first + mid) < cstart)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                const int at = (int)atomicAdd(&L.nlv, 1u);
-                L.lv_cur[at] = first + lo;
-                L.lv_left[at] = n - lo;
-                L.lv_next[at] = lo < n ? (int32_t)post_col<NARROW>(P, first + lo) : INT32_MAX;
-            }
-        }
-    }
+    sparse_row_levels<NARROW>(R, row, P, cstart, L.lv_cur, L.lv_left, L.lv_next, &L.nlv);
     __syncthreads();
     const int nl = (int)L.nlv;
     int64_t c0 = 0;
@@ -2443,13 +2542,6 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         }
         __syncthreads();
     };
-    // adds one to the count of column c when it lies in the chunk
-    auto count_col = [&](int32_t c, int32_t c0i, int32_t cendi) {
-        if (c >= c0i && c < cendi) {
-            const uint32_t cl = (uint32_t)(c - c0i);
-            atomicAdd(&L.cnt[cl >> 2], 1u << (8 * (cl & 3)));
-        }
-    };
     for (int64_t y = y0; y < y1; ++y) {
         c0 = y * kSparseCols;
         const int cw = (int)min((int64_t)kSparseCols, nq - c0);
@@ -2461,83 +2553,8 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         // every thread before any of the chunk's keys is added)
         const uint64_t thr = L.thr;
         const uint32_t imin = L.imin, n0 = L.ncand, ti = L.ti, tu = L.tu;
-        // each wave walks whole levels, four at a time: the chunk's
-        // postings of a level come first in what is left of it; the first
-        // window (one or two reads of 64, from the level's postings per
-        // remaining chunk) of the four levels is read at once, and a level
-        // whose chunk postings fill it goes on alone, four reads at a time
-        const int32_t c0i = (int32_t)c0, cendi = (int32_t)cend;
-        const int32_t rest = (int32_t)max((int64_t)1, y1 - y);
-        for (int g = w; g < nl; g += 4 * kWaves) {
-            int32_t cv[4][2];
-            bool act[4];
-            int nbf[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int l = g + i * kWaves;
-                act[i] = l < nl && L.lv_next[l < nl ? l : 0] < cendi;
-                const int64_t cur = act[i] ? L.lv_cur[l] : 0;
-                const int32_t left = act[i] ? L.lv_left[l] : 0;
-                nbf[i] = left / rest < 40 ? 1 : 2;
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int q = u * kWave + lane;
-                    cv[i][u] = (act[i] && u < nbf[i] && q < left) ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (!act[i]) continue;
-                const int l = g + i * kWaves;
-                int64_t cur = L.lv_cur[l];
-                int32_t left = L.lv_left[l];
-                int32_t next = INT32_MAX;
-                bool stop = false;
-                int took = 0;
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    if (u >= nbf[i] || stop) break;
-                    count_col(cv[i][u], c0i, cendi);
-                    const int t = __popcll(__ballot(cv[i][u] < cendi));
-                    took += t;
-                    if (t < kWave) {
-                        stop = true;
-                        next = __shfl(cv[i][u], t & (kWave - 1));
-                    }
-                }
-                cur += took;
-                left -= took;
-                if (!stop && left <= 0) stop = true;
-                while (!stop) {
-                    int32_t c[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int q = u * kWave + lane;
-                        c[u] = q < left ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
-                    }
-                    took = 0;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (stop) break;
-                        count_col(c[u], c0i, cendi);
-                        const int t = __popcll(__ballot(c[u] < cendi));
-                        took += t;
-                        if (t < kWave) {
-                            stop = true;
-                            next = __shfl(c[u], t & (kWave - 1));
-                        }
-                    }
-                    cur += took;
-                    left -= took;
-                    if (!stop && left <= 0) stop = true;
-                }
-                if (lane == 0) {
-                    L.lv_cur[l] = cur;
-                    L.lv_left[l] = left;
-                    L.lv_next[l] = left > 0 ? next : INT32_MAX;
-                }
-            }
-        }
+        sparse_walk_chunk<NARROW>(L.cnt, L.lv_cur, L.lv_left, L.lv_next, nl, P, c0, cend,
+                                  (int32_t)max((int64_t)1, y1 - y));
         __syncthreads();
         chunk_keys(thr, imin, ti, tu);
         __syncthreads();
@@ -2642,6 +2659,114 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
             ++i;
         }
     }
+}
+
+// Sparse 1000-bin histogram: pairs with I <= kIMax and U < kIU are counted
+// by (I, U) (their T is a function of the two), the rest into their bins.
+struct alignas(16) SparseHistLds {
+    alignas(16) uint32_t cnt[kSparseCols / 4];
+    int64_t lv_cur[kSparseMaxLevels];
+    int32_t lv_left[kSparseMaxLevels];
+    int32_t lv_next[kSparseMaxLevels];
+    uint32_t bins[1000];
+    float edges[1001];
+    uint32_t iu[EpiHist1000<1>::kIMax * EpiHist1000<1>::kIU];
+    uint32_t nlv;
+};
+
+// The 1000-bin int64 histogram of rows [rb, rb + gridDim.x) (grouped rows
+// of at most 255 levels, magnitudes at least the count sums) against every
+// query molecule, from the inverted index: workgroup (x, s) takes row
+// rb + x and the chunks [s cps, (s + 1) cps), counts each chunk's pairs in
+// LDS as the sparse top-k does, then bins every pair of the chunk: a count
+// of 0 is T = 0 (bin 0), any other pair gets T from the same float
+// expression as every kernel.  Counts by (I, U) as in the bit-sliced
+// histogram epilogue keep most LDS atomics off the few bins such pairs
+// share; the workgroup adds its bins to `hist` at the end.
+template <bool NARROW, int MB>
+__global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sparse_hist1000(
+    RowArgs R, int64_t rb, PostingQuery P, const void* __restrict__ cmagn, int64_t nq, int64_t cps,
+    unsigned long long* __restrict__ hist) {
+    constexpr int kIU = EpiHist1000<1>::kIU, kIMax = EpiHist1000<1>::kIMax;
+    __shared__ SparseHistLds L;
+    const int64_t row = rb + blockIdx.x;
+    const int64_t nchunks = (nq + kSparseCols - 1) / kSparseCols;
+    const int64_t y0 = (int64_t)blockIdx.y * cps;
+    const int64_t y1 = min(nchunks, y0 + cps);
+    const int ma = R.mag[row];
+    const float maf = (float)ma;
+    for (int i = threadIdx.x; i < 1000; i += blockDim.x) L.bins[i] = 0u;
+    for (int i = threadIdx.x; i < kIMax * kIU; i += blockDim.x) L.iu[i] = 0u;
+    stage_edges(L.edges);
+    if (threadIdx.x == 0) L.nlv = 0u;
+    __syncthreads();
+    sparse_row_levels<NARROW>(R, row, P, y0 * kSparseCols, L.lv_cur, L.lv_left, L.lv_next, &L.nlv);
+    __syncthreads();
+    const int nl = (int)L.nlv;
+    uint32_t zero = 0u;  // this thread's pairs with T < fl32(1/1000)
+    for (int64_t y = y0; y < y1; ++y) {
+        const int64_t c0 = y * kSparseCols;
+        const int cw = (int)min((int64_t)kSparseCols, nq - c0);
+        for (int i = threadIdx.x; i < kSparseCols / 16; i += blockDim.x)
+            reinterpret_cast<uint4*>(L.cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+        sparse_walk_chunk<NARROW>(L.cnt, L.lv_cur, L.lv_left, L.lv_next, nl, P, c0, c0 + cw,
+                                  (int32_t)max((int64_t)1, y1 - y));
+        __syncthreads();
+#pragma unroll 1
+        for (int i = 0; i < kSparseCols / 16 / kSparseBlock; ++i) {
+            const int q = (int)threadIdx.x + i * kSparseBlock;
+            const int nv = min(16, cw - 16 * q);  // columns of the block in the chunk
+            if (nv <= 0) continue;
+            const uint4 v = reinterpret_cast<const uint4*>(L.cnt)[q];
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+            // pairs with a count of 0: T = 0
+            uint32_t pos = 0u;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) pos += (uint32_t)__popc(bytes_ge(wv[h], 1u));
+            zero += (uint32_t)nv - pos;
+            if (pos == 0u) continue;
+            uint32_t mw[4 * MB];
+            {
+                const uint4* mp = reinterpret_cast<const uint4*>(static_cast<const char*>(cmagn) + MB * (c0 + 16 * q));
+#pragma unroll
+                for (int u = 0; u < MB; ++u) {
+                    const uint4 m = mp[u];
+                    mw[4 * u] = m.x;
+                    mw[4 * u + 1] = m.y;
+                    mw[4 * u + 2] = m.z;
+                    mw[4 * u + 3] = m.w;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t isct = (wv[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                if (isct == 0u) continue;
+                const uint32_t mb = MB == 1 ? (mw[j >> 2] >> (8 * (j & 3))) & 0xFFu
+                                            : (mw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+                const uint32_t u = (uint32_t)ma + mb - isct;
+                if (isct <= (uint32_t)kIMax && u < (uint32_t)kIU) {
+                    atomicAdd(&L.iu[(isct - 1) * kIU + u], 1u);
+                } else {
+                    const float t = pair_tanimoto<true>((float)isct, maf, (float)mb, ma == 0 || mb == 0u);
+                    if (!(t >= 1.0f / 1000.0f))
+                        ++zero;
+                    else
+                        atomicAdd(&L.bins[bin1000(t, L.edges)], 1u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // (I, U) counts to their bins: T = (float)I / (float)U, both exact
+    // integers (short_div equals IEEE division there)
+    zero = wave_sum_u64(zero);
+    if (lane_id() == 0 && zero) atomicAdd(&L.bins[0], zero);
+    for (int k = threadIdx.x; k < kIMax * kIU; k += blockDim.x)
+        if (L.iu[k]) atomicAdd(&L.bins[bin1000((float)(k / kIU + 1) / (float)(k % kIU), L.edges)], L.iu[k]);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 1000; k += blockDim.x)
+        if (L.bins[k]) atomicAdd(&hist[k], (unsigned long long)L.bins[k]);
 }
 
 // gpuHistogrammedLingo_multirow (kernels.py:606-667): fused Tanimoto and
@@ -5325,6 +5450,45 @@ static int topk_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int
     return topk_impl(ctx, rowbase, rowlimit, k, nullptr);
 }
 
+// The 1000-bin histogram of rows [rb, rl) on the sparse path (every row
+// grouped with at most 255 levels, magnitudes at least the count sums),
+// column ranges split as the sparse top-k's; into the handle's result.
+static int sparse_hist1000(siml_ctx* ctx, int64_t rb, int64_t rl, unsigned long long* hist) {
+    const int64_t nr = rl - rb, nq = ctx->qry->n;
+    const ColSet::Postings& X = ctx->qry->post;
+    const PostingQuery P = X.query();
+    const SparseTopkShape shp = sparse_topk_shape(nr, nq, 1);
+    GraphRegion gr(ctx);
+    TRY(gr.begin());
+    k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
+    TRY(launch_check("k_fill"));
+    TimedRegion tr(ctx, nr * nq, !gr.on);
+    const dim3 grid((unsigned)nr, (unsigned)shp.nsplit);
+    auto go = [&](auto kern, const void* mags) {
+        kern<<<grid, dim3(kSparseBlock), 0, ctx->stream>>>(ctx->ref.args(), rb, P, mags, nq, shp.cps, hist);
+    };
+    const void* m8 = X.cmag8.p;
+    const void* m16 = X.cmag16.p;
+    if (X.narrow && X.mag_bytes == 1)
+        go(k_sparse_hist1000<true, 1>, m8);
+    else if (X.narrow)
+        go(k_sparse_hist1000<true, 2>, m16);
+    else if (X.mag_bytes == 1)
+        go(k_sparse_hist1000<false, 1>, m8);
+    else
+        go(k_sparse_hist1000<false, 2>, m16);
+    TRY(launch_check("k_sparse_hist1000"));
+    tr.done();
+    if (gr.on) {
+        TRY(gr.end());
+        TimedRegion tg(ctx, nr * nq);
+        TRY(gr.launch());
+        tg.done();
+    }
+    ctx->last_kernel = "hist1000_sparse";
+    return SIML_OK;
+}
+
 static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit) {
     TRY(check_rows(ctx, rowbase, rowlimit, "getGlobalHistogram"));
     TRY(set_device(ctx));
@@ -5342,6 +5506,16 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     BitPlan bp;
     TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     const std::vector<int32_t>& mrows = bp.merge;
+    if (bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small && nq <= 0x7FFFFFFF &&
+        nr <= 0x7FFFFFFF) {
+        // the sparse path when every row can take it and the model (or the
+        // policy) prefers it
+        std::vector<int32_t> srows, smerge;
+        int prc = SIML_OK;
+        const bool sp = plan_sparse(ctx, rowbase, rowlimit, bp, srows, smerge, &prc, true);
+        TRY(prc);
+        if (sp && smerge.empty() && ctx->qry->post.mag_bytes != 0) return sparse_hist1000(ctx, rowbase, rowlimit, hist);
+    }
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     if (bp.use && ctx->persistent) TRY(ctx->scratch.reserve(256));  // persistent grid's queue counters
     // the bins are cleared by a kernel, which a graph may hold
