@@ -779,16 +779,27 @@ def workload_threshold(args):
                       {"workload": f"threshold neighbours T >= {lo} + 1000-bin histogram: {nq} near-duplicate "
                                    f"queries x {ndb}-molecule resident database", "queries": nq, "database": ndb,
                        "seed": 4})
-    # the histogram evaluates every pair: its probes are every group against
-    # every database level; the neighbour search prunes tiles by magnitude
-    hist_alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8000.0
+    if k_hist == "hist1000_sparse":
+        # sparse path: the postings the query rows' levels meet (4 B each;
+        # estimated from the first tenth of the database, whose molecules
+        # are drawn like the rest), a 16-byte index slot per query level and
+        # the bins
+        tenth = max(ndb // 10, 1)
+        met = postings_met_ragged(q, synth.ragged_slice(*db_set, 0, tenth)) * (ndb / tenth)
+        hist_alg = 4.0 * met + 16.0 * lv_q.sum() + 8000.0
+        hist_roof = roofline(hist_alg, None, hist_s, peaks, recorded_counters("k_sparse_hist1000_db10M"))
+        hist_roof["postings_met_estimate"] = met
+    else:
+        # the histogram evaluates every pair: its probes are every group
+        # against every database level
+        hist_alg = 8.0 * lv_db.sum() + 16.0 * lv_q.sum() + 8000.0
+        hist_roof = roofline(hist_alg, 16.0 * groups * lv_db.sum(), hist_s, peaks,
+                             recorded_counters("k_bits_hist1000_db10M"))
     rec.update({"csr": {"wall_ms": t_csr * 1e3, "pairs_per_s": pairs / t_csr, "kernel_ms": kms_csr / args.steps,
                         "kernel": k_csr, "evaluated_pairs_per_call": ev_pairs / args.steps,
                         "evaluated_frac": ev_pairs / args.steps / pairs},
                 "hist1000": {"wall_ms": t_hist * 1e3, "pairs_per_s": pairs / t_hist,
-                             "kernel_ms": kms_hist / args.steps, "kernel": k_hist,
-                             "roofline": roofline(hist_alg, 16.0 * groups * lv_db.sum(), hist_s, peaks,
-                                                  recorded_counters("k_bits_hist1000_db10M"))},
+                             "kernel_ms": kms_hist / args.steps, "kernel": k_hist, "roofline": hist_roof},
                 "checks": checks})
     rec["roofline"] = rec["hist1000"]["roofline"]
     db.close()
@@ -952,7 +963,7 @@ def workload_default(args):
     rec["batched"] = batched_part(args, peaks=peaks)
     log(f"batched part: {time.perf_counter() - t0:.0f} s")
     t0 = time.perf_counter()
-    rec["serving"] = serving_part(args, [30.0, 1000.0, 2000.0, 2500.0],
+    rec["serving"] = serving_part(args, [30.0, 1000.0, 2500.0, 5000.0, 10000.0],
                                   lambda r: int(min(600, max(120, 3 * r))))
     log(f"serving part: {time.perf_counter() - t0:.0f} s")
     return rec

@@ -234,12 +234,15 @@ int siml_kernel_split(siml_ctx* ctx, double* similarity_ms, double* selection_ms
 int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset);
 /* Kernel policy: AUTO serves sorted-unique sets with the bit-sliced
  * multi-row kernels (rows that do not qualify, and unsorted sets, use the
- * merge kernel) and, for the float32 matrix (multirow), chooses per call
- * between the bit-sliced kernel and the sparse path (an inverted index of
- * the query set's levels, built on first use, and per-pair counts in LDS)
- * from the index's expected postings per level; BITS never takes the sparse
- * path; SPARSE takes it whenever the query set has an index; MERGE uses the
- * merge kernel for everything.  All give the same bits.  The bit-sliced row
+ * merge kernel) and, for the float32 matrix (multirow), the top-k and the
+ * 1000-bin histogram, chooses per call between the bit-sliced kernels and
+ * the sparse path (an inverted index of the query set's levels, built on
+ * first use, and per-pair counts in LDS) from the index's expected postings
+ * per level; the sparse top-k and histogram serve calls whose rows all
+ * qualify and whose magnitudes are at least the count sums (the
+ * allocation-free top-k never takes them).  BITS never takes the sparse
+ * path; SPARSE takes it whenever the call can; MERGE uses the merge kernel
+ * for everything.  All give the same bits.  The bit-sliced row
  * groups are built by set_ref when the policy in force then is not MERGE; a
  * reference set given under MERGE is served by the merge kernel. */
 #define SIML_POLICY_AUTO 0

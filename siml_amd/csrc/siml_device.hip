@@ -5000,13 +5000,14 @@ int ensure_postings(siml_ctx* ctx) {
 // 0.72 per pair plus 0.0112 per pair and query level (1.57 and 1.11 ms).
 constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.30;
 constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
-// Top-k (no matrix to write), measured on the top-100 of 257684 lognormal
-// strings over 1M (0.09 postings met per pair, 45 levels per molecule):
-// the sparse kernel takes 0.51 per pair (clearing and scanning the counts,
-// its per-chunk steps) plus 0.30 per posting met (138 ms), the seeded
-// bit-sliced passes 0.84 per pair plus 0.0112 per pair and query level
-// (347 ms).
-constexpr double kSparseTopkPsPerPair = 0.51, kBitsTopkPsPerPair = 0.84;
+// Top-k and the 1000-bin histogram (no matrix to write), measured on the
+// top-100 of 257684 lognormal strings over 1M (0.09 postings met per pair,
+// 45 levels per molecule): the sparse kernel takes 0.45 per pair (clearing
+// and scanning the counts, its per-chunk steps) plus 0.30 per posting met
+// (123 ms), the seeded bit-sliced passes 0.84 per pair plus 0.0112 per
+// pair and query level (347 ms); the histograms of 1024 rows over 10M take
+// 8.0 and 13.7 ms.
+constexpr double kSparseTopkPsPerPair = 0.45, kBitsTopkPsPerPair = 0.84;
 
 // Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
 // levels) and the rest (merge kernel), when the sparse path applies to the
