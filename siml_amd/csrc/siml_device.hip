@@ -2360,10 +2360,13 @@ __device__ __forceinline__ void sparse_chunk_store(SparseChunkLds& W, const int6
 // Counts of the row's pairs with the chunk of columns [c0, c0 + 32768)
 // into the byte counters cnt (cleared by the caller), from W (each level's
 // first posting and posting count of the chunk, stored before the barrier
-// that precedes the call): the threads take the chunk's postings of all
-// levels in turn (a thread finds a posting's level by a binary search of
-// the prefix of the counts), four reads in flight per thread.  Every
-// thread of the workgroup calls, between barriers; it has barriers.
+// that precedes the call): each wave takes an eighth of the chunk's
+// postings of all levels in order, 64 consecutive ones per read and four
+// reads in flight, a lane's level advancing from the previous read's last
+// one (a binary search of the prefix of the counts places the wave's first
+// posting; one per posting measured 123 ms against 110 on the batched
+// top-100).  Every thread of the workgroup calls, between barriers; it has
+// barriers.
 template <bool NARROW>
 __device__ void sparse_walk_chunk(uint32_t* cnt, SparseChunkLds& W, int nl, const PostingQuery& P, int64_t c0) {
     const int lane = lane_id();
@@ -2388,23 +2391,40 @@ __device__ void sparse_walk_chunk(uint32_t* cnt, SparseChunkLds& W, int nl, cons
     __syncthreads();
     const int32_t total = W.pre[nl];
     const int32_t c0i = (int32_t)c0;
-    for (int32_t g0 = 0; g0 < total; g0 += 4 * (int32_t)blockDim.x) {
+    // wave w takes postings [s0, s1) of the chunk, 64 consecutive ones per
+    // read; a lane's level advances from the previous read's last level
+    constexpr int kWaves = kSparseBlock / kWave;
+    const int w = (int)(threadIdx.x / kWave);
+    const int32_t per = (total + kWaves - 1) / kWaves;
+    const int32_t s0 = min(total, w * per), s1 = min(total, s0 + per);
+    if (s0 >= s1) return;
+    int lv = 0;
+    {
+        int lo = 0, hi = nl - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (W.pre[mid] <= s0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        lv = lo;
+    }
+    for (int32_t g0 = s0; g0 < s1; g0 += 4 * kWave) {
         int32_t c[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int32_t g = g0 + u * (int32_t)blockDim.x + (int32_t)threadIdx.x;
+            const int32_t g = g0 + u * kWave + lane;
             c[u] = -1;
-            if (g < total) {
-                // the last level whose prefix is at most g
-                int lo = 0, hi = nl - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (W.pre[mid] <= g)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                c[u] = (int32_t)post_col<NARROW>(P, W.st[lo] + (g - W.pre[lo]));
+            int l = lv;
+            if (g < s1) {
+                while (W.pre[l + 1] <= g) ++l;
+                c[u] = (int32_t)post_col<NARROW>(P, W.st[l] + (g - W.pre[l]));
+            }
+            // the level of the read's last posting
+            if (g0 + u * kWave < s1) {
+                const int32_t glast = min(s1 - 1, g0 + u * kWave + kWave - 1);
+                lv = __shfl(l, glast - g0 - u * kWave);
             }
         }
 #pragma unroll
@@ -5002,12 +5022,12 @@ constexpr double kSparsePsPerPair = 0.69, kSparsePsPerPosting = 0.30;
 constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
 // Top-k and the 1000-bin histogram (no matrix to write), measured on the
 // top-100 of 257684 lognormal strings over 1M (0.09 postings met per pair,
-// 45 levels per molecule): the sparse kernel takes 0.45 per pair (clearing
+// 45 levels per molecule): the sparse kernel takes 0.40 per pair (clearing
 // and scanning the counts, its per-chunk steps) plus 0.30 per posting met
-// (123 ms), the seeded bit-sliced passes 0.84 per pair plus 0.0112 per
+// (110 ms), the seeded bit-sliced passes 0.84 per pair plus 0.0112 per
 // pair and query level (347 ms); the histograms of 1024 rows over 10M take
 // 8.0 and 13.7 ms.
-constexpr double kSparseTopkPsPerPair = 0.45, kBitsTopkPsPerPair = 0.84;
+constexpr double kSparseTopkPsPerPair = 0.40, kBitsTopkPsPerPair = 0.84;
 
 // Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
 // levels) and the rest (merge kernel), when the sparse path applies to the
