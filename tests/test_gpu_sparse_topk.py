@@ -166,3 +166,21 @@ def test_long_columns_two_byte_magnitudes():
     assert g._h.last_kernel() == "topk_sparse"
     rows = np.arange(0, 2500, 9)
     same(tuple(x[rows] for x in got), oracle.topk(tuple(x[rows] for x in ref), qry, 100, nthreads=0))
+
+
+@pytest.mark.parametrize("nq", [32768, 65536, 65537])
+def test_chunk_and_posting_width_boundaries(nq):
+    """One full chunk; two full chunks with uint16 postings (the widest set
+    they hold); int32 postings with a last chunk of one column -- top-k and
+    the histogram, in one launch shape and split over column ranges."""
+    qry = synth.packed_padded(nq, 10, 120, seed=171)
+    for nr in (12, 2100):
+        ref = synth.packed_padded(nr, 10, 120, seed=172 + nr)
+        g = handle(_lib.POLICY_SPARSE, ref, qry)
+        rows = np.arange(0, nr, max(1, nr // 12))
+        got = g.getTopK(0, nr, 64)
+        assert g._h.last_kernel() == "topk_sparse"
+        same(tuple(x[rows] for x in got), oracle.topk(tuple(x[rows] for x in ref), qry, 64, nthreads=0))
+        h = g.getGlobalHistogram(0, 12)
+        assert g._h.last_kernel() == "hist1000_sparse"
+        assert np.array_equal(h, oracle.hist1000(tuple(x[:12] for x in ref), qry))
