@@ -176,7 +176,9 @@ int siml_hist1000_async(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit);
  * Every call on a GPU shares that GPU's array: calls that overlap on one
  * GPU (from any handle or stream) race.  The autotuner and kernel timing
  * are not used.  The sets must already be on the device (set_ref /
- * set_query / attach_db allocate). */
+ * set_query / attach_db allocate); the bit-sliced row groups serve the call
+ * when an earlier call built them (building allocates), else the merge
+ * kernel does, with the same results. */
 #define SIML_NOALLOC_SCRATCH_BYTES (128u << 20)
 int siml_topk_noalloc(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k, int32_t* idx, float* t);
 
@@ -243,8 +245,9 @@ int siml_work_stats(siml_ctx* ctx, int64_t* pairs, int64_t* merge_len, int reset
  * allocation-free top-k never takes them).  BITS never takes the sparse
  * path; SPARSE takes it whenever the call can; MERGE uses the merge kernel
  * for everything.  All give the same bits.  The bit-sliced row
- * groups are built by set_ref when the policy in force then is not MERGE; a
- * reference set given under MERGE is served by the merge kernel. */
+ * groups serve a reference set given while the policy is not MERGE; they
+ * are built by the first call that needs them (a reference set given under
+ * MERGE is served by the merge kernel). */
 #define SIML_POLICY_AUTO 0
 #define SIML_POLICY_MERGE 1
 #define SIML_POLICY_BITS 2
@@ -301,7 +304,9 @@ int siml_launch_shape(siml_ctx* ctx, int64_t* target_wg, int32_t* topk_wave_cap,
 int siml_set_launch_shape(siml_ctx* ctx, int64_t target_wg, int32_t topk_wave_cap, int32_t topk_seed);
 /* Bit-sliced row groups of the current reference set (no counterpart in
  * the reference): the number of groups (each probes every level of every
- * query molecule once per launch) and of rows left to the merge kernel. */
+ * query molecule once per launch) and of rows left to the merge kernel;
+ * both 0 while no call has needed the groups (they are built on first use:
+ * a call that takes a sparse path builds none). */
 int siml_row_groups(siml_ctx* ctx, int64_t* groups, int64_t* merge_rows);
 /* Most row groups one bit-sliced workgroup serves (no counterpart in the
  * reference): 1, 2 or 4 (default 2).  A workgroup serving G groups holds G

@@ -2516,12 +2516,12 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
         const uint32_t at = base + (uint32_t)rank_below(m);
         if (valid && at < (uint32_t)kSparseTopkCap) L.cand[at] = key;
     };
-    // The chunk's pairs whose count reaches imin go to the wave's queue
-    // (a lane tests 16 counts at a time, a byte mask per word packed into
-    // one mask, and queues those that reach imin one at a time); each 64 queued pairs get their
-    // magnitude, and those whose ratio beats ti / tu (every one when
-    // ti = 0) their T and key; valid lanes (T > 0, key > thr) are pushed.
-    // Every lane of the workgroup calls.
+    // The chunk's pairs whose count reaches imin go to the wave's queue (a
+    // lane tests 16 counts at a time, a byte mask per word packed into one
+    // mask, and queues those that reach imin one at a time); each 64 queued
+    // pairs get their magnitude, and those whose ratio beats ti / tu (every
+    // one when ti = 0) their T and key; valid lanes (T > 0, key > thr) are
+    // pushed.  Every lane of the workgroup calls.
     auto chunk_keys = [&](uint64_t thr, uint32_t imin, uint32_t ti, uint32_t tu) {
         int qn = 0;  // queued pairs (wave-uniform)
         auto flush = [&](int n) {
@@ -3330,9 +3330,13 @@ struct RowSet {
     DevBuf elems, off, len, mag, csum;
     std::vector<int32_t> h_len;
     bool mags_small = false;
-    // bit-sliced row groups: [0] consecutive rows (built by set_ref under the
-    // AUTO policy), [1] rows ordered by magnitude (built on first use by the
-    // neighbour search, whose tile pruning needs narrow magnitude bands)
+    // row groups may serve this set (the policy was not MERGE at set_ref);
+    // per row, whether the sparse paths can take it
+    bool groups_ok = false;
+    std::vector<uint8_t> h_sparse_ok;
+    // bit-sliced row groups, built on first use: [0] consecutive rows, [1]
+    // rows ordered by magnitude (the neighbour search, whose tile pruning
+    // needs narrow magnitude bands)
     struct BitGrouping {
         bool built = false;
         DevBuf hdr, rows, ent;
@@ -3372,6 +3376,8 @@ struct RowSet {
         h_mag.clear();
         h_csum.clear();
         h_start.clear();
+        h_sparse_ok.clear();
+        groups_ok = false;
         n = -1;
     }
 };
@@ -3825,7 +3831,25 @@ int upload_rows(siml_ctx* ctx, const HostSet& s) {
     R.maxlen = info.maxlen;
     R.sorted_unique = info.sorted_unique;
     R.h_len = std::move(info.len);
-    if (ctx->policy != SIML_POLICY_MERGE) TRY(ensure_bits(ctx, 0));
+    // the sparse paths take a row with strictly ascending ASCII ids, counts
+    // of at most 15 (the index's levels) and at most 255 levels (byte
+    // counters); row groups are built on first use
+    R.groups_ok = ctx->policy != SIML_POLICY_MERGE;
+    R.h_sparse_ok.assign((size_t)s.n, 0);
+    parallel_for(s.n, [&](int64_t a, int64_t b) {
+        for (int64_t r = a; r < b; ++r) {
+            const int32_t* id = R.h_ids.data() + R.h_start[(size_t)r];
+            const int32_t* ct = R.h_counts.data() + R.h_start[(size_t)r];
+            const int32_t L = R.h_len[(size_t)r];
+            int64_t lv = 0;
+            bool ok = true;
+            for (int32_t j = 0; j < L && ok; ++j) {
+                ok = siml::ascii_id(id[j]) && ct[j] >= 0 && ct[j] <= siml::kBitMaxLevel && (j == 0 || id[j] > id[j - 1]);
+                lv += ct[j];
+            }
+            R.h_sparse_ok[(size_t)r] = ok && lv <= kSparseMaxLevels ? 1 : 0;
+        }
+    });
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SIML_OK;
 }
@@ -4027,11 +4051,16 @@ struct BitPlan {
 // another policy.  The group list goes to the handle's buffer, or to
 // glist_dst (room for every group of the grouping plus kMaxTables) when
 // given.
-int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false, int32_t* glist_dst = nullptr) {
+// The bit-sliced plan of rows [rb, rl): the groups holding them (the
+// grouping is built on first use, except with no_build, when a grouping not
+// built yet leaves the call to the merge kernel) and the rows no group holds.
+int plan_bits(siml_ctx* ctx, int64_t rb, int64_t rl, BitPlan& p, bool by_mag = false, int32_t* glist_dst = nullptr,
+              bool no_build = false) {
     RowSet& R = ctx->ref;
     p = BitPlan();
-    if (ctx->policy == SIML_POLICY_MERGE || !ctx->qry->levels_ok || !R.bits[0].built) return SIML_OK;
+    if (ctx->policy == SIML_POLICY_MERGE || !ctx->qry->levels_ok || !R.groups_ok) return SIML_OK;
     p.which = by_mag ? 1 : 0;
+    if (no_build && !R.bits[p.which].built) return SIML_OK;
     TRY(ensure_bits(ctx, p.which));
     const RowSet::BitGrouping& B = R.bits[p.which];
     p.use = true;
@@ -5029,27 +5058,28 @@ constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
 // 8.0 and 13.7 ms.
 constexpr double kSparseTopkPsPerPair = 0.40, kBitsTopkPsPerPair = 0.84;
 
-// Rows of [rb, rl) for the sparse kernel (grouped rows of at most 255
-// levels) and the rest (merge kernel), when the sparse path applies to the
-// call: policy SPARSE, or AUTO with few expected postings.
-bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, const BitPlan& bp, std::vector<int32_t>& sparse_rows,
+// Rows of [rb, rl) for the sparse kernels (rows with ascending ASCII ids,
+// counts of at most 15 and at most 255 levels) and the rest, when the sparse
+// path applies to the call: policy SPARSE, or AUTO with few expected
+// postings.  Decided before any row-group planning, so a call that takes
+// the sparse path never builds the row groups.
+bool plan_sparse(siml_ctx* ctx, int64_t rb, int64_t rl, std::vector<int32_t>& sparse_rows,
                  std::vector<int32_t>& merge_rows, int* rc, bool topk = false) {
     *rc = SIML_OK;
-    if (!bp.use || (ctx->policy != SIML_POLICY_AUTO && ctx->policy != SIML_POLICY_SPARSE)) return false;
+    const RowSet& R = ctx->ref;
+    if (!R.groups_ok || !ctx->qry->levels_ok ||
+        (ctx->policy != SIML_POLICY_AUTO && ctx->policy != SIML_POLICY_SPARSE))
+        return false;
     *rc = ensure_postings(ctx);
     if (*rc != SIML_OK) return false;
     const ColSet::Postings& X = ctx->qry->post;
     // the sparse kernel packs a level's posting count in 28 bits
     if (!X.ok || ctx->qry->n >= (int64_t(1) << 28)) return false;
-    const RowSet& R = ctx->ref;
     sparse_rows.clear();
     merge_rows.clear();
     double levels = 0.0;
-    size_t m = 0;
     for (int64_t r = rb; r < rl; ++r) {
-        while (m < bp.merge.size() && bp.merge[m] < r) ++m;
-        const bool grouped = !(m < bp.merge.size() && bp.merge[m] == r);
-        if (grouped && R.h_csum[(size_t)r] <= kSparseMaxLevels) {
+        if (R.h_sparse_ok[(size_t)r]) {
             sparse_rows.push_back((int32_t)r);
             levels += (double)R.h_csum[(size_t)r];
         } else {
@@ -5162,11 +5192,9 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
     ctx->dpitch = dp;
     if (nr == 0 || nq == 0) return SIML_OK;
     const size_t lds = row_lds_bytes(max_len_in(ctx->ref, rowbase, rowlimit));
-    BitPlan bp;
-    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     std::vector<int32_t> srows, mrows;
     int prc = SIML_OK;
-    const bool sparse = plan_sparse(ctx, rowbase, rowlimit, bp, srows, mrows, &prc);
+    const bool sparse = plan_sparse(ctx, rowbase, rowlimit, srows, mrows, &prc);
     TRY(prc);
     if (sparse) {
         // one list: the sparse kernel's rows, then the merge kernel's
@@ -5205,6 +5233,8 @@ static int multirow_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
         ctx->last_kernel = mrows.empty() ? "rows_sparse" : (srows.empty() ? "rows_merge" : "rows_sparse+merge");
         return SIML_OK;
     }
+    BitPlan bp;
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
     if (bp.use) {
         if (!bp.merge.empty()) TRY(upload_rowlist(ctx, bp.merge));
         TimedRegion tr(ctx, nr * nq);
@@ -5462,6 +5492,15 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
         ctx->cols = k;
     }
     if (nr == 0) return SIML_OK;
+    if (!mem && nq > 0 && nq <= 0x7FFFFFFF && nr <= 0x7FFFFFFF && ctx->ref.mags_small && ctx->qry->mags_small) {
+        // the sparse path when every row can take it and the model (or the
+        // policy) prefers it
+        std::vector<int32_t> srows, smerge;
+        int prc = SIML_OK;
+        const bool sp = plan_sparse(ctx, rowbase, rowlimit, srows, smerge, &prc, true);
+        TRY(prc);
+        if (sp && smerge.empty() && ctx->qry->post.mag_bytes != 0) return sparse_topk(ctx, rowbase, rowlimit, k);
+    }
     auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
     // caller memory: the group list and the row list first
     size_t used = 0;
@@ -5476,8 +5515,10 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
             return fail(SIML_ELIMIT, "getTopK: the row and group lists of %lld rows need %zu bytes of scratch, more "
                         "than the %zu available", (long long)nr, used, mem->cap);
     }
+    // caller memory: row groups only when an earlier call built them (their
+    // build allocates), else the merge kernel
     BitPlan bp;
-    TRY(plan_bits(ctx, rowbase, rowlimit, bp, false, glist_dst));
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp, false, glist_dst, mem != nullptr));
     const std::vector<int32_t>& mrows = bp.merge;
     const int32_t* d_rowlist = nullptr;
     if (!mrows.empty()) {
@@ -5496,15 +5537,6 @@ static int topk_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit, int32_t k
     // [0, 1] for every pair, so only T > 0 pairs need candidate buffers (the
     // final selection fills up with T == 0 columns in index order)
     const bool pos_only = bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small;
-    if (pos_only && !mem && nq > 0 && nq <= 0x7FFFFFFF && nr <= 0x7FFFFFFF) {
-        // the sparse path when every row can take it and the model (or the
-        // policy) prefers it
-        std::vector<int32_t> srows, smerge;
-        int prc = SIML_OK;
-        const bool sp = plan_sparse(ctx, rowbase, rowlimit, bp, srows, smerge, &prc, true);
-        TRY(prc);
-        if (sp && smerge.empty() && ctx->qry->post.mag_bytes != 0) return sparse_topk(ctx, rowbase, rowlimit, k);
-    }
     static_assert(((SIML_MAX_TOPK + kTopkSlack) + kWave - 1) / kWave <= kTopkRegs,
                   "top-k buffer exceeds wave registers");
     const int gt_plan = bp.use ? bits_tables(ctx, bp) : 1;
@@ -5699,19 +5731,18 @@ static int hist1000_async_impl(siml_ctx* ctx, int64_t rowbase, int64_t rowlimit)
         k_fill<unsigned long long><<<dim3(4), dim3(256), 0, ctx->stream>>>(hist, 1000, 0ull);
         return launch_check("k_fill");
     }
-    BitPlan bp;
-    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
-    const std::vector<int32_t>& mrows = bp.merge;
-    if (bp.use && mrows.empty() && ctx->ref.mags_small && ctx->qry->mags_small && nq <= 0x7FFFFFFF &&
-        nr <= 0x7FFFFFFF) {
+    if (ctx->ref.mags_small && ctx->qry->mags_small && nq <= 0x7FFFFFFF && nr <= 0x7FFFFFFF) {
         // the sparse path when every row can take it and the model (or the
         // policy) prefers it
         std::vector<int32_t> srows, smerge;
         int prc = SIML_OK;
-        const bool sp = plan_sparse(ctx, rowbase, rowlimit, bp, srows, smerge, &prc, true);
+        const bool sp = plan_sparse(ctx, rowbase, rowlimit, srows, smerge, &prc, true);
         TRY(prc);
         if (sp && smerge.empty() && ctx->qry->post.mag_bytes != 0) return sparse_hist1000(ctx, rowbase, rowlimit, hist);
     }
+    BitPlan bp;
+    TRY(plan_bits(ctx, rowbase, rowlimit, bp));
+    const std::vector<int32_t>& mrows = bp.merge;
     if (!mrows.empty()) TRY(upload_rowlist(ctx, mrows));
     if (bp.use && ctx->persistent) TRY(ctx->scratch.reserve(256));  // persistent grid's queue counters
     // the bins are cleared by a kernel, which a graph may hold
@@ -6313,7 +6344,7 @@ int siml_row_groups(siml_ctx* ctx, int64_t* groups, int64_t* merge_rows) {
     if (!ctx || !groups || !merge_rows) return fail(SIML_EINVAL, "null pointer");
     const RowSet::BitGrouping& B = ctx->ref.bits[0];
     *groups = B.built ? (int64_t)B.h_hdr.size() : 0;
-    *merge_rows = B.built ? (int64_t)B.merge_rows.size() : std::max<int64_t>(ctx->ref.n, 0);
+    *merge_rows = B.built ? (int64_t)B.merge_rows.size() : 0;
     return SIML_OK;
 }
 
