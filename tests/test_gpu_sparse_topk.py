@@ -15,6 +15,14 @@ from siml_amd.GPULingo import GPULingo, LingoDB
 
 pytestmark = pytest.mark.gpu
 
+# torch (caller-owned device arrays for the allocation-free entry point) is
+# imported before the library first touches the device, so that the process
+# loads one HIP runtime: torch's
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
 
 def same(got, want):
     gi, gt = got
@@ -184,3 +192,31 @@ def test_chunk_and_posting_width_boundaries(nq):
         h = g.getGlobalHistogram(0, 12)
         assert g._h.last_kernel() == "hist1000_sparse"
         assert np.array_equal(h, oracle.hist1000(tuple(x[:12] for x in ref), qry))
+
+
+def test_row_groups_built_on_first_use():
+    """A reference set whose calls take the sparse path builds no row
+    groups; the first call that needs them (the bit-sliced path under BITS)
+    builds them, with the same results; the allocation-free top-k before
+    that uses the merge kernel."""
+    if torch is None:
+        pytest.skip("torch is not importable")
+    ref = synth.packed_padded(200, 10, 120, seed=181)
+    qry = synth.packed_padded(40000, 10, 120, seed=182)
+    want = oracle.topk(ref, qry, 30, nthreads=0)
+    g = handle(_lib.POLICY_SPARSE, ref, qry)
+    same(g.getTopK(0, 200, 30), want)
+    assert g._h.last_kernel() == "topk_sparse"
+    assert g._h.row_groups() == (0, 0)
+    idx = torch.full((200, 30), -7, dtype=torch.int32, device="cuda:0")
+    t = torch.full((200, 30), 7.0, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    g._h.call("siml_topk_noalloc", 0, 200, 30, idx.data_ptr(), t.data_ptr())
+    g._h.call("siml_synchronize")
+    same((idx.cpu().numpy(), t.cpu().numpy()), want)
+    assert g._h.row_groups() == (0, 0)
+    g._h.call("siml_set_policy", _lib.POLICY_BITS)
+    same(g.getTopK(0, 200, 30), want)
+    assert g._h.last_kernel() == "topk_bits"
+    groups, merged = g._h.row_groups()
+    assert groups > 0 and merged == 0
