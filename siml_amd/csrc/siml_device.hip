@@ -2174,133 +2174,6 @@ __device__ uint32_t sparse_recover_isct(float t0, float maf, float mbf) {
     return 0u;
 }
 
-// Without the directory (the 1000-bin histogram): the row's levels found in
-// the index (Lingo level keys; a level t missing
-// from the index ends its Lingo's levels, since a molecule with level t + 1
-// has level t), each with its postings from column cstart on -- a binary
-// search, the postings being grouped by chunk in chunk order: per level the
-// next posting, the postings left and the column of the next posting
-// (INT32_MAX: none); *nlv counts the levels.  Every thread of the
-// workgroup calls, between barriers.
-template <bool NARROW>
-__device__ void sparse_row_levels_seq(const RowArgs& R, int64_t row, const PostingQuery& P, int64_t cstart,
-                                  int64_t* lv_cur, int32_t* lv_left, int32_t* lv_next, uint32_t* nlv) {
-    const int64_t o = R.off[row];
-    const int32_t len = R.len[row];
-    for (int e = threadIdx.x; e < len; e += blockDim.x) {
-        const int2 el = R.elems[o + e];
-        for (int t = 1; t <= el.y; ++t) {
-            int64_t first;
-            int32_t n;
-            if (!post_lookup(P, dev_level_key(el.x, (uint32_t)t), first, n)) break;
-            int32_t lo = 0, hi = n;
-            while (cstart > 0 && lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (post_col<NARROW>(P, first + mid) < cstart)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            const int at = (int)atomicAdd(nlv, 1u);
-            lv_cur[at] = first + lo;
-            lv_left[at] = n - lo;
-            lv_next[at] = lo < n ? (int32_t)post_col<NARROW>(P, first + lo) : INT32_MAX;
-        }
-    }
-}
-
-// Counts of the row's pairs with the chunk's columns [c0, cend) into the
-// byte counters cnt (cleared by the caller), from the levels' positions
-// left by the previous chunk.  Each wave walks whole levels, four at a
-// time: the chunk's postings of a level come first in what is left of it;
-// the first window of the four levels (one or two reads of 64, from the
-// level's postings per remaining chunk, `rest` chunks) is read at once, and
-// a level whose chunk postings fill it goes on alone, four reads at a time.
-// Every thread of the workgroup calls, between barriers.
-template <bool NARROW>
-__device__ void sparse_walk_chunk_seq(uint32_t* cnt, int64_t* lv_cur, int32_t* lv_left, int32_t* lv_next, int nl,
-                                  const PostingQuery& P, int64_t c0, int64_t cend, int32_t rest) {
-    constexpr int kWaves = kSparseBlock / kWave;
-    const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
-    const int32_t c0i = (int32_t)c0, cendi = (int32_t)cend;
-    auto count_col = [&](int32_t c) {
-        if (c >= c0i && c < cendi) {
-            const uint32_t cl = (uint32_t)(c - c0i);
-            atomicAdd(&cnt[cl >> 2], 1u << (8 * (cl & 3)));
-        }
-    };
-    for (int g = w; g < nl; g += 4 * kWaves) {
-        int32_t cv[4][2];
-        bool act[4];
-        int nbf[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int l = g + i * kWaves;
-            act[i] = l < nl && lv_next[l < nl ? l : 0] < cendi;
-            const int64_t cur = act[i] ? lv_cur[l] : 0;
-            const int32_t left = act[i] ? lv_left[l] : 0;
-            nbf[i] = left / rest < 40 ? 1 : 2;
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int q = u * kWave + lane;
-                cv[i][u] = (act[i] && u < nbf[i] && q < left) ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (!act[i]) continue;
-            const int l = g + i * kWaves;
-            int64_t cur = lv_cur[l];
-            int32_t left = lv_left[l];
-            int32_t next = INT32_MAX;
-            bool stop = false;
-            int took = 0;
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                if (u >= nbf[i] || stop) break;
-                count_col(cv[i][u]);
-                const int t = __popcll(__ballot(cv[i][u] < cendi));
-                took += t;
-                if (t < kWave) {
-                    stop = true;
-                    next = __shfl(cv[i][u], t & (kWave - 1));
-                }
-            }
-            cur += took;
-            left -= took;
-            if (!stop && left <= 0) stop = true;
-            while (!stop) {
-                int32_t c[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = u * kWave + lane;
-                    c[u] = q < left ? (int32_t)post_col<NARROW>(P, cur + q) : INT32_MAX;
-                }
-                took = 0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (stop) break;
-                    count_col(c[u]);
-                    const int t = __popcll(__ballot(c[u] < cendi));
-                    took += t;
-                    if (t < kWave) {
-                        stop = true;
-                        next = __shfl(c[u], t & (kWave - 1));
-                    }
-                }
-                cur += took;
-                left -= took;
-                if (!stop && left <= 0) stop = true;
-            }
-            if (lane == 0) {
-                lv_cur[l] = cur;
-                lv_left[l] = left;
-                lv_next[l] = left > 0 ? next : INT32_MAX;
-            }
-        }
-    }
-}
-
 // The row's levels found in the index (Lingo level keys; a level t missing
 // from the index ends its Lingo's levels, since a molecule with level t + 1
 // has level t): per level its first posting and its row of the chunk
@@ -2828,9 +2701,9 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
 // by (I, U) (their T is a function of the two), the rest into their bins.
 struct alignas(16) SparseHistLds {
     alignas(16) uint32_t cnt[kSparseCols / 4];
-    int64_t lv_cur[kSparseMaxLevels];
-    int32_t lv_left[kSparseMaxLevels];
-    int32_t lv_next[kSparseMaxLevels];
+    int64_t lv_first[kSparseMaxLevels];
+    int32_t lv_drow[kSparseMaxLevels];
+    SparseChunkLds W;
     uint32_t bins[1000];
     float edges[1001];
     uint32_t iu[EpiHist1000<1>::kIMax * EpiHist1000<1>::kIU];
@@ -2841,9 +2714,9 @@ struct alignas(16) SparseHistLds {
 // of at most 255 levels, magnitudes at least the count sums) against every
 // query molecule, from the inverted index: workgroup (x, s) takes row
 // rb + x and the chunks [s cps, (s + 1) cps), counts each chunk's pairs in
-// LDS (its waves walking the levels' postings from where the previous chunk
-// stopped: measured 8.0 ms on 10M x 1024 against 10.0 with the chunk
-// directory the top-k uses), then bins every pair of the chunk: a count
+// LDS as the sparse top-k does (10M x 1024: 7.4 ms; 8.0 with the waves
+// walking each level's postings from where the previous chunk stopped),
+// then bins every pair of the chunk: a count
 // of 0 is T = 0 (bin 0), any other pair gets T from the same float
 // expression as every kernel.  Counts by (I, U) as in the bit-sliced
 // histogram epilogue keep most LDS atomics off the few bins such pairs
@@ -2865,19 +2738,21 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
     stage_edges(L.edges);
     if (threadIdx.x == 0) L.nlv = 0u;
     __syncthreads();
-    sparse_row_levels_seq<NARROW>(R, row, P, y0 * kSparseCols, L.lv_cur, L.lv_left, L.lv_next, &L.nlv);
+    sparse_row_levels<NARROW>(R, row, P, L.lv_first, L.lv_drow, &L.nlv);
     __syncthreads();
     const int nl = (int)L.nlv;
     uint32_t zero = 0u;  // this thread's pairs with T < fl32(1/1000)
+    if (y0 < y1) sparse_chunk_store(L.W, L.lv_first, nl, sparse_chunk_load(L.lv_drow, P, nl, y0));
     for (int64_t y = y0; y < y1; ++y) {
         const int64_t c0 = y * kSparseCols;
         const int cw = (int)min((int64_t)kSparseCols, nq - c0);
         for (int i = threadIdx.x; i < kSparseCols / 16; i += blockDim.x)
             reinterpret_cast<uint4*>(L.cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
-        sparse_walk_chunk_seq<NARROW>(L.cnt, L.lv_cur, L.lv_left, L.lv_next, nl, P, c0, c0 + cw,
-                                      (int32_t)max((int64_t)1, y1 - y));
+        sparse_walk_chunk<NARROW>(L.cnt, L.W, nl, P, c0);
         __syncthreads();
+        // the next chunk's directory entries load while this one is binned
+        const SparseRange nxt = y + 1 < y1 ? sparse_chunk_load(L.lv_drow, P, nl, y + 1) : SparseRange{};
 #pragma unroll 1
         for (int i = 0; i < kSparseCols / 16 / kSparseBlock; ++i) {
             const int q = (int)threadIdx.x + i * kSparseBlock;
@@ -2922,6 +2797,7 @@ __global__ __launch_bounds__(kSparseBlock) __attribute__((amdgpu_waves_per_eu(6)
             }
         }
         __syncthreads();
+        if (y + 1 < y1) sparse_chunk_store(L.W, L.lv_first, nl, nxt);
     }
     // (I, U) counts to their bins: T = (float)I / (float)U, both exact
     // integers (short_div equals IEEE division there)
@@ -5055,7 +4931,7 @@ constexpr double kBitsPsPerPair = 0.72, kBitsPsPerPairLevel = 0.0112;
 // and scanning the counts, its per-chunk steps) plus 0.30 per posting met
 // (110 ms), the seeded bit-sliced passes 0.84 per pair plus 0.0112 per
 // pair and query level (347 ms); the histograms of 1024 rows over 10M take
-// 8.0 and 13.7 ms.
+// 7.4 and 13.7 ms.
 constexpr double kSparseTopkPsPerPair = 0.40, kBitsTopkPsPerPair = 0.84;
 
 // Rows of [rb, rl) for the sparse kernels (rows with ascending ASCII ids,
