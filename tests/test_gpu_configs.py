@@ -4,13 +4,15 @@ against the oracle and through size-independent properties:
 * batched uneven query sets: a 1M-molecule resident database (lognormal
   lengths, median 45, seed 3) and 2000 query sets of geometric(1/128)
   sizes clipped to [1, 4096]; top-100 per query molecule.  One batched
-  call equals the 2000 per-set calls on every row, and sampled rows equal
+  call (the sparse kernel) equals the same call through the bit-sliced
+  kernels and the 2000 per-set calls on every row, and sampled rows equal
   the oracle over the whole database.
 * threshold search + histogram: a 10M-molecule database (seed 4) and 1024
   near-duplicate queries; CSR lists for T >= 0.7 and the 1000-bin int64
   histogram of all 1.024e10 similarities.  The bins sum to the pair count,
-  the CSR is exactly the histogram's tail (bins 700..999: T >= fl32(0.7)),
-  every list ascends, and sampled rows equal the oracle.
+  the sparse histogram equals the bit-sliced one bin for bin, the CSR is
+  exactly the histogram's tail (bins 700..999: T >= fl32(0.7)), every list
+  ascends, and sampled rows equal the oracle.
 * serving: the same 10M database; Poisson arrivals of query batches
   (geometric(1/32) sizes clipped to [1, 512]), half top-50 and half
   threshold 0.8, continuously batched over two slots.  Every request's
@@ -28,7 +30,7 @@ import numpy as np
 import pytest
 
 import oracle
-from siml_amd import compiler, synth
+from siml_amd import _lib, compiler, synth
 from siml_amd.GPULingo import GPULingo, LingoDB
 from siml_amd.serving import Request, Server, run_poisson
 
@@ -114,6 +116,17 @@ def test_batched_1m_database_2000_sets_top100():
         g.set_refsmiles_ragged(*q)
         bi, bt = g.getTopK(0, nq, k)
         assert bi.shape == (nq, k)
+        assert g._h.last_kernel() == "topk_sparse"
+        say(f"batched top-{k} through the bit-sliced kernels")
+        gb = GPULingo(0)
+        gb._h.call("siml_set_policy", _lib.POLICY_BITS)
+        gb.attach_db(db)
+        gb.set_refsmiles_ragged(*q)
+        xi, xt = gb.getTopK(0, nq, k)
+        assert gb._h.last_kernel() == "topk_bits"
+        assert np.array_equal(xi, bi), f"{np.count_nonzero(xi != bi)} indices differ"
+        assert np.array_equal(bits(xt), bits(bt))
+        del gb, xi, xt
         say("per-set top-k: 2000 calls")
         for i in range(nsets):
             a, b = int(bounds[i]), int(bounds[i + 1])
@@ -167,6 +180,14 @@ def test_threshold_csr_and_histogram_10m(db10m):
     say("1000-bin histogram of 1024 x 10M pairs")
     h = g.getGlobalHistogram(0, NQ)
     assert h.dtype == np.int64 and int(h.sum()) == NQ * NDB
+    assert g._h.last_kernel() == "hist1000_sparse"
+    b = GPULingo(0)
+    b._h.call("siml_set_policy", _lib.POLICY_BITS)
+    b.attach_db(db)
+    b.set_refsmiles_ragged(*q)
+    assert np.array_equal(b.getGlobalHistogram(0, NQ), h)
+    assert b._h.last_kernel() == "hist1000_bits"
+    del b
     say("CSR neighbour lists, T >= 0.7")
     offs, idx = g.getNeighborsCSR(0, NQ, 0.7)
     assert int(offs[-1]) == int(h[700:].sum()) > 0
