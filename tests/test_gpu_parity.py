@@ -77,6 +77,32 @@ def test_multirow_all_pairs_32768_sampled(gpu):
     gpu._retrieve = None  # leave the 4 GiB result on the device
 
 
+def test_multirow_all_pairs_32768_full_both_kernels():
+    """The whole 32768 x 32768 matrix (1.07e9 pairs of the all-pairs
+    workload) equal to the oracle on every pair, through the sparse kernel
+    (AUTO's choice here) and through the bit-sliced kernel: the
+    reference's own acceptance check, exact equality over the full N^2
+    matrix (examples/benchmark.py:189-206), at this size.  Compared in
+    blocks of 4096 rows."""
+    s = synth.packed_padded(32768, 20, 120, seed=2)
+    hs = []
+    for policy, name in ((_lib.POLICY_AUTO, "rows_sparse"), (_lib.POLICY_BITS, "rows_bits")):
+        g = GPULingo(0)
+        g._h.call("siml_set_policy", policy)
+        _set(g, s, s)
+        g.getMultipleRows_async(0, 32768)
+        assert g._h.last_kernel() == name
+        hs.append(g)
+    got = np.empty((4096, 32768), np.float32)
+    for b0 in range(0, 32768, 4096):
+        want = oracle.tanimoto_block(tuple(x[b0:b0 + 4096] for x in s), s, nthreads=0)
+        for g in hs:
+            g._h.call("siml_peek_rows_f32", b0, b0 + 4096, _lib.f32p(got), 32768)
+            assert_same_f32(got, want)
+    for g in hs:
+        g._retrieve = None  # leave the 4 GiB results on the device
+
+
 def test_single_row_semantics(gpu, demo_lines):
     ref = list(oracle.pack(demo_lines[:50]))
     qry = list(oracle.pack(demo_lines[1000:3000]))
